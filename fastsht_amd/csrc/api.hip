@@ -1,0 +1,377 @@
+// C ABI of libfastsht (declared in include/fastsht.h).
+//
+// One handle = one (nside, lmax, nsim, niter, pol) transform configuration
+// with its resident tables and workspace: the state the reference keeps in
+// process-global Fortran module variables (src/sht_data_module.f90:10-109),
+// made per-instance.  Every entry point returns 0 on success or a negative
+// status; fsht_last_error() gives the message.
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include "../../include/fastsht.h"
+#include "fsht_internal.h"
+
+namespace fsht {
+hipError_t setup_tables(Handle* h);
+hipError_t launch_norm_alm(const double*, double*, const double*, const double*, double, int, int, hipStream_t);
+hipError_t launch_filt_alm(double*, const double*, int, int, hipStream_t);
+hipError_t launch_convert_healpy(const double*, double*, int, int, int, hipStream_t);
+void synth_offsets(Handle* h, int ncomp);
+}  // namespace fsht
+
+using fsht::Handle;
+
+struct fsht_handle {
+  Handle h;
+};
+
+static thread_local std::string g_err;
+
+static int fail(Handle* h, int code, const std::string& msg) {
+  if (h) h->err = msg;
+  g_err = msg;
+  return code;
+}
+
+#define HIPCHK(h, expr)                                                                              \
+  do {                                                                                               \
+    hipError_t _e = (expr);                                                                          \
+    if (_e != hipSuccess)                                                                            \
+      return fail((h), FSHT_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(_e));             \
+  } while (0)
+
+static void free_all(Handle* h) {
+  void* ptrs[] = {h->d_geo, h->d_i0, h->d_ab, h->d_spin, h->d_aboff, h->d_tw, h->d_slot, h->d_seedoff,
+                  h->d_seed_st, h->d_seed_v, h->d_goff, h->d_ana_items[0], h->d_ana_items[1],
+                  h->d_syn_items[0], h->d_syn_items[1], h->d_fft_jobs, h->d_blue, h->d_chirp, h->d_filt,
+                  h->d_twid, h->ws_spec, h->ws_specT, h->ws_specR, h->ws_G, h->ws_alm[0], h->ws_alm[1]};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+}
+
+static bool is_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
+
+extern "C" {
+
+const char* fsht_version(void) { return "fastsht_amd 0.1 (gfx950, FP64 MFMA)"; }
+
+const char* fsht_last_error(const fsht_handle* h) {
+  if (h && !h->h.err.empty()) return h->h.err.c_str();
+  return g_err.c_str();
+}
+
+int fsht_create(int nside, int lmax, int nsim, int niter, int pol, fsht_handle** out) {
+  if (!out) return fail(nullptr, FSHT_ERR_ARG, "out handle pointer is NULL");
+  *out = nullptr;
+  if (!is_pow2(nside) || nside > 1024)
+    return fail(nullptr, FSHT_ERR_ARG, "nside must be a power of two in [1, 1024]");
+  if (lmax < 0 || lmax > 4 * nside)
+    return fail(nullptr, FSHT_ERR_ARG, "lmax must be in [0, 4*nside]");
+  if (nsim < 1) return fail(nullptr, FSHT_ERR_ARG, "nsim must be >= 1");
+  if (niter < 0) return fail(nullptr, FSHT_ERR_ARG, "niter must be >= 0");
+  fsht_handle* H = new fsht_handle();
+  Handle* h = &H->h;
+  h->nside = nside;
+  h->lmax = lmax;
+  h->nsim = nsim;
+  h->niter = niter;
+  h->pol = pol ? 1 : 0;
+  h->nring = 4 * nside - 1;
+  h->nhr = 2 * nside - 1;
+  h->nnorth = 2 * nside;
+  h->npix = 12LL * nside * nside;
+  h->nsimp = (nsim + 15) / 16 * 16;
+  h->pa = 4.0 * 3.141592653589793238462643383279502884 / (double)h->npix;
+  hipError_t e = hipGetDevice(&h->device);
+  if (e == hipSuccess) e = hipStreamCreate(&h->own_stream);
+  h->stream = h->own_stream;
+  if (e == hipSuccess) e = fsht::setup_tables(h);
+  if (e == hipSuccess) e = fsht::build_bluestein(h);
+  if (e != hipSuccess) {
+    std::string msg = h->err.empty() ? std::string("setup: ") + hipGetErrorString(e) : h->err;
+    free_all(h);
+    if (h->own_stream) (void)hipStreamDestroy(h->own_stream);
+    delete H;
+    return fail(nullptr, e == hipErrorInvalidValue ? FSHT_ERR_ARG : FSHT_ERR_HIP, msg);
+  }
+  // workspace
+  const int ncomp = h->pol ? 2 : 1;
+  h->ncomp_ws = ncomp;
+  fsht::synth_offsets(h, 2);
+  const long long nspec = (long long)ncomp * nsim * h->npix;
+  const long long nspecT = h->npix * (long long)ncomp * h->nsimp;
+  const long long nG = h->G_size / (ncomp == 2 ? 1 : 2);
+  auto alloc = [&](double** p, long long n) -> hipError_t { return hipMalloc((void**)p, (size_t)n * sizeof(double)); };
+  e = alloc(&h->ws_spec, nspec);
+  if (e == hipSuccess) e = alloc(&h->ws_specT, nspecT);
+  if (e == hipSuccess && niter > 0) e = alloc(&h->ws_specR, nspecT);
+  if (e == hipSuccess) e = alloc(&h->ws_G, nG);
+  if (e != hipSuccess) {
+    free_all(h);
+    (void)hipStreamDestroy(h->own_stream);
+    delete H;
+    return fail(nullptr, FSHT_ERR_NOMEM, std::string("workspace allocation: ") + hipGetErrorString(e));
+  }
+  *out = H;
+  return FSHT_OK;
+}
+
+int fsht_destroy(fsht_handle* H) {
+  if (!H) return FSHT_OK;
+  Handle* h = &H->h;
+  (void)hipStreamSynchronize(h->stream);
+  free_all(h);
+  (void)hipStreamDestroy(h->own_stream);
+  delete H;
+  return FSHT_OK;
+}
+
+int fsht_synchronize(fsht_handle* H) {
+  if (!H) return fail(nullptr, FSHT_ERR_ARG, "NULL handle");
+  HIPCHK(&H->h, hipStreamSynchronize(H->h.stream));
+  return FSHT_OK;
+}
+
+int fsht_get_stream(fsht_handle* H, void** stream) {
+  if (!H || !stream) return fail(nullptr, FSHT_ERR_ARG, "NULL argument");
+  *stream = (void*)H->h.stream;
+  return FSHT_OK;
+}
+
+int fsht_set_stream(fsht_handle* H, void* stream) {
+  if (!H) return fail(nullptr, FSHT_ERR_ARG, "NULL handle");
+  Handle* h = &H->h;
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  h->stream = stream ? (hipStream_t)stream : h->own_stream;
+  return FSHT_OK;
+}
+
+int fsht_info(const fsht_handle* H, long long* out, int n) {
+  if (!H || !out) return fail(nullptr, FSHT_ERR_ARG, "NULL argument");
+  const Handle* h = &H->h;
+  long long v[8] = {h->nside, h->lmax, h->nsim, h->niter, h->pol, h->npix, h->nsimp, h->G_size};
+  for (int i = 0; i < n && i < 8; ++i) out[i] = v[i];
+  return FSHT_OK;
+}
+
+int fsht_ring_tab(const fsht_handle* H, int* out) {
+  if (!H || !out) return fail(nullptr, FSHT_ERR_ARG, "NULL argument");
+  std::memcpy(out, H->h.i0.data(), H->h.i0.size() * sizeof(int));
+  return FSHT_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Orchestration helpers
+// ---------------------------------------------------------------------------
+static long long alm_count(const Handle* h) { return (long long)h->nsim * (h->lmax + 1) * (h->lmax + 1); }
+
+static int ensure_dev_alm(Handle* h, int k) {
+  if (!h->ws_alm[k]) HIPCHK(h, hipMalloc((void**)&h->ws_alm[k], (size_t)alm_count(h) * sizeof(double)));
+  return FSHT_OK;
+}
+
+// forward ring FFT of ncomp map sets + transpose into ws_specT
+static int forward_spectra(Handle* h, const double* m0, const double* m1, int loc) {
+  const long long cnt = (long long)h->nsim * h->npix;
+  const double* in[2] = {m0, m1};
+  if (!loc) {  // stage host maps into ws_spec, transform in place
+    HIPCHK(h, hipMemcpyAsync(h->ws_spec, m0, cnt * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    if (h->spin)
+      HIPCHK(h, hipMemcpyAsync(h->ws_spec + cnt, m1, cnt * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    in[0] = h->ws_spec;
+    in[1] = h->ws_spec + cnt;
+  }
+  HIPCHK(h, fsht::launch_ring_fft(h, in, h->ws_spec, 0, nullptr));
+  HIPCHK(h, fsht::launch_transpose_to_T(h, h->ws_spec, h->ws_specT));
+  return FSHT_OK;
+}
+
+// spectrum in ws_specT -> maps
+static int inverse_spectra(Handle* h, double* m0, double* m1, int loc) {
+  const long long cnt = (long long)h->nsim * h->npix;
+  HIPCHK(h, fsht::launch_transpose_from_T(h, h->ws_specT, h->ws_spec));
+  double* out[2] = {m0, m1};
+  if (!loc) {
+    out[0] = h->ws_spec;
+    out[1] = h->ws_spec + cnt;
+  }
+  HIPCHK(h, fsht::launch_ring_fft(h, nullptr, h->ws_spec, 1, out));
+  if (!loc) {
+    HIPCHK(h, hipMemcpyAsync(m0, h->ws_spec, cnt * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    if (h->spin)
+      HIPCHK(h, hipMemcpyAsync(m1, h->ws_spec + cnt, cnt * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  }
+  return FSHT_OK;
+}
+
+static int check_common(fsht_handle* H, int loc) {
+  if (!H) return fail(nullptr, FSHT_ERR_ARG, "NULL handle");
+  if (loc != FSHT_HOST && loc != FSHT_DEVICE) return fail(&H->h, FSHT_ERR_ARG, "loc must be FSHT_HOST or FSHT_DEVICE");
+  H->h.err.clear();
+  return FSHT_OK;
+}
+
+// analysis with optional Jacobi refinement; alm outputs are device pointers
+static int analysis_iter(Handle* h, double* a0, double* a1, int niter, int want0, int want1) {
+  HIPCHK(h, fsht::launch_analysis(h, h->ws_specT, a0, a1, 0.0, want0, want1));
+  if (niter > 0 && !h->ws_specR) HIPCHK(h, hipMalloc((void**)&h->ws_specR, (size_t)h->npix * (h->spin ? 2 : 1) * h->nsimp * sizeof(double)));
+  for (int it = 0; it < niter; ++it) {
+    HIPCHK(h, fsht::launch_synthesis(h, a0, a1, h->ws_G));
+    HIPCHK(h, fsht::launch_fold(h, h->ws_G, h->ws_specT, h->ws_specR, 1));
+    HIPCHK(h, fsht::launch_analysis(h, h->ws_specR, a0, a1, 1.0, want0, want1));
+  }
+  return FSHT_OK;
+}
+
+extern "C" {
+
+int fsht_t2alm(fsht_handle* H, const double* maps, double* alms, int niter, int loc) {
+  int rc = check_common(H, loc);
+  if (rc) return rc;
+  Handle* h = &H->h;
+  if (!maps) return fail(h, FSHT_ERR_ARG, "maps is NULL");
+  if (niter < 0) return fail(h, FSHT_ERR_ARG, "niter must be >= 0");
+  h->spin = 0;
+  if ((rc = forward_spectra(h, maps, nullptr, loc))) return rc;
+  double* dst = alms;
+  const bool keep = (alms == nullptr);
+  if (keep || !loc) {
+    if ((rc = ensure_dev_alm(h, 0))) return rc;
+    dst = h->ws_alm[0];
+  }
+  if ((rc = analysis_iter(h, dst, nullptr, niter, 1, 0))) return rc;
+  if (!keep && !loc) {
+    HIPCHK(h, hipMemcpyAsync(alms, dst, alm_count(h) * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+  }
+  h->have_dev_alm = keep ? 1 : 0;
+  return FSHT_OK;
+}
+
+int fsht_get_alm(fsht_handle* H, double* alms, int loc) {
+  int rc = check_common(H, loc);
+  if (rc) return rc;
+  Handle* h = &H->h;
+  if (!h->have_dev_alm || !h->ws_alm[0]) return fail(h, FSHT_ERR_STATE, "no device-resident alm: call t2alm with alms=NULL first");
+  HIPCHK(h, hipMemcpyAsync(alms, h->ws_alm[0], alm_count(h) * sizeof(double),
+                           loc ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return FSHT_OK;
+}
+
+int fsht_alm2t(fsht_handle* H, const double* alms, double* maps, int loc) {
+  int rc = check_common(H, loc);
+  if (rc) return rc;
+  Handle* h = &H->h;
+  if (!alms || !maps) return fail(h, FSHT_ERR_ARG, "alms/maps is NULL");
+  h->spin = 0;
+  const double* a = alms;
+  if (!loc) {
+    if ((rc = ensure_dev_alm(h, 0))) return rc;
+    HIPCHK(h, hipMemcpyAsync(h->ws_alm[0], alms, alm_count(h) * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    a = h->ws_alm[0];
+    h->have_dev_alm = 0;
+  }
+  HIPCHK(h, fsht::launch_synthesis(h, a, nullptr, h->ws_G));
+  HIPCHK(h, fsht::launch_fold(h, h->ws_G, nullptr, h->ws_specT, 0));
+  if ((rc = inverse_spectra(h, maps, nullptr, loc))) return rc;
+  if (!loc) HIPCHK(h, hipStreamSynchronize(h->stream));
+  return FSHT_OK;
+}
+
+static int qu_forward(fsht_handle* H, const double* Q, const double* U, double* E, double* B, int niter, int loc,
+                      int want0, int want1) {
+  int rc = check_common(H, loc);
+  if (rc) return rc;
+  Handle* h = &H->h;
+  if (!h->pol) return fail(h, FSHT_ERR_STATE, "spin-2 transforms need a handle created with pol=1");
+  if (!Q || !U) return fail(h, FSHT_ERR_ARG, "Q/U is NULL");
+  if ((want0 && !E) || (want1 && !B)) return fail(h, FSHT_ERR_ARG, "output alm is NULL");
+  if (niter < 0) return fail(h, FSHT_ERR_ARG, "niter must be >= 0");
+  h->spin = 2;
+  if ((rc = forward_spectra(h, Q, U, loc))) return rc;
+  double* e = E;
+  double* b = B;
+  if (!loc) {
+    if ((rc = ensure_dev_alm(h, 0))) return rc;
+    if ((rc = ensure_dev_alm(h, 1))) return rc;
+    e = h->ws_alm[0];
+    b = h->ws_alm[1];
+    h->have_dev_alm = 0;
+  }
+  if (!want0) e = b;  // never written
+  if (!want1) b = e;
+  if ((rc = analysis_iter(h, e, b, niter, want0, want1))) return rc;
+  if (!loc) {
+    if (want0) HIPCHK(h, hipMemcpyAsync(E, e, alm_count(h) * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    if (want1) HIPCHK(h, hipMemcpyAsync(B, b, alm_count(h) * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+  }
+  return FSHT_OK;
+}
+
+int fsht_qu2eb(fsht_handle* H, const double* Q, const double* U, double* E, double* B, int niter, int loc) {
+  return qu_forward(H, Q, U, E, B, niter, loc, 1, 1);
+}
+int fsht_qu2e(fsht_handle* H, const double* Q, const double* U, double* E, int loc) {
+  return qu_forward(H, Q, U, E, nullptr, 0, loc, 1, 0);
+}
+int fsht_qu2b(fsht_handle* H, const double* Q, const double* U, double* B, int loc) {
+  return qu_forward(H, Q, U, nullptr, B, 0, loc, 0, 1);
+}
+
+int fsht_eb2qu(fsht_handle* H, const double* E, const double* B, double* Q, double* U, int loc) {
+  int rc = check_common(H, loc);
+  if (rc) return rc;
+  Handle* h = &H->h;
+  if (!h->pol) return fail(h, FSHT_ERR_STATE, "spin-2 transforms need a handle created with pol=1");
+  if (!E || !B || !Q || !U) return fail(h, FSHT_ERR_ARG, "NULL argument");
+  h->spin = 2;
+  const double* e = E;
+  const double* b = B;
+  if (!loc) {
+    if ((rc = ensure_dev_alm(h, 0))) return rc;
+    if ((rc = ensure_dev_alm(h, 1))) return rc;
+    HIPCHK(h, hipMemcpyAsync(h->ws_alm[0], E, alm_count(h) * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, hipMemcpyAsync(h->ws_alm[1], B, alm_count(h) * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    e = h->ws_alm[0];
+    b = h->ws_alm[1];
+    h->have_dev_alm = 0;
+  }
+  HIPCHK(h, fsht::launch_synthesis(h, e, b, h->ws_G));
+  HIPCHK(h, fsht::launch_fold(h, h->ws_G, nullptr, h->ws_specT, 0));
+  if ((rc = inverse_spectra(h, Q, U, loc))) return rc;
+  if (!loc) HIPCHK(h, hipStreamSynchronize(h->stream));
+  return FSHT_OK;
+}
+
+// ---- alm utilities (device pointers, caller's stream; NULL = default stream)
+int fsht_norm_alm(const double* alm_in, double* alm_out, const double* fac, const double* shift, double m0, int lmax,
+                  int nsim, void* stream) {
+  if (!alm_in || !alm_out || !fac || !shift || lmax < 0 || nsim < 1) return fail(nullptr, FSHT_ERR_ARG, "bad argument");
+  hipError_t e = fsht::launch_norm_alm(alm_in, alm_out, fac, shift, m0, lmax, nsim, (hipStream_t)stream);
+  return e == hipSuccess ? FSHT_OK : fail(nullptr, FSHT_ERR_HIP, hipGetErrorString(e));
+}
+
+int fsht_filt_alm(double* alm, const double* win, int lmax, int nsim, void* stream) {
+  if (!alm || !win || lmax < 0 || nsim < 1) return fail(nullptr, FSHT_ERR_ARG, "bad argument");
+  hipError_t e = fsht::launch_filt_alm(alm, win, lmax, nsim, (hipStream_t)stream);
+  return e == hipSuccess ? FSHT_OK : fail(nullptr, FSHT_ERR_HIP, hipGetErrorString(e));
+}
+
+int fsht_convert_alm_healpy(const double* alm, double* alm_hp, int lmax, int nsim, void* stream) {
+  if (!alm || !alm_hp || lmax < 0 || nsim < 1) return fail(nullptr, FSHT_ERR_ARG, "bad argument");
+  hipError_t e = fsht::launch_convert_healpy(alm, alm_hp, lmax, nsim, 0, (hipStream_t)stream);
+  return e == hipSuccess ? FSHT_OK : fail(nullptr, FSHT_ERR_HIP, hipGetErrorString(e));
+}
+
+int fsht_convert_alm_healpy_inv(const double* alm_hp, double* alm, int lmax, int nsim, void* stream) {
+  if (!alm || !alm_hp || lmax < 0 || nsim < 1) return fail(nullptr, FSHT_ERR_ARG, "bad argument");
+  hipError_t e = fsht::launch_convert_healpy(alm_hp, alm, lmax, nsim, 1, (hipStream_t)stream);
+  return e == hipSuccess ? FSHT_OK : fail(nullptr, FSHT_ERR_HIP, hipGetErrorString(e));
+}
+
+}  // extern "C"

@@ -1,0 +1,134 @@
+// Internal declarations shared by the fastsht_amd HIP sources (not part of the C ABI).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace fsht {
+
+// -------------------------------------------------------------------------
+// Geometry of one north ring i in [0, 2*nside) and its south mirror.
+// -------------------------------------------------------------------------
+struct RingGeo {
+  long long i1N;  // first pixel of north ring i
+  long long i1S;  // first pixel of its mirror nring-1-i (== i1N for the equator)
+  int nfft;       // ring length
+  int pad_;
+  double z;       // cos(theta)
+  double s;       // sin(theta)
+};
+
+// One ring-FFT job: a north ring and its mirror (or the equator ring alone).
+struct FftJob {
+  int ring;      // north ring index
+  int n;         // ring length
+  int M;         // transform length used in LDS (n if a power of two, else Bluestein size)
+  int blue;      // index into the Bluestein tables, -1 for direct power-of-two rings
+};
+
+struct BlueInfo {
+  long long chirp_off;  // offset of w_j = exp(-i pi j^2 / n), j < n
+  long long filt_off;   // offset of FFT_M(b)/M (M entries)
+};
+
+// Work item of the Legendre kernels.
+struct LegItem {
+  int m;
+  int par;     // parity / symmetry class (0: l+m even, 1: odd)
+  int blk;     // analysis: l-block; synthesis: ring block
+  int colblk;  // column block
+};
+
+// Tunables (also used by the host to size work lists).
+constexpr int kWG = 256;          // threads per Legendre workgroup
+constexpr int kAnaRings0 = 256;   // analysis ring sub-block (spin 0)
+constexpr int kAnaRings2 = 128;   // analysis ring sub-block (spin 2, x2 functions)
+constexpr int kAnaLB = 256;       // analysis l-block (l steps)
+constexpr int kAnaCols = 64;      // analysis columns per workgroup (4 waves x 16)
+constexpr int kSynRings = 256;    // synthesis rings per workgroup (4 waves x 64)
+constexpr int kSynCols = 64;      // synthesis columns per workgroup
+constexpr int kFlushExp = -899;   // seeds: a value p*2^e is kept iff frexp-exponent(p)+e >= this
+
+struct Handle;
+
+// Kernel launchers (k_*.hip). All enqueue on h->stream.
+hipError_t launch_seeds(Handle* h);
+hipError_t build_bluestein(Handle* h);
+hipError_t launch_ring_fft(Handle* h, const double* const* maps_comp, double* spec, int inverse,
+                           double* const* out_comp);
+hipError_t launch_transpose_to_T(Handle* h, const double* spec, double* specT);
+hipError_t launch_transpose_from_T(Handle* h, const double* specT, double* spec);
+hipError_t launch_analysis(Handle* h, const double* specT, double* alm0, double* alm1, double beta,
+                           int want0, int want1);
+hipError_t launch_synthesis(Handle* h, const double* alm0, const double* alm1, double* G);
+hipError_t launch_fold(Handle* h, const double* G, const double* specT_t, double* specT_out,
+                       int residual);
+
+// -------------------------------------------------------------------------
+// Handle: one transform configuration with its resident tables and workspace.
+// -------------------------------------------------------------------------
+struct Handle {
+  int nside = 0, lmax = 0, nsim = 0, niter = 0, pol = 0;
+  int nring = 0, nhr = 0, nnorth = 0;  // nnorth = 2*nside rings incl. equator
+  long long npix = 0;
+  int nsimp = 0;  // nsim rounded up to 16
+  double pa = 0;
+  hipStream_t stream = nullptr;      // stream all work is enqueued on
+  hipStream_t own_stream = nullptr;  // stream created (and destroyed) by the handle
+  int device = 0;
+  std::string err;
+
+  // host tables
+  std::vector<int> i0;          // ring_tab per m (0..lmax)
+  std::vector<long long> aboff; // offset of order m in the (a,b) table
+  std::vector<long long> seedoff;
+  std::vector<int> nblk;        // analysis l-blocks per m
+  std::vector<long long> goff;  // offset (in doubles) of order m in the m-major synthesis buffer
+
+  // device tables
+  RingGeo* d_geo = nullptr;
+  int* d_i0 = nullptr;
+  double2* d_ab = nullptr;      // (a_l, b_l) per (m, l)
+  double2* d_spin = nullptr;    // (lam_l, normal_l) per (m, l)
+  long long* d_aboff = nullptr;
+  double2* d_tw = nullptr;      // (cos m phi0, sin m phi0) per (m, north ring)
+  int* d_slot = nullptr;        // relative PERM slot*4 + class per (m, north ring)
+  long long* d_seedoff = nullptr;
+  int* d_seed_st = nullptr;
+  double2* d_seed_v = nullptr;
+  long long* d_goff = nullptr;
+  LegItem* d_ana_items[2] = {nullptr, nullptr};  // per spin (0, 2)
+  LegItem* d_syn_items[2] = {nullptr, nullptr};
+  int n_ana_items[2] = {0, 0}, n_syn_items[2] = {0, 0};
+
+  // FFT tables
+  std::vector<FftJob> fft_jobs;          // host copy, grouped by class
+  std::vector<int> fft_class_begin;      // start index of each LDS class in fft_jobs
+  std::vector<int> fft_class_M;          // max M of each class
+  FftJob* d_fft_jobs = nullptr;
+  BlueInfo* d_blue = nullptr;
+  double2* d_chirp = nullptr;
+  double2* d_filt = nullptr;
+  double2* d_twid = nullptr;  // exp(-2 pi i k / Mmax), k < Mmax/2
+  int Mmax = 0;
+  std::vector<BlueInfo> blue_host;
+  std::vector<int> blue_n, blue_M;
+
+  // workspace (device)
+  int ncomp_ws = 0;            // components the workspace is sized for (1 or 2)
+  double* ws_spec = nullptr;   // ncomp*nsim maps of npix (sim-major)
+  double* ws_specT = nullptr;  // npix rows of ncomp*nsimp (sim-fastest)
+  double* ws_specR = nullptr;  // same shape: residual / second spectrum
+  double* ws_G = nullptr;      // m-major synthesis output
+  long long G_size = 0;
+  double* ws_alm[2] = {nullptr, nullptr};  // device alm (nsim,(lmax+1)^2) for host-pointer calls
+  double* ws_map[2] = {nullptr, nullptr};  // device maps for host-pointer calls
+  int have_dev_alm = 0;  // result of the last forward transform kept on device
+
+  // current call's spin (0 or 2)
+  int spin = 0;
+};
+
+}  // namespace fsht

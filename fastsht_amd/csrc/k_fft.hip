@@ -1,0 +1,418 @@
+// Ring FFTs over HEALPix RING rings and the spectrum transposes.
+//
+// Replaces map2fft / fft2map (src/compute_fft.f90:15-280), which run MKL DFTI
+// on the host ring by ring: forward real FFT, unnormalised, PERM packing
+// (slot 0 = Re X_0, slot 1 = Re X_{n/2}, slots 2k,2k+1 = Re/Im X_k), inverse
+// unnormalised.  Here one workgroup transforms one north ring together with
+// its south mirror for one map (z = x_N + i x_S, one complex FFT, then the two
+// real spectra are separated).  Power-of-two rings use a radix-2 FFT in LDS;
+// other polar rings (length 4q with q not a power of two, often with a large
+// prime factor) use Bluestein's chirp-z convolution with a power-of-two FFT.
+#include <cmath>
+#include <thread>
+
+#include "fsht_internal.h"
+
+namespace fsht {
+
+static inline bool is_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
+static inline int next_pow2(int n) {
+  int p = 1;
+  while (p < n) p <<= 1;
+  return p;
+}
+__host__ __device__ static inline int ilog2(int n) {
+  int k = 0;
+  while ((1 << k) < n) ++k;
+  return k;
+}
+
+// ---------------------------------------------------------------------------
+// Host: radix-2 FFT used once per handle to transform the Bluestein filters.
+// ---------------------------------------------------------------------------
+static void host_fft(std::vector<double2>& a, int sign) {
+  const int n = (int)a.size();
+  const int lg = ilog2(n);
+  for (int i = 0; i < n; ++i) {
+    unsigned r = 0;
+    for (int b = 0; b < lg; ++b)
+      if (i & (1 << b)) r |= 1u << (lg - 1 - b);
+    if ((int)r > i) std::swap(a[i], a[r]);
+  }
+  for (int len = 2; len <= n; len <<= 1) {
+    const int half = len / 2;
+    for (int pos = 0; pos < half; ++pos) {
+      const long double ang = (long double)sign * 2.0L * 3.141592653589793238462643383279502884L * pos / len;
+      const double wr = (double)cosl(ang), wi = (double)sinl(ang);
+      for (int i = pos; i < n; i += len) {
+        const double2 u = a[i];
+        const double2 v0 = a[i + half];
+        const double2 v = make_double2(v0.x * wr - v0.y * wi, v0.x * wi + v0.y * wr);
+        a[i] = make_double2(u.x + v.x, u.y + v.y);
+        a[i + half] = make_double2(u.x - v.x, u.y - v.y);
+      }
+    }
+  }
+}
+
+// Job list (grouped by LDS size class), twiddles and Bluestein tables.
+hipError_t build_bluestein(Handle* h) {
+  const int nside = h->nside;
+  const int nn = h->nnorth;
+  struct J {
+    int ring, n, M;
+  };
+  std::vector<J> jobs;
+  for (int i = 0; i < nn; ++i) {
+    const int n = (i < nside) ? 4 * (i + 1) : 4 * nside;
+    const int M = is_pow2(n) ? n : next_pow2(2 * n - 1);
+    jobs.push_back({i, n, M});
+  }
+  // Bluestein tables per distinct non-power-of-two length
+  std::vector<int> blue_of_ring(nn, -1);
+  h->blue_n.clear();
+  h->blue_M.clear();
+  h->blue_host.clear();
+  long long coff = 0, foff = 0;
+  for (auto& j : jobs) {
+    if (is_pow2(j.n)) continue;
+    bool found = false;
+    for (size_t b = 0; b < h->blue_n.size(); ++b)
+      if (h->blue_n[b] == j.n) {
+        blue_of_ring[j.ring] = (int)b;
+        found = true;
+      }
+    if (found) continue;
+    blue_of_ring[j.ring] = (int)h->blue_n.size();
+    h->blue_n.push_back(j.n);
+    h->blue_M.push_back(j.M);
+    h->blue_host.push_back(BlueInfo{coff, foff});
+    coff += j.n;
+    foff += j.M;
+  }
+  const int nb = (int)h->blue_n.size();
+  std::vector<double2> chirp(std::max(1LL, coff)), filt(std::max(1LL, foff));
+  {
+    unsigned nth = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    std::vector<std::thread> th;
+    for (unsigned t = 0; t < nth; ++t) {
+      th.emplace_back([&, t]() {
+        for (int b = (int)t; b < nb; b += (int)nth) {
+          const int n = h->blue_n[b], M = h->blue_M[b];
+          const long long c0 = h->blue_host[b].chirp_off, f0 = h->blue_host[b].filt_off;
+          std::vector<double2> bx(M, make_double2(0.0, 0.0));
+          for (int j = 0; j < n; ++j) {
+            // w_j = exp(-i pi j^2/n), argument reduced exactly: j^2 mod 2n
+            const long long r = ((long long)j * j) % (2LL * n);
+            const long double ang = 3.141592653589793238462643383279502884L * (long double)r / (long double)n;
+            const double c = (double)cosl(ang), s = (double)sinl(ang);
+            chirp[c0 + j] = make_double2(c, -s);
+            bx[j] = make_double2(c, s);  // conj(w_j)
+            if (j) bx[M - j] = make_double2(c, s);
+          }
+          host_fft(bx, -1);
+          for (int k = 0; k < M; ++k) filt[f0 + k] = make_double2(bx[k].x / M, bx[k].y / M);
+        }
+      });
+    }
+    for (auto& x : th) x.join();
+  }
+  // group jobs by LDS class
+  const int classes[] = {256, 512, 1024, 2048, 4096, 8192};
+  h->fft_jobs.clear();
+  h->fft_class_begin.clear();
+  h->fft_class_M.clear();
+  int lastM = 0;
+  for (int ci = 0; ci < 6; ++ci) {
+    const int cM = classes[ci];
+    h->fft_class_begin.push_back((int)h->fft_jobs.size());
+    h->fft_class_M.push_back(cM);
+    for (auto& j : jobs)
+      if (j.M <= cM && j.M > lastM) h->fft_jobs.push_back(FftJob{j.ring, j.n, j.M, blue_of_ring[j.ring]});
+    lastM = cM;
+  }
+  h->fft_class_begin.push_back((int)h->fft_jobs.size());
+  int Mmax = 2;
+  for (auto& j : jobs) Mmax = std::max(Mmax, j.M);
+  if (Mmax > 8192) {
+    h->err = "ring length needs an LDS transform larger than 8192 points (nside > 1024 not supported)";
+    return hipErrorInvalidValue;
+  }
+  h->Mmax = Mmax;
+  std::vector<double2> tw(Mmax / 2);
+  for (int k = 0; k < Mmax / 2; ++k) {
+    const long double ang = -2.0L * 3.141592653589793238462643383279502884L * k / Mmax;
+    tw[k] = make_double2((double)cosl(ang), (double)sinl(ang));
+  }
+  hipError_t e;
+  auto up = [&](auto** d, const auto& v) -> hipError_t {
+    using T = typename std::remove_reference<decltype(v[0])>::type;
+    hipError_t ee = hipMalloc((void**)d, std::max<size_t>(1, v.size()) * sizeof(T));
+    if (ee != hipSuccess) return ee;
+    if (v.empty()) return hipSuccess;
+    return hipMemcpy(*d, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice);
+  };
+  if ((e = up(&h->d_fft_jobs, h->fft_jobs)) != hipSuccess) return e;
+  if ((e = up(&h->d_blue, h->blue_host)) != hipSuccess) return e;
+  if ((e = up(&h->d_chirp, chirp)) != hipSuccess) return e;
+  if ((e = up(&h->d_filt, filt)) != hipSuccess) return e;
+  if ((e = up(&h->d_twid, tw)) != hipSuccess) return e;
+  return hipSuccess;
+}
+
+// ---------------------------------------------------------------------------
+// Device FFT helpers (one workgroup, data in LDS).
+// ---------------------------------------------------------------------------
+__device__ inline double2 cmul(double2 a, double2 b) {
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ inline double2 cconj(double2 a) { return make_double2(a.x, -a.y); }
+__device__ inline double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ inline double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+
+// Decimation in frequency, forward (e^{-2 pi i jk/M}): natural in, bit-reversed out.
+__device__ void lds_dif(double2* buf, int M, int Mmax, const double2* __restrict__ tw) {
+  for (int len = M; len >= 2; len >>= 1) {
+    const int half = len >> 1;
+    const int stride = Mmax / len;
+    for (int t = threadIdx.x; t < (M >> 1); t += blockDim.x) {
+      const int pos = t & (half - 1);
+      const int i = (t - pos) * 2 + pos;
+      const double2 u = buf[i], v = buf[i + half];
+      buf[i] = cadd(u, v);
+      buf[i + half] = cmul(csub(u, v), tw[pos * stride]);
+    }
+    __syncthreads();
+  }
+}
+
+// Decimation in time, inverse (e^{+2 pi i jk/M}, unnormalised): bit-reversed in, natural out.
+__device__ void lds_dit_inv(double2* buf, int M, int Mmax, const double2* __restrict__ tw) {
+  for (int len = 2; len <= M; len <<= 1) {
+    const int half = len >> 1;
+    const int stride = Mmax / len;
+    for (int t = threadIdx.x; t < (M >> 1); t += blockDim.x) {
+      const int pos = t & (half - 1);
+      const int i = (t - pos) * 2 + pos;
+      const double2 u = buf[i];
+      const double2 v = cmul(buf[i + half], cconj(tw[pos * stride]));
+      buf[i] = cadd(u, v);
+      buf[i + half] = csub(u, v);
+    }
+    __syncthreads();
+  }
+}
+
+__device__ inline int brev(int x, int lg) { return (int)(__brev((unsigned)x) >> (32 - lg)); }
+
+// Forward: maps (sim-major, npix per map) -> spec (same layout, PERM per ring).
+// Inverse: spec -> maps.  grid.x = job within the class, grid.y = map index.
+template <int MAXM>
+__global__ __launch_bounds__(256) void k_ring_fft(const FftJob* __restrict__ jobs, int job0,
+                                                  const RingGeo* __restrict__ geo, const BlueInfo* __restrict__ blue,
+                                                  const double2* __restrict__ chirp, const double2* __restrict__ filt,
+                                                  const double2* __restrict__ tw, int Mmax, long long npix, int nsim,
+                                                  const double* __restrict__ in0, const double* __restrict__ in1,
+                                                  double* __restrict__ out0, double* __restrict__ out1, int inverse) {
+  __shared__ double2 buf[MAXM];
+  const FftJob jb = jobs[job0 + blockIdx.x];
+  const int mapi = blockIdx.y;
+  const int comp = mapi / nsim, k = mapi - comp * nsim;
+  const double* in = (comp ? in1 : in0) + (long long)k * npix;
+  double* out = (comp ? out1 : out0) + (long long)k * npix;
+  const RingGeo g = geo[jb.ring];
+  const bool pair = g.i1S != g.i1N;
+  const int n = jb.n, M = jb.M, lg = ilog2(M);
+  const bool isblue = jb.blue >= 0;
+  const double2* w = isblue ? chirp + blue[jb.blue].chirp_off : nullptr;
+  const double2* fl = isblue ? filt + blue[jb.blue].filt_off : nullptr;
+
+  if (!inverse) {
+    // ---- load z = x_N + i x_S
+    for (int j = threadIdx.x; j < M; j += blockDim.x) {
+      double2 z = make_double2(0.0, 0.0);
+      if (j < n) {
+        z.x = in[g.i1N + j];
+        z.y = pair ? in[g.i1S + j] : 0.0;
+        if (isblue) z = cmul(z, w[j]);
+      }
+      buf[j] = z;
+    }
+    __syncthreads();
+    lds_dif(buf, M, Mmax, tw);
+    if (isblue) {
+      for (int r = threadIdx.x; r < M; r += blockDim.x) buf[r] = cmul(buf[r], fl[brev(r, lg)]);
+      __syncthreads();
+      lds_dit_inv(buf, M, Mmax, tw);
+    }
+    // ---- separate the two real spectra, write PERM
+    for (int kk = threadIdx.x; kk <= n / 2; kk += blockDim.x) {
+      const int nk = (n - kk) % n;
+      double2 Zk, Zn;
+      if (isblue) {
+        Zk = cmul(buf[kk], w[kk]);
+        Zn = cmul(buf[nk], w[nk]);
+      } else {
+        Zk = buf[brev(kk, lg)];
+        Zn = buf[brev(nk, lg)];
+      }
+      const double2 XN = make_double2(0.5 * (Zk.x + Zn.x), 0.5 * (Zk.y - Zn.y));
+      const double2 XS = make_double2(0.5 * (Zk.y + Zn.y), -0.5 * (Zk.x - Zn.x));
+      if (kk == 0) {
+        out[g.i1N] = XN.x;
+        if (pair) out[g.i1S] = XS.x;
+      } else if (kk == n / 2) {
+        out[g.i1N + 1] = XN.x;
+        if (pair) out[g.i1S + 1] = XS.x;
+      } else {
+        out[g.i1N + 2 * kk] = XN.x;
+        out[g.i1N + 2 * kk + 1] = XN.y;
+        if (pair) {
+          out[g.i1S + 2 * kk] = XS.x;
+          out[g.i1S + 2 * kk + 1] = XS.y;
+        }
+      }
+    }
+  } else {
+    // ---- build Z_k = X_N[k] + i X_S[k], k in [0, n)
+    for (int kk = threadIdx.x; kk < M; kk += blockDim.x) {
+      double2 Z = make_double2(0.0, 0.0);
+      if (kk < n) {
+        const int kh = kk <= n / 2 ? kk : n - kk;
+        const bool cj = kk > n / 2;
+        double2 XN, XS = make_double2(0.0, 0.0);
+        if (kh == 0) {
+          XN = make_double2(in[g.i1N], 0.0);
+          if (pair) XS = make_double2(in[g.i1S], 0.0);
+        } else if (kh == n / 2) {
+          XN = make_double2(in[g.i1N + 1], 0.0);
+          if (pair) XS = make_double2(in[g.i1S + 1], 0.0);
+        } else {
+          XN = make_double2(in[g.i1N + 2 * kh], in[g.i1N + 2 * kh + 1]);
+          if (pair) XS = make_double2(in[g.i1S + 2 * kh], in[g.i1S + 2 * kh + 1]);
+        }
+        if (cj) {
+          XN.y = -XN.y;
+          XS.y = -XS.y;
+        }
+        Z = make_double2(XN.x - XS.y, XN.y + XS.x);
+        if (isblue) Z = cmul(cconj(Z), w[kk]);
+      }
+      if (isblue)
+        buf[kk] = Z;
+      else if (kk < M)
+        buf[brev(kk, lg)] = Z;
+    }
+    __syncthreads();
+    if (isblue) {
+      lds_dif(buf, M, Mmax, tw);
+      for (int r = threadIdx.x; r < M; r += blockDim.x) buf[r] = cmul(buf[r], fl[brev(r, lg)]);
+      __syncthreads();
+    }
+    lds_dit_inv(buf, M, Mmax, tw);
+    for (int j = threadIdx.x; j < n; j += blockDim.x) {
+      double2 z = buf[j];
+      if (isblue) z = cconj(cmul(z, w[j]));
+      out[g.i1N + j] = z.x;
+      if (pair) out[g.i1S + j] = z.y;
+    }
+  }
+}
+
+hipError_t launch_ring_fft(Handle* h, const double* const* in_comp, double* spec, int inverse,
+                           double* const* out_comp) {
+  // forward: in_comp = user maps per component, out = spec (+ comp offset)
+  // inverse: in = spec, out_comp = user maps per component
+  const int ncomp = h->spin ? 2 : 1;
+  const long long cstride = (long long)h->nsim * h->npix;
+  const double* in0 = inverse ? spec : in_comp[0];
+  const double* in1 = inverse ? spec + cstride : (ncomp > 1 ? in_comp[1] : in_comp[0]);
+  double* out0 = inverse ? out_comp[0] : spec;
+  double* out1 = inverse ? (ncomp > 1 ? out_comp[1] : out_comp[0]) : spec + cstride;
+  const int nmaps = ncomp * h->nsim;
+  for (size_t c = 0; c + 1 < h->fft_class_begin.size(); ++c) {
+    const int b = h->fft_class_begin[c], e = h->fft_class_begin[c + 1];
+    if (e <= b) continue;
+    dim3 grid(e - b, nmaps);
+    const int M = h->fft_class_M[c];
+#define FSHT_LAUNCH_FFT(MM)                                                                                      \
+  hipLaunchKernelGGL(k_ring_fft<MM>, grid, dim3(256), 0, h->stream, h->d_fft_jobs, b, h->d_geo, h->d_blue,      \
+                     h->d_chirp, h->d_filt, h->d_twid, h->Mmax, h->npix, h->nsim, in0, in1, out0, out1, inverse)
+    switch (M) {
+      case 256: FSHT_LAUNCH_FFT(256); break;
+      case 512: FSHT_LAUNCH_FFT(512); break;
+      case 1024: FSHT_LAUNCH_FFT(1024); break;
+      case 2048: FSHT_LAUNCH_FFT(2048); break;
+      case 4096: FSHT_LAUNCH_FFT(4096); break;
+      default: FSHT_LAUNCH_FFT(8192); break;
+    }
+#undef FSHT_LAUNCH_FFT
+    hipError_t err = hipGetLastError();
+    if (err != hipSuccess) return err;
+  }
+  return hipSuccess;
+}
+
+// ---------------------------------------------------------------------------
+// Transposes between spec (map-major: [map][pix]) and specT (pixel-major,
+// sims fastest: [pix][comp*nsimp + sim]).  64x64 tiles through LDS.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_to_T(const double* __restrict__ spec, double* __restrict__ specT,
+                                              long long npix, int nsim, int nsimp, int ncomp) {
+  __shared__ double tile[64][65];
+  const long long p0 = (long long)blockIdx.x * 64;
+  const int c0 = blockIdx.y * 64;  // column in [0, ncomp*nsimp)
+  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+  for (int r = ty; r < 64; r += 4) {
+    const int col = c0 + r;
+    const int comp = col / nsimp, sim = col - comp * nsimp;
+    const long long p = p0 + tx;
+    double v = 0.0;
+    if (comp < ncomp && sim < nsim && p < npix) v = spec[((long long)comp * nsim + sim) * npix + p];
+    tile[r][tx] = v;
+  }
+  __syncthreads();
+  const int ld = ncomp * nsimp;
+  for (int r = ty; r < 64; r += 4) {
+    const long long p = p0 + r;
+    const int col = c0 + tx;
+    if (p < npix && col < ld) specT[p * ld + col] = tile[tx][r];
+  }
+}
+
+__global__ __launch_bounds__(256) void k_from_T(const double* __restrict__ specT, double* __restrict__ spec,
+                                                long long npix, int nsim, int nsimp, int ncomp) {
+  __shared__ double tile[64][65];
+  const long long p0 = (long long)blockIdx.x * 64;
+  const int c0 = blockIdx.y * 64;
+  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+  const int ld = ncomp * nsimp;
+  for (int r = ty; r < 64; r += 4) {
+    const long long p = p0 + r;
+    const int col = c0 + tx;
+    tile[r][tx] = (p < npix && col < ld) ? specT[p * ld + col] : 0.0;
+  }
+  __syncthreads();
+  for (int r = ty; r < 64; r += 4) {
+    const int col = c0 + r;
+    const int comp = col / nsimp, sim = col - comp * nsimp;
+    const long long p = p0 + tx;
+    if (comp < ncomp && sim < nsim && p < npix) spec[((long long)comp * nsim + sim) * npix + p] = tile[tx][r];
+  }
+}
+
+hipError_t launch_transpose_to_T(Handle* h, const double* spec, double* specT) {
+  const int ncomp = h->spin ? 2 : 1;
+  dim3 grid((unsigned)((h->npix + 63) / 64), (ncomp * h->nsimp + 63) / 64);
+  hipLaunchKernelGGL(k_to_T, grid, dim3(256), 0, h->stream, spec, specT, h->npix, h->nsim, h->nsimp, ncomp);
+  return hipGetLastError();
+}
+
+hipError_t launch_transpose_from_T(Handle* h, const double* specT, double* spec) {
+  const int ncomp = h->spin ? 2 : 1;
+  dim3 grid((unsigned)((h->npix + 63) / 64), (ncomp * h->nsimp + 63) / 64);
+  hipLaunchKernelGGL(k_from_T, grid, dim3(256), 0, h->stream, specT, spec, h->npix, h->nsim, h->nsimp, ncomp);
+  return hipGetLastError();
+}
+
+}  // namespace fsht

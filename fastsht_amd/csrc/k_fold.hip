@@ -1,0 +1,127 @@
+// m-major synthesis output -> ring spectra (PERM packing), with aliasing.
+//
+// Replaces mat2fft_fast and mat2fft_fast_iter (src/fft_mapping.f90:84-190)
+// plus the zeroing of unused PERM slots (src/sht_main.f90:137-146).  The
+// reference scatters one m at a time and relies on ascending m with
+// set-then-accumulate to fold the orders that alias onto the same ring
+// frequency (rings shorter than 2(lmax+1)).  Here every output slot gathers
+// its aliases explicitly: slot k of a ring of length n receives the orders
+// m = k, n-k, n+k, 2n-k, ... (m <= lmax, ring kept for m), rotated by
+// e^{+i m phi0}, conjugated for negative frequencies and doubled on the DC and
+// Nyquist slots for m > 0 (src/sht_data_init.f90:542-573).
+//
+// residual = 0: out = X            (alm2t / eb2qu: spectrum of the synthesis)
+// residual = 1: out = Xt - n * X   (Jacobi residual in the ring-Fourier domain,
+//                                   src/fft_mapping.f90:177-183)
+#include "fsht_internal.h"
+
+namespace fsht {
+
+template <int NCOMP>
+__global__ __launch_bounds__(256) void k_fold(const RingGeo* __restrict__ geo, const int* __restrict__ i0tab,
+                                              const double2* __restrict__ tw, const double* __restrict__ G,
+                                              const long long* __restrict__ goff, const double* __restrict__ Xt,
+                                              double* __restrict__ out, int residual, int lmax, int nhr,
+                                              int nnorth, int nsimp) {
+  const int i = blockIdx.x;  // north ring
+  const int kk = blockIdx.y * 4 + threadIdx.y;
+  const int col = blockIdx.z * 64 + threadIdx.x;  // spectrum column: comp*nsimp + sim
+  const int ldT = NCOMP * nsimp;
+  if (col >= ldT) return;
+  const RingGeo g = geo[i];
+  const int n = g.nfft;
+  if (kk > n / 2) return;
+  const bool pair = g.i1S != g.i1N;
+  const int comp = col / nsimp, sim = col - comp * nsimp;
+  const int ncols = 2 * NCOMP * nsimp;
+  const int cre = (comp * 2) * nsimp + sim, cim = (comp * 2 + 1) * nsimp + sim;
+  const bool edge = (kk == 0) || (kk == n / 2);
+
+  double XNr = 0.0, XNi = 0.0, XSr = 0.0, XSi = 0.0;
+  // aliases in ascending m: k, n-k, n+k, 2n-k, ...  (edge slots: k, k+n, ...)
+  for (int base = 0;; base += n) {
+    bool stop = false;
+    for (int side = 0; side < 2; ++side) {
+      int m;
+      if (side == 0) {
+        m = base + kk;
+      } else {
+        if (edge) break;
+        m = base + n - kk;
+      }
+      if (m > lmax) {
+        stop = true;
+        break;
+      }
+      const int i0 = i0tab[m];
+      if (i < i0) {  // ring not kept for this (and, i0 being monotone, any larger) m
+        stop = true;
+        break;
+      }
+      const int nr = nhr + 1 - i0;
+      const long long gb = goff[m] / (NCOMP == 2 ? 1 : 2) + (long long)(i - i0) * ncols;
+      const long long go = (long long)nr * ncols;  // offset of the odd class
+      const double er = G[gb + cre], ei = G[gb + cim];
+      const double orr = G[gb + go + cre], oi = G[gb + go + cim];
+      const double2 cs = tw[(long long)m * nnorth + i];
+      // north = even + odd, south = even - odd; rotate by e^{+i m phi0}
+      const double nr_ = er + orr, ni_ = ei + oi;
+      const double WNr = nr_ * cs.x - ni_ * cs.y, WNi = nr_ * cs.y + ni_ * cs.x;
+      const double sr_ = er - orr, si_ = ei - oi;
+      const double WSr = sr_ * cs.x - si_ * cs.y, WSi = sr_ * cs.y + si_ * cs.x;
+      if (edge) {
+        const double f = (m == 0) ? 1.0 : 2.0;
+        XNr += f * WNr;
+        XSr += f * WSr;
+      } else if (side == 0) {
+        XNr += WNr;
+        XNi += WNi;
+        XSr += WSr;
+        XSi += WSi;
+      } else {
+        XNr += WNr;
+        XNi -= WNi;
+        XSr += WSr;
+        XSi -= WSi;
+      }
+    }
+    if (stop) break;
+  }
+  // write PERM slots
+  int s0, s1 = -1;
+  if (kk == 0)
+    s0 = 0;
+  else if (kk == n / 2)
+    s0 = 1;
+  else {
+    s0 = 2 * kk;
+    s1 = 2 * kk + 1;
+  }
+  const double nf = (double)n;
+  auto put = [&](long long pix, double v) {
+    const long long a = pix * ldT + col;
+    out[a] = residual ? Xt[a] - nf * v : v;
+  };
+  put(g.i1N + s0, XNr);
+  if (s1 >= 0) put(g.i1N + s1, XNi);
+  if (pair) {
+    put(g.i1S + s0, XSr);
+    if (s1 >= 0) put(g.i1S + s1, XSi);
+  }
+}
+
+hipError_t launch_fold(Handle* h, const double* G, const double* specT_t, double* specT_out, int residual) {
+  const int ncomp = h->spin ? 2 : 1;
+  const int maxk = 2 * h->nside;  // n/2 for the longest ring
+  dim3 block(64, 4);
+  dim3 grid(h->nnorth, (maxk + 1 + 3) / 4, (ncomp * h->nsimp + 63) / 64);
+  if (ncomp == 2)
+    hipLaunchKernelGGL(k_fold<2>, grid, block, 0, h->stream, h->d_geo, h->d_i0, h->d_tw, G, h->d_goff, specT_t,
+                       specT_out, residual, h->lmax, h->nhr, h->nnorth, h->nsimp);
+  else
+    hipLaunchKernelGGL(k_fold<1>, grid, block, 0, h->stream, h->d_geo, h->d_i0, h->d_tw, G, h->d_goff, specT_t,
+                       specT_out, residual, h->lmax, h->nhr, h->nnorth, h->nsimp);
+  return hipGetLastError();
+}
+
+}  // namespace fsht

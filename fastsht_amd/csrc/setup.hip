@@ -1,0 +1,480 @@
+// Handle set-up: ring geometry, ring-skip table, Legendre recurrence constants,
+// gather/twiddle tables, Legendre start seeds (device kernel), work lists.
+//
+// Replaces the reference's sht_data_alloc / sht_set_data chain
+// (src/sht_data_init.f90:2-311, 383-686) and its plm seed tables
+// (plm_table/*.bin read at SHT.py:34-45): everything here is computed from the
+// HEALPix closed form and the normalised-Legendre recurrence.
+#include <algorithm>
+#include <cmath>
+#include <thread>
+
+#include "fsht_internal.h"
+
+namespace fsht {
+
+static constexpr double kPi = 3.141592653589793238462643383279502884;
+
+// ---------------------------------------------------------------------------
+// HEALPix RING geometry (Gorski et al. 2005).  Ring r = i+1 counted from the
+// north pole.  Pixel offsets/lengths as in src/sht_data_init.f90:655-675.
+// ---------------------------------------------------------------------------
+void ring_geometry(int nside, std::vector<RingGeo>& geo, std::vector<double>& phi0) {
+  const int nring = 4 * nside - 1;
+  const int nn = 2 * nside;
+  geo.resize(nn);
+  phi0.resize(nn);
+  auto i1_of = [&](long long i) -> long long {
+    const long long ns = nside;
+    if (i < ns) return 2 * i * (i + 1);
+    if (i < 3 * ns - 1) return 2 * ns + 4 * ns * i - 2 * ns * ns;
+    return 12 * ns * ns - 2 * (4 * ns - i - 1) * (4 * ns - i);
+  };
+  for (int i = 0; i < nn; ++i) {
+    const int r = i + 1;
+    double z, s;
+    if (r < nside) {
+      const double t = ((double)r / nside) * ((double)r / nside) / 3.0;
+      z = 1.0 - t;
+      s = std::sqrt(t * (2.0 - t));
+      phi0[i] = kPi / (4.0 * r);
+    } else {
+      z = (4.0 * nside - 2.0 * r) / (3.0 * nside);
+      s = std::sqrt((1.0 - z) * (1.0 + z));
+      phi0[i] = ((r - nside) % 2 == 0) ? kPi / (4.0 * nside) : 0.0;
+    }
+    RingGeo g;
+    g.i1N = i1_of(i);
+    g.i1S = i1_of(nring - 1 - i);
+    g.nfft = (i < nside) ? 4 * (i + 1) : 4 * nside;
+    g.pad_ = 0;
+    g.z = z;
+    g.s = s;
+    geo[i] = g;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Recurrence constants (src/get_plm_recursive.f90:31-37, sht_data_init.f90:401-416).
+// ---------------------------------------------------------------------------
+static void diag_fac(int mmax, std::vector<double>& out) {
+  out.resize(mmax + 1);
+  out[0] = 1.0;
+  for (int m = 1; m <= mmax; ++m) out[m] = out[m - 1] * (2.0 * m + 1.0) / 2.0 / (double)m;
+}
+
+static void ab_coeffs(int m, int lmax, double* a, double* b) {
+  const double mm = (double)m * (double)m;
+  a[0] = 0.0;
+  b[0] = -1.0;
+  for (int l = m + 1; l <= lmax; ++l) {
+    const double dl = (double)l;
+    const double c1 = (dl - 1.0) * (dl - 1.0);
+    const double c2 = (2.0 * dl - 3.0) * (2.0 * dl - 1.0);
+    const double c3 = (2.0 * dl - 1.0) * (2.0 * dl + 1.0);
+    const double c4 = dl * dl;
+    const double al = std::sqrt(c3 / (c4 - mm));
+    a[l - m] = al;
+    b[l - m] = std::sqrt((c1 - mm) / c2) * al;
+  }
+}
+
+// x^n as mantissa in [0.5,1) and binary exponent, by binary powering.
+static void pow_scaled_host(double x, int n, double* mant, long* ex) {
+  int ex0;
+  double base = std::frexp(x, &ex0);
+  long be = ex0;
+  double r = 1.0;
+  long re = 0;
+  while (n > 0) {
+    if (n & 1) {
+      int e;
+      r = std::frexp(r * base, &e);
+      re += e + be;
+    }
+    n >>= 1;
+    if (n) {
+      int e;
+      base = std::frexp(base * base, &e);
+      be = 2 * be + e;
+    }
+  }
+  *mant = r;
+  *ex = re;
+}
+
+// max_{m<=l<=lcap} log2|P_lm| for one ring, scaled recurrence.
+static double ring_maxlog2(int m, int lcap, double z, double s, const double* dfac, const double* a,
+                           const double* b) {
+  double ms;
+  long e;
+  pow_scaled_host(s, m, &ms, &e);
+  double p1 = std::sqrt(dfac[m] / 4.0 / kPi) * ms;
+  if (m & 1) p1 = -p1;
+  int fe;
+  p1 = std::frexp(p1, &fe);
+  e += fe;
+  double p2 = 0.0;
+  double best = -1e300;
+  for (int l = m; l <= lcap; ++l) {
+    double p;
+    if (l == m) {
+      p = p1;
+    } else {
+      p = std::fma(a[l - m] * z, p1, -(b[l - m] * p2));
+      p2 = p1;
+      p1 = p;
+    }
+    if (e < 0 && std::fabs(p1) > 0x1p100) {
+      long sh = (-e < 100) ? -e : 100;
+      p1 = std::ldexp(p1, (int)-sh);
+      p2 = std::ldexp(p2, (int)-sh);
+      e += sh;
+      p = p1;
+    }
+    if (p != 0.0) {
+      double lg = std::log2(std::fabs(p)) + (double)e;
+      if (lg > best) best = lg;
+    }
+  }
+  return best;
+}
+
+// Ring-skip table: first north ring whose max_{m<=l<=lcap}|P_lm| reaches
+// 1e-20, lcap = 4*nside (nside <= 1024) or 6143 (nside 2048); these are the
+// rules that reproduce SHT_ring_skip_table/*.bin.  Bisection over rings (the
+// maximum grows monotonically toward the equator).
+void ring_tab(int nside, int nm, const std::vector<RingGeo>& geo, std::vector<int>& out) {
+  const int lcap = (nside <= 1024) ? 4 * nside : 6143;
+  const double thr2 = -20.0 / 0.30102999566398119521;
+  std::vector<double> dfac;
+  diag_fac(std::max(nm, 1), dfac);
+  out.assign(nm, 0);
+  const int nn = 2 * nside;
+  unsigned nth = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  std::vector<std::thread> th;
+  for (unsigned t = 0; t < nth; ++t) {
+    th.emplace_back([&, t]() {
+      std::vector<double> a(lcap + 2), b(lcap + 2);
+      for (int m = (int)t; m < nm; m += (int)nth) {
+        const int lc = std::max(lcap, m);
+        ab_coeffs(m, lc, a.data(), b.data());
+        int lo = 0, hi = nn;
+        while (lo < hi) {
+          int mid = (lo + hi) / 2;
+          double lg = ring_maxlog2(m, lc, geo[mid].z, geo[mid].s, dfac.data(), a.data(), b.data());
+          if (lg >= thr2)
+            hi = mid;
+          else
+            lo = mid + 1;
+        }
+        out[m] = lo;
+      }
+    });
+  }
+  for (auto& x : th) x.join();
+}
+
+// ---------------------------------------------------------------------------
+// Device: Legendre start seeds.  For order m, kept north ring i and analysis
+// l-block b (l in [m + b*LB, m + (b+1)*LB)), store the first l of the block at
+// which P_lm is non-negligible (>= 2^-900), st, and P_st, P_st+1.  A plain
+// FP64 recurrence restarted from these two values reproduces the exponent-
+// rescaled recurrence bit for bit.  Block 0 doubles as the synthesis start.
+// ---------------------------------------------------------------------------
+__device__ static void pow_scaled_dev(double x, int n, double* mant, long long* ex) {
+  int ex0;
+  double base = frexp(x, &ex0);
+  long long be = ex0;
+  double r = 1.0;
+  long long re = 0;
+  while (n > 0) {
+    if (n & 1) {
+      int e;
+      r = frexp(r * base, &e);
+      re += e + be;
+    }
+    n >>= 1;
+    if (n) {
+      int e;
+      base = frexp(base * base, &e);
+      be = 2 * be + e;
+    }
+  }
+  *mant = r;
+  *ex = re;
+}
+
+__global__ void k_seeds(int lmax, int nhr, const RingGeo* __restrict__ geo, const int* __restrict__ i0,
+                        const double* __restrict__ dfac, const double2* __restrict__ ab,
+                        const long long* __restrict__ aboff, const long long* __restrict__ seedoff,
+                        int* __restrict__ seed_st, double2* __restrict__ seed_v) {
+  const int m = blockIdx.y;
+  const int i = i0[m] + blockIdx.x * blockDim.x + threadIdx.x;
+  if (i > nhr) return;
+  const int nr = nhr + 1 - i0[m];
+  const int nb = (lmax + 1 - m + kAnaLB - 1) / kAnaLB;
+  const long long base = seedoff[m] + (i - i0[m]);
+  for (int b = 0; b < nb; ++b) {
+    seed_st[base + (long long)b * nr] = 0x3fffffff;
+    seed_v[base + (long long)b * nr] = make_double2(0.0, 0.0);
+  }
+  const double z = geo[i].z, s = geo[i].s;
+  const double2* abm = ab + aboff[m];
+  double ms;
+  long long e;
+  pow_scaled_dev(s, m, &ms, &e);
+  double p1 = sqrt(dfac[m] / 4.0 / kPi) * ms;
+  if (m & 1) p1 = -p1;
+  int fe;
+  p1 = frexp(p1, &fe);
+  e += fe;
+  double p2 = 0.0;
+  int first = -1;
+  int cur_st = -1, cur_b = -1;
+  double v1 = 0.0;
+  for (int l = m; l <= lmax; ++l) {
+    double p;
+    if (l == m) {
+      p = p1;
+    } else {
+      const double2 c = abm[l - m];
+      p = fma(c.x * z, p1, -(c.y * p2));
+      p2 = p1;
+      p1 = p;
+    }
+    if (e < 0 && fabs(p1) > 0x1p100) {
+      long long sh = (-e < 100) ? -e : 100;
+      p1 = ldexp(p1, (int)-sh);
+      p2 = ldexp(p2, (int)-sh);
+      e += sh;
+      p = p1;
+    }
+    if (first < 0 && p != 0.0) {
+      int fp;
+      frexp(p, &fp);
+      if ((long long)fp + e >= kFlushExp) first = l;
+    }
+    if (first < 0) continue;
+    const double val = ldexp(p, (int)e);
+    // second seed value of the previous start (may fall in the next block)
+    if (cur_b >= 0 && l == cur_st + 1) {
+      seed_v[base + (long long)cur_b * nr] = make_double2(v1, val);
+    }
+    const int b = (l - m) / kAnaLB;
+    const int L0 = m + b * kAnaLB;
+    const int st = first > L0 ? first : L0;
+    if (l == st) {  // start of this block's non-zero run
+      cur_st = st;
+      cur_b = b;
+      v1 = val;
+      seed_st[base + (long long)b * nr] = st;
+      seed_v[base + (long long)b * nr] = make_double2(v1, 0.0);
+    }
+  }
+}
+
+hipError_t launch_seeds(Handle* h) {
+  dim3 grid((h->nnorth + 255) / 256, h->lmax + 1);
+  // diag_fac table lives right after the seeds in a temporary buffer
+  std::vector<double> dfac;
+  diag_fac(h->lmax + 1, dfac);
+  double* d_dfac = nullptr;
+  hipError_t e = hipMalloc(&d_dfac, dfac.size() * sizeof(double));
+  if (e != hipSuccess) return e;
+  e = hipMemcpyAsync(d_dfac, dfac.data(), dfac.size() * sizeof(double), hipMemcpyHostToDevice, h->stream);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_seeds, grid, dim3(256), 0, h->stream, h->lmax, h->nhr, h->d_geo, h->d_i0, d_dfac,
+                     h->d_ab, h->d_aboff, h->d_seedoff, h->d_seed_st, h->d_seed_v);
+  e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  e = hipStreamSynchronize(h->stream);
+  hipFree(d_dfac);
+  return e;
+}
+
+// ---------------------------------------------------------------------------
+// Host tables that depend on (nside, lmax): (a,b), spin constants, gather
+// slots and twiddles, seed offsets, synthesis-buffer offsets, work lists.
+// ---------------------------------------------------------------------------
+struct HostTables {
+  std::vector<RingGeo> geo;
+  std::vector<double> phi0;
+  std::vector<double2> ab, spin, tw;
+  std::vector<int> slot;
+};
+
+static void fill_ab_spin(Handle* h, HostTables& t) {
+  const int lmax = h->lmax;
+  h->aboff.resize(lmax + 2);
+  long long off = 0;
+  for (int m = 0; m <= lmax; ++m) {
+    h->aboff[m] = off;
+    off += lmax + 1 - m;
+  }
+  h->aboff[lmax + 1] = off;
+  t.ab.resize(off);
+  t.spin.resize(off);
+  std::vector<double> a(lmax + 1), b(lmax + 1);
+  for (int m = 0; m <= lmax; ++m) {
+    ab_coeffs(m, lmax, a.data(), b.data());
+    const int m1 = std::max(1, m);
+    for (int l = m; l <= lmax; ++l) {
+      t.ab[h->aboff[m] + l - m] = make_double2(a[l - m], b[l - m]);
+      // spin-2 constants (src/get_plm_recursive.f90:105-107, sht_data_init.f90:398-400)
+      double lam = 0.0;
+      if (l > m1) lam = 2.0 * std::sqrt((double)(2 * l + 1) / (2 * l - 1) * (double)((long long)l * l - (long long)m * m));
+      double nl = 0.0;
+      if (l >= 2) nl = 1.0 / std::sqrt((double)(l + 2) * (double)(l + 1) * (double)l * (double)(l - 1));
+      t.spin[h->aboff[m] + l - m] = make_double2(lam, nl);
+    }
+  }
+}
+
+// Gather slot class per (m, north ring): relative PERM slot and the sign of
+// the imaginary part (src/sht_data_init.f90:448-490 tab/fac).
+static void fill_slots(Handle* h, HostTables& t) {
+  const int lmax = h->lmax, nn = h->nnorth;
+  t.slot.resize((size_t)(lmax + 1) * nn);
+  t.tw.resize((size_t)(lmax + 1) * nn);
+  for (int m = 0; m <= lmax; ++m) {
+    for (int i = 0; i < nn; ++i) {
+      const int n = t.geo[i].nfft;
+      const int pos = m % n;
+      int rel, cls;
+      if (pos == 0) {
+        rel = 0;
+        cls = 0;
+      } else if (pos < n / 2) {
+        rel = 2 * pos;
+        cls = 1;
+      } else if (pos == n / 2) {
+        rel = 1;
+        cls = 0;
+      } else {
+        rel = 2 * (n - pos);
+        cls = 2;
+      }
+      t.slot[(size_t)m * nn + i] = rel * 4 + cls;
+      const double ang = (double)m * t.phi0[i];
+      t.tw[(size_t)m * nn + i] = make_double2(std::cos(ang), std::sin(ang));
+    }
+  }
+}
+
+static void fill_work(Handle* h) {
+  const int lmax = h->lmax;
+  h->nblk.resize(lmax + 1);
+  h->seedoff.resize(lmax + 2);
+  h->goff.resize(lmax + 2);
+  long long so = 0, go = 0;
+  for (int m = 0; m <= lmax; ++m) {
+    const int nr = h->nhr + 1 - h->i0[m];
+    h->nblk[m] = (lmax + 1 - m + kAnaLB - 1) / kAnaLB;
+    h->seedoff[m] = so;
+    so += (long long)h->nblk[m] * nr;
+    h->goff[m] = go;
+  }
+  h->seedoff[lmax + 1] = so;
+}
+
+static int ceil_div(int a, int b) { return (a + b - 1) / b; }
+
+// XCD-aware ordering: the hardware deals workgroups round-robin to 8 XCDs, so
+// item k of the launch runs on XCD k%8.  Lay the list out so consecutive items
+// of the natural order (same m, sharing gathered data and seeds) land on one XCD.
+static std::vector<LegItem> xcd_order(const std::vector<LegItem>& v) {
+  const size_t n = v.size();
+  const size_t per = (n + 7) / 8;
+  std::vector<LegItem> out(per * 8);
+  for (size_t k = 0; k < per * 8; ++k) {
+    const size_t x = k % 8, j = k / 8;
+    const size_t src = x * per + j;
+    out[k] = src < n ? v[src] : LegItem{-1, 0, 0, 0};
+  }
+  return out;
+}
+
+static hipError_t upload_items(const std::vector<LegItem>& v, LegItem** d, int* n) {
+  std::vector<LegItem> o = xcd_order(v);
+  *n = (int)o.size();
+  if (o.empty()) return hipSuccess;
+  hipError_t e = hipMalloc(d, o.size() * sizeof(LegItem));
+  if (e != hipSuccess) return e;
+  return hipMemcpy(*d, o.data(), o.size() * sizeof(LegItem), hipMemcpyHostToDevice);
+}
+
+static hipError_t build_items(Handle* h) {
+  const int lmax = h->lmax;
+  for (int sp = 0; sp < 2; ++sp) {
+    const int ncomp = sp ? 2 : 1;
+    const int ncols = 2 * ncomp * h->nsimp;
+    std::vector<LegItem> ana, syn;
+    for (int m = 0; m <= lmax; ++m) {
+      const int nr = h->nhr + 1 - h->i0[m];
+      // columns: at m = 0 the imaginary columns carry nothing
+      for (int par = 0; par < 2; ++par) {
+        const int ncb = ceil_div(ncols, kAnaCols);
+        for (int b = 0; b < h->nblk[m]; ++b)
+          for (int cb = 0; cb < ncb; ++cb) ana.push_back(LegItem{m, par, b, cb});
+        const int nrb = ceil_div(nr, kSynRings);
+        const int ncs = ceil_div(ncols, kSynCols);
+        for (int rb = 0; rb < nrb; ++rb)
+          for (int cb = 0; cb < ncs; ++cb) syn.push_back(LegItem{m, par, rb, cb});
+      }
+    }
+    hipError_t e = upload_items(ana, &h->d_ana_items[sp], &h->n_ana_items[sp]);
+    if (e != hipSuccess) return e;
+    e = upload_items(syn, &h->d_syn_items[sp], &h->n_syn_items[sp]);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
+template <class T>
+static hipError_t upload(T** d, const std::vector<T>& v) {
+  if (v.empty()) return hipSuccess;
+  hipError_t e = hipMalloc(d, v.size() * sizeof(T));
+  if (e != hipSuccess) return e;
+  return hipMemcpy(*d, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice);
+}
+
+// Size of the m-major synthesis buffer for ncomp components.
+void synth_offsets(Handle* h, int ncomp) {
+  long long go = 0;
+  const int ncols = 2 * ncomp * h->nsimp;
+  for (int m = 0; m <= h->lmax; ++m) {
+    h->goff[m] = go;
+    go += 2LL * (h->nhr + 1 - h->i0[m]) * ncols;
+  }
+  h->goff[h->lmax + 1] = go;
+  h->G_size = go;
+}
+
+hipError_t setup_tables(Handle* h) {
+  HostTables t;
+  ring_geometry(h->nside, t.geo, t.phi0);
+  ring_tab(h->nside, h->lmax + 1, t.geo, h->i0);
+  fill_ab_spin(h, t);
+  fill_slots(h, t);
+  fill_work(h);
+  hipError_t e;
+  if ((e = upload(&h->d_geo, t.geo)) != hipSuccess) return e;
+  if ((e = upload(&h->d_i0, h->i0)) != hipSuccess) return e;
+  if ((e = upload(&h->d_ab, t.ab)) != hipSuccess) return e;
+  if ((e = upload(&h->d_spin, t.spin)) != hipSuccess) return e;
+  if ((e = upload(&h->d_aboff, h->aboff)) != hipSuccess) return e;
+  if ((e = upload(&h->d_tw, t.tw)) != hipSuccess) return e;
+  if ((e = upload(&h->d_slot, t.slot)) != hipSuccess) return e;
+  if ((e = upload(&h->d_seedoff, h->seedoff)) != hipSuccess) return e;
+  const long long nseed = h->seedoff[h->lmax + 1];
+  if ((e = hipMalloc(&h->d_seed_st, std::max(1LL, nseed) * sizeof(int))) != hipSuccess) return e;
+  if ((e = hipMalloc(&h->d_seed_v, std::max(1LL, nseed) * sizeof(double2))) != hipSuccess) return e;
+  if ((e = launch_seeds(h)) != hipSuccess) return e;
+  synth_offsets(h, 2);  // sized for the largest (spin-2) layout; spin-0 uses a prefix
+  if ((e = upload(&h->d_goff, h->goff)) != hipSuccess) return e;
+  if ((e = build_items(h)) != hipSuccess) return e;
+  return hipSuccess;
+}
+
+}  // namespace fsht

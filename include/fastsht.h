@@ -1,0 +1,119 @@
+/*
+ * fastsht.h -- C ABI of libfastsht (MI355X / gfx950).
+ *
+ * Batched HEALPix-RING spherical harmonic transforms in FP64: many maps at
+ * once, every transform stage on the GPU.  Drop-in boundary for the
+ * reference's Fortran module `fastSHT` (liuhao-cn/fastSHT, wrapped by f90wrap
+ * and driven from SHT.py); each entry point names the reference interface it
+ * replaces.
+ *
+ * Layouts (all Fortran order, float64), as in the reference:
+ *   maps  (npix, nsim)             map k at maps + k*npix, npix = 12*nside^2, RING order
+ *   alms  (nsim, lmax+1, lmax+1)   Re a_lm at [k, l, m] (l >= m),
+ *                                  Im a_lm at [k, l-m, lmax+1-m] (m >= 1)
+ *
+ * Pointers are host memory (loc = FSHT_HOST; the call copies through HBM and
+ * returns when results are in host memory) or device memory (loc =
+ * FSHT_DEVICE; the call is asynchronous on the handle's stream, see
+ * fsht_get_stream / fsht_synchronize).
+ *
+ * Every function returns FSHT_OK (0) or a negative status; fsht_last_error()
+ * describes the failure.  A handle must not be used from two host threads at
+ * once; distinct handles are independent.
+ */
+#ifndef FASTSHT_H
+#define FASTSHT_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FSHT_OK 0
+#define FSHT_ERR_ARG (-1)   /* invalid argument */
+#define FSHT_ERR_NOMEM (-2) /* device allocation failed */
+#define FSHT_ERR_HIP (-3)   /* HIP runtime error */
+#define FSHT_ERR_STATE (-4) /* call not valid for this handle's configuration/state */
+
+#define FSHT_HOST 0
+#define FSHT_DEVICE 1
+
+typedef struct fsht_handle fsht_handle;
+
+/* Library identification string. */
+const char* fsht_version(void);
+
+/* Message of the last failure on this handle (or of the calling thread when h is NULL). */
+const char* fsht_last_error(const fsht_handle* h);
+
+/* Create a transform configuration on the current HIP device: ring geometry,
+ * ring-skip table, Legendre start seeds and workspace stay resident.
+ * Replaces SHT.__init__ (SHT.py:17-62) -> sht_data_alloc + sht_set_data
+ * (src/sht_data_init.f90:2-311).  nside: power of two <= 1024;
+ * 0 <= lmax <= 4*nside; pol != 0 enables the spin-2 calls. */
+int fsht_create(int nside, int lmax, int nsim, int niter, int pol, fsht_handle** out);
+
+/* Release a handle.  Replaces sht_data_free (src/sht_data_init.f90:319-372). */
+int fsht_destroy(fsht_handle* h);
+
+/* The handle's HIP stream (hipStream_t) and a blocking wait on it. */
+int fsht_get_stream(fsht_handle* h, void** stream);
+/* Enqueue this handle's work on a caller stream (hipStream_t; NULL restores the
+ * handle's own stream).  The caller keeps ownership of the stream. */
+int fsht_set_stream(fsht_handle* h, void* stream);
+int fsht_synchronize(fsht_handle* h);
+
+/* out[0..n) = {nside, lmax, nsim, niter, pol, npix, nsim padded to 16, m-major buffer size}. */
+int fsht_info(const fsht_handle* h, long long* out, int n);
+
+/* The ring-skip table used (first kept north ring per m, m = 0..lmax), as read
+ * from SHT_ring_skip_table/NNNN.bin at SHT.py:31-32. */
+int fsht_ring_tab(const fsht_handle* h, int* out);
+
+/* Temperature analysis maps -> alms, with niter Jacobi refinements.
+ * niter = 0 replaces t2alm (src/sht_main.f90:38-89); niter > 0 replaces
+ * t2alm_iter_old (src/sht_main.f90:276-367), called from SHT.t2alm
+ * (SHT.py:72-107).  alms = NULL keeps the result on the device for
+ * fsht_get_alm (the reference's get_alm=False path, SHT.py:88-95). */
+int fsht_t2alm(fsht_handle* h, const double* maps, double* alms, int niter, int loc);
+
+/* Fetch the alms kept by fsht_t2alm(..., alms=NULL).  Replaces get_alm
+ * (src/sht_main.f90:1294-1302) / SHT.get_alms (SHT.py:214-227). */
+int fsht_get_alm(fsht_handle* h, double* alms, int loc);
+
+/* Temperature synthesis alms -> maps.  Replaces alm2t (src/sht_main.f90:96-155)
+ * / SHT.alm2t (SHT.py:109-122). */
+int fsht_alm2t(fsht_handle* h, const double* alms, double* maps, int loc);
+
+/* Spin-2 analysis Q,U -> E,B with niter Jacobi refinements.  niter = 0
+ * replaces QU2EB (src/sht_main.f90:383-436); niter > 0 replaces
+ * qu2eb_iter_old (src/sht_main.f90:563-673); SHT.qu2eb (SHT.py:124-144). */
+int fsht_qu2eb(fsht_handle* h, const double* Q, const double* U, double* E, double* B, int niter, int loc);
+
+/* E only / B only.  Replace QU2E (src/sht_main.f90:955-1005) and QU2B
+ * (src/sht_main.f90:1016-1064); SHT.qu2e / SHT.qu2b (SHT.py:146-176). */
+int fsht_qu2e(fsht_handle* h, const double* Q, const double* U, double* E, int loc);
+int fsht_qu2b(fsht_handle* h, const double* Q, const double* U, double* B, int loc);
+
+/* Spin-2 synthesis E,B -> Q,U.  Replaces EB2QU (src/sht_main.f90:1079-1147)
+ * / SHT.eb2qu (SHT.py:179-190). */
+int fsht_eb2qu(fsht_handle* h, const double* E, const double* B, double* Q, double* U, int loc);
+
+/* alm utilities on device arrays (stream: hipStream_t or NULL). */
+/* out = in*fac(l) + shift(l), with an extra factor m0 on m = 0.
+ * Replaces norm_alm1 (src/norm_alm.f90:14-33). */
+int fsht_norm_alm(const double* alm_in, double* alm_out, const double* fac, const double* shift, double m0,
+                  int lmax, int nsim, void* stream);
+/* alm *= win(l) in place.  Replaces filt_alm1 (src/filt_alm.f90:16-36). */
+int fsht_filt_alm(double* alm, const double* win, int lmax, int nsim, void* stream);
+/* Square packing -> healpy m-major order: alm_hp (2, (lmax+1)(lmax+2)/2, nsim)
+ * Fortran order (Re, Im interleaved).  Replaces convert_alm_healpy1
+ * (src/convert_alm.f90:34-56) used by SHT.convert_alm_healpy (SHT.py:204-210). */
+int fsht_convert_alm_healpy(const double* alm, double* alm_hp, int lmax, int nsim, void* stream);
+/* Inverse of the above.  Replaces convert_alm_healpy_inv1 (src/convert_alm.f90:87-107). */
+int fsht_convert_alm_healpy_inv(const double* alm_hp, double* alm, int lmax, int nsim, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FASTSHT_H */

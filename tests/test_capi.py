@@ -1,0 +1,85 @@
+"""CPU tests of the boundary: the C-ABI library loads, exports every symbol the
+header declares, and rejects bad arguments on the host side (no GPU work)."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from conftest import ROOT
+
+
+def _declared():
+    src = open(os.path.join(ROOT, "include", "fastsht.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(fsht_[a-z_0-9]+)\s*\(", src)))
+
+
+def test_header_and_binding_agree():
+    from fastsht_amd import _lib
+    assert _declared() == sorted(_lib.EXPORTS)
+
+
+def test_library_exports_every_declared_symbol():
+    from fastsht_amd import _lib
+    lib = _lib.load()
+    for name in _declared():
+        assert hasattr(lib, name), name
+    assert b"gfx950" in lib.fsht_version()
+
+
+def test_create_rejects_bad_arguments():
+    from fastsht_amd import _lib
+    lib = _lib.load()
+    h = ctypes.c_void_p()
+    for args in [(3, 8, 1, 0, 0), (2048, 100, 1, 0, 0), (8, 33, 1, 0, 0), (8, 8, 0, 0, 0), (8, 8, 1, -1, 0)]:
+        rc = lib.fsht_create(*args, ctypes.byref(h))
+        assert rc == _lib.FSHT_ERR_ARG, args
+        assert h.value is None
+        assert lib.fsht_last_error(None)
+
+
+def test_null_handle_calls_fail_cleanly():
+    from fastsht_amd import _lib
+    lib = _lib.load()
+    assert lib.fsht_t2alm(None, None, None, 0, 0) == _lib.FSHT_ERR_ARG
+    assert lib.fsht_synchronize(None) == _lib.FSHT_ERR_ARG
+    assert lib.fsht_destroy(None) == _lib.FSHT_OK
+    assert lib.fsht_norm_alm(None, None, None, None, 1.0, 4, 1, None) == _lib.FSHT_ERR_ARG
+
+
+def test_sht_api_surface():
+    """The drop-in object keeps SHT.py's method names and defaults."""
+    import inspect
+
+    from fastsht_amd import SHT
+    sig = inspect.signature(SHT.__init__)
+    assert list(sig.parameters)[1:] == ["nside", "lmax", "nsim", "niter", "pol", "all_buff"]
+    assert sig.parameters["niter"].default == 0 and sig.parameters["pol"].default is False
+    for name in ["set_maps", "set_alms", "t2alm", "alm2t", "qu2eb", "qu2e", "qu2b", "eb2qu", "fix_eb",
+                 "convert_alm_healpy", "get_alms"]:
+        assert callable(getattr(SHT, name)), name
+    t2 = inspect.signature(SHT.t2alm)
+    assert list(t2.parameters)[1:] == ["maps_in", "alms_in", "method", "get_alm"]
+    assert t2.parameters["method"].default == "old" and t2.parameters["get_alm"].default is True
+
+
+def test_no_cpu_fallback_without_gpu():
+    """Without a GPU the product raises; it never computes on the CPU."""
+    import torch
+
+    if torch.cuda.is_available():
+        pytest.skip("GPU present")
+    from fastsht_amd import SHT, FastSHTError
+    with pytest.raises(FastSHTError):
+        SHT(4, 8, 2)
+
+
+def test_product_does_not_import_oracle():
+    pkg = os.path.join(ROOT, "fastsht_amd")
+    for dirpath, _, files in os.walk(pkg):
+        for f in files:
+            if f.endswith((".py", ".hip", ".h", ".cpp")):
+                txt = open(os.path.join(dirpath, f)).read()
+                assert "oracle" not in txt.replace("reference's", "").lower() or f == "__init__.py", f

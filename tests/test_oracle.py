@@ -1,0 +1,295 @@
+"""CPU tests pinning the oracle (oracle/oracle.py + oracle/plm_oracle.c).
+
+Pins, none of which re-types the oracle's own formulas:
+  * the reference's data tables (ring-skip table, Legendre seed tables), via
+    the sampled fixture tests/golden/ref_tables.npz and, where present, the
+    full files under /root/reference;
+  * closed forms and independent special-function codes (scipy sph_harm_y,
+    an explicit Wigner-d sum for spin-weighted harmonics, mpmath at high l);
+  * a brute-force direct-sum SHT at tiny nside;
+  * invariants: Jacobi convergence on band-limited input, eb2qu/qu2eb
+    round trip, aliasing (lmax > 2 nside) against brute force.
+"""
+import math
+import os
+
+import numpy as np
+import pytest
+
+from conftest import REFERENCE, have_reference, maxrel
+from oracle import oracle as O
+
+
+# ---------------------------------------------------------------------------
+# reference data tables
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("ns", [1, 2, 4, 8, 16, 32, 64, 128, 256, 512, 1024])
+def test_ring_tab_matches_reference_samples(golden, ns):
+    n = int(golden[f"rt_{ns}_len"])
+    lcap = 4 * ns if ns <= 1024 else 6143
+    rt = O.ring_tab(ns, nm=n, lcap=lcap)
+    m = golden[f"rt_{ns}_m"]
+    assert np.array_equal(rt[m], golden[f"rt_{ns}_v"])
+
+
+def test_ring_tab_2048_samples(golden):
+    # nside 2048: table length 6144, rule evaluated with l <= 6143
+    m = golden["rt_2048_m"]
+    rt = O.ring_tab(2048, nm=int(m.max()) + 1, lcap=6143)
+    assert np.array_equal(rt[m], golden["rt_2048_v"])
+
+
+@pytest.mark.skipif(not have_reference(), reason="reference tables not mounted")
+@pytest.mark.parametrize("ns", [1, 4, 16, 64, 256, 1024])
+def test_ring_tab_matches_reference_files(ns):
+    ref = np.fromfile(f"{REFERENCE}/SHT_ring_skip_table/{ns:04d}.bin", dtype=np.int32)
+    assert np.array_equal(O.ring_tab(ns, nm=len(ref), lcap=len(ref) - 1), ref)
+
+
+@pytest.mark.parametrize("ns", [8, 16, 32])
+def test_ring_tab_bisection_equals_scan(ns):
+    a = O.ring_tab(ns)
+    b = O.ring_tab(ns, exhaustive=True)
+    assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("ns", [2, 4, 8, 16, 32, 64, 128, 256])
+def test_plm_seed_tables(golden, ns):
+    """pos(i,m) = first l in [m, 4 nside-1] with |P_lm(theta_i)| >= 1e-20 (else m);
+    val1/val2 = P_{pos,m}, P_{pos+1,m} (plm_table/*.bin, read at SHT.py:34-45)."""
+    ii, mm, pos = golden[f"plm_{ns}_i"], golden[f"plm_{ns}_m"], golden[f"plm_{ns}_pos"]
+    nm = 4 * ns + 1
+    worst = 0.0
+    for i, m, p in zip(ii, mm, pos):
+        P, _ = O.plm_north(ns, nm, int(m), np.array([int(i)]))
+        lim = 4 * ns - 1
+        ab = np.abs(P[0, : max(0, lim - m + 1)])
+        ok = np.nonzero(ab >= 1e-20)[0]
+        mine = int(m) + int(ok[0]) if len(ok) else int(m)
+        assert mine == int(p), (ns, i, m)
+        if f"plm_{ns}_v1" in golden:
+            k = list(zip(ii, mm)).index((i, m))
+            for v, off in ((golden[f"plm_{ns}_v1"][k], 0), (golden[f"plm_{ns}_v2"][k], 1)):
+                if abs(v) > 2.0 ** -880:  # the oracle flushes below 2^-900
+                    worst = max(worst, abs(P[0, p + off - m] - v) / abs(v))
+    assert worst < 2e-13
+
+
+# ---------------------------------------------------------------------------
+# Legendre functions against independent codes
+# ---------------------------------------------------------------------------
+def test_plm_vs_scipy_small_l():
+    from scipy.special import sph_harm_y
+    th = np.array([0.01, 0.3, 1.0, 1.5707963267948966, 2.0, 3.1])
+    lmax = 30
+    for m in [0, 1, 2, 7, 19, 30]:
+        P = O.plm_generic(m, lmax, np.cos(th), np.sin(th))
+        for l in range(m, lmax + 1):
+            Y = sph_harm_y(l, m, th, 0.0).real
+            np.testing.assert_allclose(P[:, l - m], Y, rtol=1e-11, atol=1e-13)
+
+
+def _pbar_mp(l, m, th, dps=1500):
+    """Normalised P_lm (Condon-Shortley phase) from the terminating Gauss
+    hypergeometric series in 1500-digit arithmetic:
+    P_l^m(x) = (-1)^m (1-x^2)^{m/2}/(2^m m!) (l+m)!/(l-m)! 2F1(m-l, m+l+1; m+1; (1-x)/2)."""
+    import mpmath as mp
+    mp.mp.dps = dps
+    x, s = mp.cos(mp.mpf(th)), mp.sin(mp.mpf(th))
+    F = mp.hyp2f1(m - l, m + l + 1, m + 1, (1 - x) / 2)
+    P = (-1) ** m * s ** m / (2 ** m * mp.factorial(m)) * mp.factorial(l + m) / mp.factorial(l - m) * F
+    return float(mp.sqrt((2 * l + 1) / (4 * mp.pi) * mp.factorial(l - m) / mp.factorial(l + m)) * P)
+
+
+@pytest.mark.parametrize("l,m,th", [(2000, 0, 0.3), (2047, 100, 0.7), (1500, 1200, 1.2), (3000, 2500, 1.0),
+                                    (1000, 600, 0.7)])
+def test_plm_vs_hypergeometric_high_l(l, m, th):
+    P = O.plm_generic(m, l, [math.cos(th)], [math.sin(th)])[0, -1]
+    ref = _pbar_mp(l, m, th)
+    assert abs(P - ref) <= 3e-13 * abs(ref), (l, m)
+
+
+@pytest.mark.parametrize("l,m,th", [(2000, 1500, 0.6), (800, 700, 0.9), (2047, 1900, 1.1)])
+def test_plm_pmm_underflow_regime(l, m, th):
+    """P_mm at large m on a polar ring is below 1e-308 (sin(0.6)^1500 ~ 1e-372);
+    the oracle must still return the true values where they are representable."""
+    P = O.plm_generic(m, l, [math.cos(th)], [math.sin(th)])[0, -1]
+    ref = _pbar_mp(l, m, th)
+    assert abs(P - ref) <= 3e-13 * abs(ref)
+
+
+def _wigner_d(j, mp_, m, b):
+    f = math.factorial
+    c, s_ = math.cos(b / 2), math.sin(b / 2)
+    tot = 0.0
+    for s in range(0, 2 * j + 1):
+        a1, a2, a3, a4 = j + m - s, s, mp_ - m + s, j - mp_ - s
+        if min(a1, a2, a3, a4) < 0:
+            continue
+        tot += ((-1) ** (mp_ - m + s) * math.sqrt(f(j + mp_) * f(j - mp_) * f(j + m) * f(j - m))
+                / (f(a1) * f(a2) * f(a3) * f(a4)) * c ** (2 * j + m - mp_ - 2 * s) * s_ ** (mp_ - m + 2 * s))
+    return tot
+
+
+def _sY(s, l, m, th):
+    return (-1) ** m * math.sqrt((2 * l + 1) / (4 * math.pi)) * _wigner_d(l, m, -s, th)
+
+
+def test_spin2_functions_vs_wigner_d():
+    """P = c_m Y_0, F1 = c_m (Y_2 + Y_-2)/2, F2 = c_m (Y_2 - Y_-2)/2 with one
+    sign c_m = (-1)^m per m (Wigner-d convention of _sY); the reference's F1/F2
+    (get_plm_recursive.f90:103-133) must satisfy this for every l, m, ring."""
+    ns, lmax = 4, 9
+    R = O.OracleSHT(ns, lmax, 1)
+    for m in range(lmax + 1):
+        P0, P1, P2 = R.plm(m, pol=True)
+        c = (-1) ** m
+        for i in [2, 4, 7, 11]:
+            th = math.atan2(R.s[i], R.z[i])
+            for l in range(m, lmax + 1):
+                y0 = _sY(0, l, m, th)
+                assert abs(P0[i, l - m] - c * y0) < 1e-12
+                if l >= 2:
+                    yp, ym = _sY(2, l, m, th), _sY(-2, l, m, th)
+                    assert abs(P1[i, l - m] - c * (yp + ym) / 2) < 1e-12, (l, m, i)
+                    assert abs(P2[i, l - m] - c * (yp - ym) / 2) < 1e-12, (l, m, i)
+
+
+def test_spin2_closed_form_l2():
+    # F1_{20} = sqrt(15/(32 pi)) sin^2 theta, F2_{l0} = 0
+    R = O.OracleSHT(8, 10, 1)
+    _, P1, P2 = R.plm(0, pol=True)
+    s = R.s[: 2 * 8]
+    # the reference formula cancels terms ~1/sin^2 down to ~sin^2 near the pole:
+    # absolute, not relative, agreement is what double precision allows there
+    np.testing.assert_allclose(P1[: 16, 2], math.sqrt(15 / (32 * math.pi)) * s ** 2, rtol=1e-13, atol=2e-14)
+    assert np.all(P2 == 0)
+
+
+# ---------------------------------------------------------------------------
+# geometry
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("ns", [1, 2, 8, 64])
+def test_ring_geometry(ns):
+    i1, nfft = O.ring_info(ns)
+    assert i1[0] == 0 and i1[-1] + nfft[-1] == 12 * ns * ns
+    assert np.all(i1[1:] == i1[:-1] + nfft[:-1])
+    z, s = O.ring_z_s(ns)
+    np.testing.assert_allclose(z ** 2 + s ** 2, 1.0, rtol=0, atol=5e-16)
+    # equal-area: each ring's solid-angle band matches nfft * 4pi/npix on average
+    th, ph = O.pixel_angles(ns)
+    assert abs(np.cos(th).sum()) < 1e-9 * len(th)
+
+
+# ---------------------------------------------------------------------------
+# transforms against brute force and invariants
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("ns,lmax", [(1, 2), (1, 4), (2, 5), (2, 8), (4, 11), (4, 16), (8, 20)])
+def test_t2alm_alm2t_vs_direct_sum(ns, lmax):
+    rng = np.random.default_rng(7)
+    nsim = 3
+    maps = np.asfortranarray(rng.standard_normal((12 * ns * ns, nsim)))
+    S = O.OracleSHT(ns, lmax, nsim)
+    assert maxrel(S.t2alm(maps), O.brute_t2alm(maps, ns, lmax)) < 1e-13
+    alm = O.random_alm(lmax, nsim, seed=1)
+    assert maxrel(S.alm2t(alm), O.brute_alm2t(alm, ns)) < 1e-13
+
+
+def test_jacobi_converges_on_bandlimited():
+    ns, lmax, nsim = 8, 12, 2
+    S = O.OracleSHT(ns, lmax, nsim)
+    alm = O.random_alm(lmax, nsim, seed=5)
+    t = S.alm2t(alm)
+    errs = [maxrel(S.t2alm(t, niter=k), alm) for k in range(4)]
+    assert errs[0] > errs[1] > errs[2] > errs[3]
+    assert errs[3] < 1e-3 * errs[0]
+
+
+def test_qu_roundtrip_bandlimited():
+    ns, lmax, nsim = 8, 12, 2
+    S = O.OracleSHT(ns, lmax, nsim)
+    E = O.random_alm(lmax, nsim, seed=1)
+    B = O.random_alm(lmax, nsim, seed=2)
+    for a in (E, B):  # spin-2 fields have no l < 2 content
+        a[:, :2, :] = 0.0
+        a[:, :, lmax:] = 0.0
+        a[:, 0, lmax] = 0.0
+    E[:, :, 0][:, :2] = 0.0
+    Q, U = S.eb2qu(E, B)
+    E3, B3 = S.qu2eb(Q, U, niter=3)
+    E0, B0 = S.qu2eb(Q, U, niter=0)
+    assert maxrel(E3, E) < 0.2 * maxrel(E0, E)
+    assert maxrel(B3, B) < 0.2 * maxrel(B0, B)
+    assert maxrel(E3, E) < 1e-3
+
+
+def test_qu2e_qu2b_consistent_with_qu2eb():
+    ns, lmax, nsim = 4, 9, 2
+    rng = np.random.default_rng(2)
+    Q = np.asfortranarray(rng.standard_normal((12 * ns * ns, nsim)))
+    U = np.asfortranarray(rng.standard_normal((12 * ns * ns, nsim)))
+    S = O.OracleSHT(ns, lmax, nsim)
+    E, B = S.qu2eb(Q, U)
+    assert maxrel(S.qu2e(Q, U), E) < 1e-14
+    assert maxrel(S.qu2b(Q, U), B) < 1e-14
+
+
+def test_planted_parity_mistake_is_caught():
+    """A dropped (-1)^{l+m} on the south mirror must break the direct-sum check."""
+    ns, lmax, nsim = 2, 5, 2
+    rng = np.random.default_rng(3)
+    maps = np.asfortranarray(rng.standard_normal((12 * ns * ns, nsim)))
+    S = O.OracleSHT(ns, lmax, nsim)
+    good = S.t2alm(maps)
+    orig = S.plm
+
+    def broken(m, pol=False):
+        P = [p.copy() for p in orig(m, pol)]
+        south = np.arange(2 * ns, S.nring)
+        P[0][south] = np.abs(P[0][south]) * np.sign(P[0][S.nring - 1 - south])
+        return tuple(P)
+
+    S.plm = broken
+    S._pcache = {}
+    bad = S.t2alm(maps)
+    brute = O.brute_t2alm(maps, ns, lmax)
+    assert maxrel(good, brute) < 1e-13
+    assert maxrel(bad, brute) > 1e-3
+
+
+# ---------------------------------------------------------------------------
+# alm utilities
+# ---------------------------------------------------------------------------
+def test_convert_alm_healpy_order():
+    lmax, nsim = 6, 2
+    alm = O.random_alm(lmax, nsim, seed=9)
+    hp = O.convert_alm_healpy(alm)
+    for m in range(lmax + 1):
+        for l in range(m, lmax + 1):
+            idx = m * (2 * lmax + 1 - m) // 2 + l  # healpy Alm.getidx
+            im = alm[:, l - m, lmax + 1 - m] if m else 0.0
+            np.testing.assert_array_equal(hp[idx], alm[:, l, m] + 1j * im)
+    assert np.array_equal(O.convert_alm_healpy_inv(hp, lmax), alm)
+
+
+def test_norm_and_filt():
+    lmax, nsim = 5, 3
+    alm = O.random_alm(lmax, nsim, seed=4)
+    fac = np.arange(1, lmax + 2, dtype=float)
+    shift = np.linspace(0, 1, lmax + 1)
+    out = O.norm_alm(alm, fac, shift, 0.5)
+    hp_in, hp_out = O.convert_alm_healpy(alm), O.convert_alm_healpy(out)
+    for m in range(lmax + 1):
+        for l in range(m, lmax + 1):
+            idx = m * (2 * lmax + 1 - m) // 2 + l
+            f = fac[l] * (0.5 if m == 0 else 1.0)
+            exp_re = hp_in[idx].real * f + shift[l]
+            np.testing.assert_allclose(hp_out[idx].real, exp_re, rtol=1e-15)
+            if m:
+                np.testing.assert_allclose(hp_out[idx].imag, hp_in[idx].imag * f + shift[l], rtol=1e-15)
+    win = np.linspace(1, 0, lmax + 1)
+    f2 = O.convert_alm_healpy(O.filt_alm(alm, win))
+    for m in range(lmax + 1):
+        for l in range(m, lmax + 1):
+            idx = m * (2 * lmax + 1 - m) // 2 + l
+            np.testing.assert_allclose(f2[idx], hp_in[idx] * win[l], rtol=1e-15)
