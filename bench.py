@@ -1,0 +1,213 @@
+"""Benchmark: batched temperature SHT round trip on one MI355X.
+
+One step = t2alm(niter=3) + alm2t over nsim maps at nside=1024, lmax=2047
+(FP64, inputs resident in HBM), on synthetic maps built as alm2t of Gaussian
+alm with C_l = 1/(l(l+1)) (l >= 2) plus white noise of variance 1e-3.
+Prints one JSON line (see README/DESIGN for the fields).
+
+  python bench.py [--steps K] [--warmup W] [--nside 1024 --lmax 2047 --nsim 200 --niter 3]
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+# MI355X published peaks
+PEAK_FP64_MFMA_TFLOPS = 78.6
+PEAK_HBM_GBS = 8000.0
+KERNEL_CLASSES = ["ring_fft_fwd", "ring_fft_inv", "transpose", "legendre_analysis", "legendre_synthesis",
+                  "alias_fold", "spectrum_gather"]
+
+
+def legendre_flops_per_map(nside, lmax, i0):
+    """Algorithmic FP64 flops of one spin-0 Legendre pass for one map with the
+    north/south fold: sum_m g_m * 2 * (lmax+1-m) * (2*nside - i0(m)), g_0 = 1,
+    g_m = 2 (Re and Im share P)."""
+    tot = 0
+    for m in range(lmax + 1):
+        g = 1 if m == 0 else 2
+        tot += g * 2 * (lmax + 1 - m) * (2 * nside - int(i0[m]))
+    return float(tot)
+
+
+def make_inputs(sht, seed):
+    """Seeded Gaussian alm (C_l = 1/(l(l+1))) and maps = alm2t(alm) + N(0, 1e-3), on the GPU."""
+    import torch
+
+    L = sht.lmax + 1
+    nsim = sht.nsim
+    g = torch.Generator(device="cuda")
+    g.manual_seed(seed)
+    l = torch.arange(L, dtype=torch.float64, device="cuda")
+    cl = torch.zeros(L, dtype=torch.float64, device="cuda")
+    cl[2:] = 1.0 / (l[2:] * (l[2:] + 1.0))
+    # storage order (col, row, sim): col = m for Re (row = l >= m); col c >= 1
+    # holds Im of m = L - c at rows l - m.
+    c = torch.arange(L, device="cuda")[:, None]
+    r = torch.arange(L, device="cuda")[None, :]
+    re_part = r >= c
+    ell = torch.where(re_part, r, r + L - c).clamp(max=L - 1)
+    half = torch.where(re_part & (c == 0), torch.ones_like(cl[0]), torch.full_like(cl[0], 0.5))
+    sd = torch.sqrt(cl[ell] * half)
+    store = torch.randn((L, L, nsim), dtype=torch.float64, device="cuda", generator=g) * sd[:, :, None]
+    alm = store.permute(2, 1, 0)  # (nsim, L, L) Fortran order
+    maps = sht.alm2t(alm)
+    noise = torch.randn((nsim, sht.npix), dtype=torch.float64, device="cuda", generator=g) * math.sqrt(1e-3)
+    maps.add_(noise.t())
+    del store, noise
+    return alm, maps
+
+
+def cpu_baseline(seconds_budget=20.0):
+    """The oracle (CPU FP64 restatement of the reference algorithm: scipy/numpy
+    FFT, OpenBLAS dgemm per m, C Legendre recursion) on a bounded sample of the
+    same workload: nside=1024, lmax=2047, one map, one forward pass
+    (t2alm niter=0) and one backward pass (alm2t).  A t2alm(niter=3)+alm2t
+    round trip costs 4 forward + 4 backward Legendre passes, so maps/s =
+    1 / (4 * (t_fwd + t_bwd)) (the two ring-FFT passes of the round trip are
+    included in t_fwd and t_bwd once each; the 3 extra FFT-free passes are
+    counted at full pass cost, an upper bound on the CPU time)."""
+    import numpy as np
+    from threadpoolctl import threadpool_info
+
+    from oracle import oracle as O
+
+    nside, lmax = 1024, 2047
+    S = O.OracleSHT(nside, lmax, 1)
+    rng = np.random.default_rng(0)
+    maps = np.asfortranarray(rng.standard_normal((12 * nside * nside, 1)))
+    t0 = time.perf_counter()
+    a = S.t2alm(maps, niter=0)
+    t1 = time.perf_counter()
+    S.alm2t(a)
+    t2 = time.perf_counter()
+    tf, tb = t1 - t0, t2 - t1
+    blas = [p.get("num_threads", 1) for p in threadpool_info() if p.get("user_api") == "blas"]
+    return {
+        "value": 1.0 / (4.0 * (tf + tb)),
+        "unit": "maps/s",
+        "cores": int(max(blas) if blas else 1),
+        "kind": "port",
+        "sample": (f"oracle (numpy FFT + OpenBLAS dgemm + C recursion), nside=1024 lmax=2047, 1 map: "
+                   f"t2alm(niter=0) {tf:.1f}s + alm2t {tb:.1f}s; t2alm(niter=3)+alm2t = 4x(fwd+bwd) passes"),
+        "cpu": _cpu_model(),
+    }
+
+
+def _cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--nside", type=int, default=1024)
+    ap.add_argument("--lmax", type=int, default=None)
+    ap.add_argument("--nsim", type=int, default=200)
+    ap.add_argument("--niter", type=int, default=3)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); "
+              "the transform shards over maps with no communication, see DESIGN.md", file=sys.stderr)
+        sys.exit(2)
+    lmax = args.lmax if args.lmax is not None else 2 * args.nside - 1
+
+    import torch
+
+    from fastsht_amd import SHT, _lib
+
+    torch.cuda.set_device(0)
+    sht = SHT(args.nside, lmax, args.nsim, niter=args.niter)
+    alm, maps = make_inputs(sht, args.seed)
+    alm_out = torch.empty_like(alm.permute(2, 1, 0).contiguous()).permute(2, 1, 0)
+    maps_out = torch.empty((args.nsim, sht.npix), dtype=torch.float64, device="cuda").t()
+    lib = _lib.load()
+
+    def step():
+        sht.t2alm(maps, alm_out)
+        sht.alm2t(alm_out, maps_out)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    _lib.check(lib.fsht_profile(sht._h, 1), sht._h)
+    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    start.record()
+    for _ in range(args.steps):
+        step()
+    stop.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms_total = start.elapsed_time(stop)
+    prof = (ctypes.c_double * 14)()
+    _lib.check(lib.fsht_profile_read(sht._h, prof, 14), sht._h)
+    _lib.check(lib.fsht_profile(sht._h, 0), sht._h)
+    kern = {KERNEL_CLASSES[c]: {"launches": int(prof[2 * c]), "ms": prof[2 * c + 1]} for c in range(7)}
+    ms_per_step = ms_total / args.steps
+    maps_per_s = args.nsim * args.steps / (ms_total / 1e3)
+
+    # roofline of the dominant kernel (largest total time)
+    i0 = sht.ring_tab()
+    fpm = legendre_flops_per_map(args.nside, lmax, i0)
+    dom = max(("legendre_analysis", "legendre_synthesis"), key=lambda k: kern[k]["ms"])
+    avg_ms = kern[dom]["ms"] / max(1, kern[dom]["launches"])
+    flops_launch = fpm * args.nsim
+    achieved = flops_launch / (avg_ms / 1e3) / 1e12
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", f"traffic_{dom}_nside{args.nside}_nsim{args.nsim}.json")
+    if os.path.exists(tf):
+        traffic = json.load(open(tf)).get("bytes_per_launch")
+
+    # sanity: the round trip reproduces the inputs' alm (noise-limited)
+    rel = float(((alm_out - alm).abs().max() / alm.abs().max()).item())
+
+    out = {
+        "metric": "maps/s t2alm+alm2t @nside=1024 FP64 (t2alm niter=3 + alm2t round trip)",
+        "value": maps_per_s,
+        "unit": "maps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic: seeded Gaussian alm (C_l=1/(l(l+1))) -> alm2t + N(0,1e-3) noise, resident in HBM",
+        "config": {"workload": f"t2alm(niter={args.niter})+alm2t nside={args.nside} lmax={lmax} nsim={args.nsim} T maps",
+                   "nside": args.nside, "lmax": lmax, "nsim": args.nsim, "niter": args.niter,
+                   "parallelism": "1 GPU (maps shard with no communication)"},
+        "roofline": {"bound": "mfma", "kernel": dom, "achieved": achieved, "peak": PEAK_FP64_MFMA_TFLOPS,
+                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_MFMA_TFLOPS, "traffic": traffic,
+                     "flops_per_launch": flops_launch, "avg_launch_ms": avg_ms},
+        "kernels_ms_per_step": {k: v["ms"] / args.steps for k, v in kern.items()},
+        "wall_s": wall,
+        "roundtrip_alm_rel_err_vs_input": rel,
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline()
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

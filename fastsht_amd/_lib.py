@@ -9,7 +9,8 @@ import ctypes
 import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libfastsht.so")
+# FSHT_LIBRARY selects another build of the same library (e.g. a tuning variant)
+LIB_PATH = os.environ.get("FSHT_LIBRARY", os.path.join(_HERE, "libfastsht.so"))
 
 FSHT_OK = 0
 FSHT_ERR_ARG = -1
@@ -24,7 +25,7 @@ EXPORTS = (
     "fsht_version", "fsht_last_error", "fsht_create", "fsht_destroy", "fsht_get_stream",
     "fsht_set_stream", "fsht_synchronize", "fsht_info", "fsht_ring_tab", "fsht_t2alm",
     "fsht_get_alm", "fsht_alm2t", "fsht_qu2eb", "fsht_qu2e", "fsht_qu2b", "fsht_eb2qu",
-    "fsht_norm_alm", "fsht_filt_alm", "fsht_convert_alm_healpy", "fsht_convert_alm_healpy_inv",
+    "fsht_profile", "fsht_profile_read", "fsht_norm_alm", "fsht_filt_alm", "fsht_convert_alm_healpy", "fsht_convert_alm_healpy_inv",
 )
 
 _lib = None
@@ -68,6 +69,8 @@ def load():
     lib.fsht_qu2e.argtypes = [H, dp, dp, dp, i]
     lib.fsht_qu2b.argtypes = [H, dp, dp, dp, i]
     lib.fsht_eb2qu.argtypes = [H, dp, dp, dp, dp, i]
+    lib.fsht_profile.argtypes = [H, i]
+    lib.fsht_profile_read.argtypes = [H, ctypes.POINTER(ctypes.c_double), i]
     lib.fsht_norm_alm.argtypes = [dp, dp, dp, dp, d, i, i, vp]
     lib.fsht_filt_alm.argtypes = [dp, dp, i, i, vp]
     lib.fsht_convert_alm_healpy.argtypes = [dp, dp, i, i, vp]

@@ -9,7 +9,7 @@
 #include <cstring>
 #include <string>
 
-#include "../../include/fastsht.h"
+#include "fastsht.h"
 #include "fsht_internal.h"
 
 namespace fsht {
@@ -122,6 +122,11 @@ int fsht_destroy(fsht_handle* H) {
   Handle* h = &H->h;
   (void)hipStreamSynchronize(h->stream);
   free_all(h);
+  for (auto& t : h->timed) {
+    (void)hipEventDestroy(t.a);
+    (void)hipEventDestroy(t.b);
+  }
+  for (auto e : h->event_pool) (void)hipEventDestroy(e);
   (void)hipStreamDestroy(h->own_stream);
   delete H;
   return FSHT_OK;
@@ -164,6 +169,34 @@ int fsht_ring_tab(const fsht_handle* H, int* out) {
 }  // extern "C"
 
 // ---------------------------------------------------------------------------
+// Optional per-kernel-class timing: HIP events recorded on the handle's stream
+// around each launch (fsht_profile / fsht_profile_read).
+// ---------------------------------------------------------------------------
+static hipEvent_t get_event(Handle* h) {
+  if (!h->event_pool.empty()) {
+    hipEvent_t e = h->event_pool.back();
+    h->event_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  (void)hipEventCreate(&e);
+  return e;
+}
+
+template <class F>
+static hipError_t timed(Handle* h, int cls, F&& f) {
+  if (!h->profiling) return f();
+  hipEvent_t a = get_event(h), b = get_event(h);
+  hipError_t e = hipEventRecord(a, h->stream);
+  if (e != hipSuccess) return e;
+  e = f();
+  if (e != hipSuccess) return e;
+  e = hipEventRecord(b, h->stream);
+  h->timed.push_back(Handle::Timed{cls, a, b});
+  return e;
+}
+
+// ---------------------------------------------------------------------------
 // Orchestration helpers
 // ---------------------------------------------------------------------------
 static long long alm_count(const Handle* h) { return (long long)h->nsim * (h->lmax + 1) * (h->lmax + 1); }
@@ -184,21 +217,21 @@ static int forward_spectra(Handle* h, const double* m0, const double* m1, int lo
     in[0] = h->ws_spec;
     in[1] = h->ws_spec + cnt;
   }
-  HIPCHK(h, fsht::launch_ring_fft(h, in, h->ws_spec, 0, nullptr));
-  HIPCHK(h, fsht::launch_transpose_to_T(h, h->ws_spec, h->ws_specT));
+  HIPCHK(h, timed(h, fsht::KC_FFT_FWD, [&] { return fsht::launch_ring_fft(h, in, h->ws_spec, 0, nullptr); }));
+  HIPCHK(h, timed(h, fsht::KC_TRANSPOSE, [&] { return fsht::launch_transpose_to_T(h, h->ws_spec, h->ws_specT); }));
   return FSHT_OK;
 }
 
 // spectrum in ws_specT -> maps
 static int inverse_spectra(Handle* h, double* m0, double* m1, int loc) {
   const long long cnt = (long long)h->nsim * h->npix;
-  HIPCHK(h, fsht::launch_transpose_from_T(h, h->ws_specT, h->ws_spec));
+  HIPCHK(h, timed(h, fsht::KC_TRANSPOSE, [&] { return fsht::launch_transpose_from_T(h, h->ws_specT, h->ws_spec); }));
   double* out[2] = {m0, m1};
   if (!loc) {
     out[0] = h->ws_spec;
     out[1] = h->ws_spec + cnt;
   }
-  HIPCHK(h, fsht::launch_ring_fft(h, nullptr, h->ws_spec, 1, out));
+  HIPCHK(h, timed(h, fsht::KC_FFT_INV, [&] { return fsht::launch_ring_fft(h, nullptr, h->ws_spec, 1, out); }));
   if (!loc) {
     HIPCHK(h, hipMemcpyAsync(m0, h->ws_spec, cnt * sizeof(double), hipMemcpyDeviceToHost, h->stream));
     if (h->spin)
@@ -216,12 +249,14 @@ static int check_common(fsht_handle* H, int loc) {
 
 // analysis with optional Jacobi refinement; alm outputs are device pointers
 static int analysis_iter(Handle* h, double* a0, double* a1, int niter, int want0, int want1) {
-  HIPCHK(h, fsht::launch_analysis(h, h->ws_specT, a0, a1, 0.0, want0, want1));
+  HIPCHK(h, timed(h, fsht::KC_GATHER, [&] { return fsht::launch_gather(h, h->ws_specT, h->ws_G); }));
+  HIPCHK(h, timed(h, fsht::KC_ANALYSIS, [&] { return fsht::launch_analysis(h, h->ws_G, a0, a1, 0.0, want0, want1); }));
   if (niter > 0 && !h->ws_specR) HIPCHK(h, hipMalloc((void**)&h->ws_specR, (size_t)h->npix * (h->spin ? 2 : 1) * h->nsimp * sizeof(double)));
   for (int it = 0; it < niter; ++it) {
-    HIPCHK(h, fsht::launch_synthesis(h, a0, a1, h->ws_G));
-    HIPCHK(h, fsht::launch_fold(h, h->ws_G, h->ws_specT, h->ws_specR, 1));
-    HIPCHK(h, fsht::launch_analysis(h, h->ws_specR, a0, a1, 1.0, want0, want1));
+    HIPCHK(h, timed(h, fsht::KC_SYNTHESIS, [&] { return fsht::launch_synthesis(h, a0, a1, h->ws_G); }));
+    HIPCHK(h, timed(h, fsht::KC_FOLD, [&] { return fsht::launch_fold(h, h->ws_G, h->ws_specT, h->ws_specR, 1); }));
+    HIPCHK(h, timed(h, fsht::KC_GATHER, [&] { return fsht::launch_gather(h, h->ws_specR, h->ws_G); }));
+    HIPCHK(h, timed(h, fsht::KC_ANALYSIS, [&] { return fsht::launch_analysis(h, h->ws_G, a0, a1, 1.0, want0, want1); }));
   }
   return FSHT_OK;
 }
@@ -275,8 +310,8 @@ int fsht_alm2t(fsht_handle* H, const double* alms, double* maps, int loc) {
     a = h->ws_alm[0];
     h->have_dev_alm = 0;
   }
-  HIPCHK(h, fsht::launch_synthesis(h, a, nullptr, h->ws_G));
-  HIPCHK(h, fsht::launch_fold(h, h->ws_G, nullptr, h->ws_specT, 0));
+  HIPCHK(h, timed(h, fsht::KC_SYNTHESIS, [&] { return fsht::launch_synthesis(h, a, nullptr, h->ws_G); }));
+  HIPCHK(h, timed(h, fsht::KC_FOLD, [&] { return fsht::launch_fold(h, h->ws_G, nullptr, h->ws_specT, 0); }));
   if ((rc = inverse_spectra(h, maps, nullptr, loc))) return rc;
   if (!loc) HIPCHK(h, hipStreamSynchronize(h->stream));
   return FSHT_OK;
@@ -341,10 +376,38 @@ int fsht_eb2qu(fsht_handle* H, const double* E, const double* B, double* Q, doub
     b = h->ws_alm[1];
     h->have_dev_alm = 0;
   }
-  HIPCHK(h, fsht::launch_synthesis(h, e, b, h->ws_G));
-  HIPCHK(h, fsht::launch_fold(h, h->ws_G, nullptr, h->ws_specT, 0));
+  HIPCHK(h, timed(h, fsht::KC_SYNTHESIS, [&] { return fsht::launch_synthesis(h, e, b, h->ws_G); }));
+  HIPCHK(h, timed(h, fsht::KC_FOLD, [&] { return fsht::launch_fold(h, h->ws_G, nullptr, h->ws_specT, 0); }));
   if ((rc = inverse_spectra(h, Q, U, loc))) return rc;
   if (!loc) HIPCHK(h, hipStreamSynchronize(h->stream));
+  return FSHT_OK;
+}
+
+int fsht_profile(fsht_handle* H, int enable) {
+  if (!H) return fail(nullptr, FSHT_ERR_ARG, "NULL handle");
+  Handle* h = &H->h;
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  for (auto& t : h->timed) {
+    h->event_pool.push_back(t.a);
+    h->event_pool.push_back(t.b);
+  }
+  h->timed.clear();
+  h->profiling = enable ? 1 : 0;
+  return FSHT_OK;
+}
+
+int fsht_profile_read(fsht_handle* H, double* out, int n) {
+  if (!H || !out) return fail(nullptr, FSHT_ERR_ARG, "NULL argument");
+  Handle* h = &H->h;
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  double acc[2 * fsht::KC_COUNT] = {0};
+  for (auto& t : h->timed) {
+    float ms = 0.f;
+    HIPCHK(h, hipEventElapsedTime(&ms, t.a, t.b));
+    acc[2 * t.cls] += 1.0;
+    acc[2 * t.cls + 1] += ms;
+  }
+  for (int i = 0; i < n && i < 2 * fsht::KC_COUNT; ++i) out[i] = acc[i];
   return FSHT_OK;
 }
 

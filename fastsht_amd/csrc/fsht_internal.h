@@ -60,7 +60,8 @@ hipError_t launch_ring_fft(Handle* h, const double* const* maps_comp, double* sp
                            double* const* out_comp);
 hipError_t launch_transpose_to_T(Handle* h, const double* spec, double* specT);
 hipError_t launch_transpose_from_T(Handle* h, const double* specT, double* spec);
-hipError_t launch_analysis(Handle* h, const double* specT, double* alm0, double* alm1, double beta,
+hipError_t launch_gather(Handle* h, const double* specT, double* F);
+hipError_t launch_analysis(Handle* h, const double* F, double* alm0, double* alm1, double beta,
                            int want0, int want1);
 hipError_t launch_synthesis(Handle* h, const double* alm0, const double* alm1, double* G);
 hipError_t launch_fold(Handle* h, const double* G, const double* specT_t, double* specT_out,
@@ -129,6 +130,18 @@ struct Handle {
 
   // current call's spin (0 or 2)
   int spin = 0;
+
+  // optional per-kernel-class timing with HIP events on `stream`
+  int profiling = 0;
+  struct Timed {
+    int cls;
+    hipEvent_t a, b;
+  };
+  std::vector<Timed> timed;
+  std::vector<hipEvent_t> event_pool;
 };
+
+// kernel classes reported by fsht_profile_read
+enum KernelClass { KC_FFT_FWD = 0, KC_FFT_INV, KC_TRANSPOSE, KC_ANALYSIS, KC_SYNTHESIS, KC_FOLD, KC_GATHER, KC_COUNT };
 
 }  // namespace fsht

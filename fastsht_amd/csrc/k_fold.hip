@@ -110,6 +110,68 @@ __global__ __launch_bounds__(256) void k_fold(const RingGeo* __restrict__ geo, c
   }
 }
 
+// ---------------------------------------------------------------------------
+// Ring spectra -> m-major analysis operand (src/fft_mapping.f90:19-68,
+// fft2mat_fast, for all m at once).  For order m and kept north ring i:
+//   X_N = slot(m) of ring i rotated by e^{-i m phi0}, conjugated for negative
+//   frequencies (only the real part survives on DC / Nyquist slots),
+//   X_S the same on the mirror ring, then the parity folds
+//   F_even = X_N + X_S, F_odd = X_N - X_S   (equator: both = X_N).
+// Layout (same as the synthesis buffer): [m][fold][ring - i0(m)][comp][re/im][sim].
+// ---------------------------------------------------------------------------
+template <int NCOMP>
+__global__ __launch_bounds__(256) void k_gather(const RingGeo* __restrict__ geo, const int* __restrict__ i0tab,
+                                                const double2* __restrict__ tw, const int* __restrict__ slot,
+                                                const double* __restrict__ specT, double* __restrict__ F,
+                                                const long long* __restrict__ goff, int nhr, int nnorth,
+                                                int nsimp) {
+  const int m = blockIdx.x;
+  const int i0 = i0tab[m];
+  const int ldT = NCOMP * nsimp;
+  const int col = blockIdx.z * 64 + threadIdx.x;  // spectrum column comp*nsimp + sim
+  if (col >= ldT) return;
+  const int comp = col / nsimp, sim = col - comp * nsimp;
+  const int nr = nhr + 1 - i0;
+  const int ncols = 2 * NCOMP * nsimp;
+  const long long fb = goff[m] / (NCOMP == 2 ? 1 : 2);
+  const long long odd = (long long)nr * ncols;
+  const int cre = (comp * 2) * nsimp + sim, cim = cre + nsimp;
+  const long long mrow = (long long)m * nnorth;
+  for (int i = i0 + blockIdx.y * 16 + threadIdx.y; i <= nhr && i < i0 + (blockIdx.y + 1) * 16; i += 4) {
+    const int sinfo = slot[mrow + i];
+    const int rel = sinfo >> 2, cls = sinfo & 3;
+    const double sg = cls == 1 ? 1.0 : (cls == 2 ? -1.0 : 0.0);
+    const double2 cs = tw[mrow + i];
+    const RingGeo g = geo[i];
+    const double* pN = specT + (g.i1N + rel) * (long long)ldT + col;
+    const double* pS = specT + (g.i1S + rel) * (long long)ldT + col;
+    const double xr = pN[0], xi = pN[ldT], yr = pS[0], yi = pS[ldT];
+    const double Nr = xr * cs.x + sg * xi * cs.y;
+    const double Ni = sg * xi * cs.x - xr * cs.y;
+    const double Sr = yr * cs.x + sg * yi * cs.y;
+    const double Si = sg * yi * cs.x - yr * cs.y;
+    const double fs = (g.i1S != g.i1N) ? 1.0 : 0.0;
+    const long long row = fb + (long long)(i - i0) * ncols;
+    F[row + cre] = fma(fs, Sr, Nr);
+    F[row + cim] = fma(fs, Si, Ni);
+    F[row + odd + cre] = fma(-fs, Sr, Nr);
+    F[row + odd + cim] = fma(-fs, Si, Ni);
+  }
+}
+
+hipError_t launch_gather(Handle* h, const double* specT, double* F) {
+  const int ncomp = h->spin ? 2 : 1;
+  dim3 block(64, 4);
+  dim3 grid(h->lmax + 1, (h->nnorth + 15) / 16, (ncomp * h->nsimp + 63) / 64);
+  if (ncomp == 2)
+    hipLaunchKernelGGL(k_gather<2>, grid, block, 0, h->stream, h->d_geo, h->d_i0, h->d_tw, h->d_slot, specT, F,
+                       h->d_goff, h->nhr, h->nnorth, h->nsimp);
+  else
+    hipLaunchKernelGGL(k_gather<1>, grid, block, 0, h->stream, h->d_geo, h->d_i0, h->d_tw, h->d_slot, specT, F,
+                       h->d_goff, h->nhr, h->nnorth, h->nsimp);
+  return hipGetLastError();
+}
+
 hipError_t launch_fold(Handle* h, const double* G, const double* specT_t, double* specT_out, int residual) {
   const int ncomp = h->spin ? 2 : 1;
   const int maxk = 2 * h->nside;  // n/2 for the longest ring

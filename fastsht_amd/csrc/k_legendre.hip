@@ -15,6 +15,14 @@
 // v_mfma_f64_16x16x4_f64 operand layout (verified on gfx950):
 //   A[i][k] in lane i + 16k, B[k][j] in lane j + 16k,
 //   D[i][j] in lane j + 16*(i%4), accumulator register i/4.
+//
+// FP64 MFMA and FP64 VALU share one datapath on gfx950, so the recurrence is
+// kept lean: coefficients are staged in LDS, every global gather is
+// branch-free (clamped addresses, selected-away values) so loads are issued in
+// batches, and the seed injection (2 compares + 2 selects per step) only runs
+// in the 32-step chunks where some lane of the wave actually starts.
+#include <type_traits>
+
 #include "fsht_internal.h"
 
 namespace fsht {
@@ -66,14 +74,6 @@ __device__ inline Src syn_src(int spin, int otype, int f) {
   }
 }
 
-// One recurrence step with start-seed injection.
-__device__ inline double rec_step(double2 c, double x, double p1, double p2, int l, int st, double2 v) {
-  double p = fma(c.x * x, p1, -(c.y * p2));
-  p = (l == st) ? v.x : p;
-  p = (l == st + 1) ? v.y : p;
-  return p;
-}
-
 // Spin-2 functions from P_l and P_{l-1} (src/get_plm_recursive.f90:111-124).
 __device__ inline double spin_f(int f, double P, double Pm1, double2 sp, int l, int m, double x, double bw,
                                 double two_on_s2, double cw) {
@@ -86,32 +86,71 @@ __device__ inline double spin_f(int f, double P, double Pm1, double2 sp, int l, 
   return (Pm1 * lam - P * (2.0 * x * (dl - 1.0))) * cw * nl;
 }
 
+// Per-thread recurrence state for one ring.
+struct Rec {
+  double p1, p2;  // P_{l-1}, P_{l-2}
+  double x, bw, t2, cw;
+  int st;         // first non-negligible l (seeds injected at st, st+1)
+  double2 sv;     // P_st, P_st+1
+};
+
+// Run 32 recurrence steps l = lc .. lc+31 with coefficients cf[0..31]
+// (zero beyond lmax) and hand every value (P, or F_f for spin 2) to emit(s, v).
+template <int SPIN, bool INJ, class Emit>
+__device__ inline void gen32(Rec& r, const double2* cf, const double2* scf, int lc, int m, int f, Emit&& emit) {
+#pragma unroll
+  for (int s = 0; s < 32; ++s) {
+    const double2 c = cf[s];
+    double p = fma(c.x * r.x, r.p1, -(c.y * r.p2));
+    if (INJ) {
+      const int l = lc + s;
+      p = (l == r.st) ? r.sv.x : p;
+      p = (l == r.st + 1) ? r.sv.y : p;
+    }
+    double v = p;
+    if (SPIN) v = spin_f(f, p, r.p1, scf[s], lc + s, m, r.x, r.bw, r.t2, r.cw);
+    emit(s, v);
+    r.p2 = r.p1;
+    r.p1 = p;
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Analysis: alm[l in class par, cols] = pa * sum_{north rings} F(l, ring) * X(ring, cols)
-// Workgroup = (m, parity class, l-block of kAnaLB, 64 columns); 4 waves, each
-// owning 16 columns and all 8 l-tiles of the block.  Rings are consumed in
-// sub-blocks: the workgroup gathers its X columns for the sub-block into
-// registers (PERM slot gather + e^{-i m phi0} + north/south fold, i.e. the
-// reference's fft2mat), then sweeps the l-block generating P (one ring per
-// thread, plain FP64 recurrence restarted from the resident seeds) into LDS.
+// X is the m-major folded spectrum written by k_gather.  Workgroup = (m,
+// parity class, l-block of kAnaLB, 64 columns); 4 waves, each owning 16
+// columns and all 8 l-tiles of the block.  Rings are consumed in sub-blocks:
+// each wave loads its X columns for the sub-block into registers, then the
+// workgroup sweeps the l-block generating P (one ring per thread, plain FP64
+// recurrence restarted from the resident seeds) into LDS.
 // ---------------------------------------------------------------------------
+
+// Workgroups per CU the register allocation must allow (spin 0 / spin 2).
+#ifndef FSHT_ANA_MINBLK0
+#define FSHT_ANA_MINBLK0 2
+#endif
+#ifndef FSHT_SYN_MINBLK0
+#define FSHT_SYN_MINBLK0 2
+#endif
+
 template <int SPIN>
-__global__ __launch_bounds__(256) void k_analysis(
+__global__ __launch_bounds__(256, SPIN ? 1 : FSHT_ANA_MINBLK0) void k_analysis(
     const LegItem* __restrict__ items, const RingGeo* __restrict__ geo, const int* __restrict__ i0tab,
     const double2* __restrict__ ab, const double2* __restrict__ spc, const long long* __restrict__ aboff,
-    const double2* __restrict__ tw, const int* __restrict__ slot, const long long* __restrict__ seedoff,
-    const int* __restrict__ seed_st, const double2* __restrict__ seed_v, const double* __restrict__ specT,
-    double* __restrict__ alm0, double* __restrict__ alm1, int nsim, int nsimp, int lmax, int nhr, int nnorth,
-    double pa, double beta, int want0, int want1) {
+    const long long* __restrict__ seedoff, const int* __restrict__ seed_st, const double2* __restrict__ seed_v,
+    const double* __restrict__ F, const long long* __restrict__ goff, double* __restrict__ alm0,
+    double* __restrict__ alm1, int nsim, int nsimp, int lmax, int nhr, double pa, double beta, int want0,
+    int want1) {
   constexpr int RS = SPIN ? kAnaRings2 : kAnaRings0;  // rings per sub-block
   constexpr int NCOMP = SPIN ? 2 : 1;
   __shared__ double Pt[256][17];
+  __shared__ double2 cf[kAnaLB];
+  __shared__ double2 scf[SPIN ? kAnaLB : 1];
   const LegItem it = items[blockIdx.x];
   if (it.m < 0) return;
   const int m = it.m, par = it.par;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int li = lane & 15, q = lane >> 4;
-  const int ldT = NCOMP * nsimp;
   const int ncols = 2 * NCOMP * nsimp;
   const int i0 = i0tab[m];
   const int nr = nhr + 1 - i0;
@@ -119,109 +158,88 @@ __global__ __launch_bounds__(256) void k_analysis(
   const int L1 = min(L0 + kAnaLB, lmax + 1);
   const int nt = (L1 - L0 - par + 31) / 32;  // l-tiles in this block
   // this wave's output column tile
-  const int ct = it.colblk * 4 + wave;
-  const int c0 = ct * 16;
-  const int otype = c0 / nsimp;
-  const int sim0 = c0 - otype * nsimp;
+  const int c0 = (it.colblk * 4 + wave) * 16;
+  const bool colok = c0 < ncols;
+  const int otype = colok ? c0 / nsimp : 0;
+  const int sim0 = colok ? c0 - otype * nsimp : 0;
   const int ocomp = otype >> 1, oreim = otype & 1;
-  bool active = (c0 < ncols) && !(m == 0 && oreim == 1);
+  bool active = colok && !(m == 0 && oreim == 1);
   if (SPIN) active = active && (ocomp == 0 ? want0 : want1);
-  const long long mrow = (long long)m * nnorth;
-  const double2* abm = ab + aboff[m];
-  const double2* spm = spc + aboff[m];
+
+  // recurrence coefficients of the block (zero beyond lmax: P stays 0 there)
+  {
+    const double2* abm = ab + aboff[m];
+    const double2* spm = spc + aboff[m];
+    for (int k = tid; k < kAnaLB; k += 256) {
+      const int l = L0 + k;
+      cf[k] = (l <= lmax) ? abm[l - m] : make_double2(0.0, 0.0);
+      if (SPIN) scf[k] = (l <= lmax) ? spm[l - m] : make_double2(0.0, 0.0);
+    }
+  }
 
   d4 D[8];
 #pragma unroll
   for (int t = 0; t < 8; ++t) D[t] = d4{0.0, 0.0, 0.0, 0.0};
 
+  const int myf = SPIN ? (tid / RS) : 0;
+  const int myr = SPIN ? (tid % RS) : tid;
+
+  const long long fb = goff[m] / (SPIN ? 1 : 2);
   for (int rb = 0; rb < nr; rb += RS) {
-    // ---- gather B fragments (X columns) for the sub-block into registers
+    // ---- this thread's recurrence state (its ring of the sub-block)
+    Rec rec;
+    {
+      const int i = i0 + rb + myr;
+      const bool valid = i <= nhr;
+      const int ic = valid ? i : nhr;
+      const RingGeo g = geo[ic];
+      const long long so = seedoff[m] + (long long)it.blk * nr + (ic - i0);
+      rec.st = valid ? seed_st[so] : 0x3fffffff;
+      rec.sv = seed_v[so];
+      rec.x = g.z;
+      const double s2 = g.s * g.s;
+      rec.bw = g.z / s2;
+      rec.t2 = 2.0 / s2;
+      rec.cw = (double)m / s2;
+      rec.p1 = 0.0;
+      rec.p2 = 0.0;
+    }
+    // ---- B fragments: the wave's 16 columns of the folded, rotated ring
+    // spectra for the sub-block's rings (one coalesced row read per ring)
     double bfr[64];
 #pragma unroll
     for (int ks = 0; ks < 64; ++ks) {
       const int krow = 4 * ks + q;
-      const int f = SPIN ? (krow / RS) : 0;
+      const int f = SPIN ? (ks >= 32 ? 1 : 0) : 0;
       const int r = SPIN ? (krow % RS) : krow;
       const int i = i0 + rb + r;
-      double val = 0.0;
-      if (active && i <= nhr) {
-        const Src s = ana_src(SPIN, otype, f);
-        const int fold = par ^ s.flip;
-        const int sinfo = slot[mrow + i];
-        const int rel = sinfo >> 2, cls = sinfo & 3;
-        const double sg = cls == 1 ? 1.0 : (cls == 2 ? -1.0 : 0.0);
-        const double2 cs = tw[mrow + i];
-        const RingGeo g = geo[i];
-        const int col = s.comp * nsimp + sim0 + li;
-        const double* pN = specT + (g.i1N + rel) * (long long)ldT + col;
-        const double xr = pN[0];
-        const double xi = cls ? pN[ldT] : 0.0;
-        // e^{-i m phi0} rotation with conjugation for negative frequencies
-        double Fr = xr * cs.x + sg * xi * cs.y;
-        double Fi = sg * xi * cs.x - xr * cs.y;
-        if (g.i1S != g.i1N) {
-          const double* pS = specT + (g.i1S + rel) * (long long)ldT + col;
-          const double yr = pS[0];
-          const double yi = cls ? pS[ldT] : 0.0;
-          const double Gr = yr * cs.x + sg * yi * cs.y;
-          const double Gi = sg * yi * cs.x - yr * cs.y;
-          if (fold) {
-            Fr -= Gr;
-            Fi -= Gi;
-          } else {
-            Fr += Gr;
-            Fi += Gi;
-          }
-        }
-        val = s.sign * (s.reim ? Fi : Fr);
-      }
-      bfr[ks] = val;
+      const bool valid = i <= nhr;
+      const int ic = valid ? i : nhr;
+      const Src s = ana_src(SPIN, otype, f);
+      const int fold = par ^ s.flip;
+      const int col = (s.comp * 2 + s.reim) * nsimp + sim0 + li;
+      const double v = F[fb + ((long long)fold * nr + (ic - i0)) * ncols + col];
+      bfr[ks] = (active && valid) ? s.sign * v : 0.0;
     }
-
-    // ---- per-thread recurrence state for its (ring, function)
-    const int myf = SPIN ? (tid / RS) : 0;
-    const int myr = SPIN ? (tid % RS) : tid;
-    const int mi = i0 + rb + myr;
-    const bool ring_ok = mi <= nhr;
-    int st = 0x3fffffff;
-    double2 sv = make_double2(0.0, 0.0);
-    double x = 0.0, bw = 0.0, t2 = 0.0, cw = 0.0;
-    if (ring_ok) {
-      const long long so = seedoff[m] + (long long)it.blk * nr + (mi - i0);
-      st = seed_st[so];
-      sv = seed_v[so];
-      const RingGeo g = geo[mi];
-      x = g.z;
-      const double s2 = g.s * g.s;
-      bw = x / s2;
-      t2 = 2.0 / s2;
-      cw = (double)m / s2;
-    }
-    double p1 = 0.0, p2 = 0.0;
 
 #pragma unroll
     for (int t = 0; t < 8; ++t) {
       if (t < nt) {
-        // generate 32 l-steps, keep the 16 values of the parity class
         const int lc = L0 + 32 * t;
-#pragma unroll 4
-        for (int s = 0; s < 32; ++s) {
-          const int l = lc + s;
-          double p = 0.0;
-          if (l <= lmax) {
-            const double2 c = abm[l - m];
-            p = rec_step(c, x, p1, p2, l, st, sv);
-            if ((s & 1) == par) {
-              double val = p;
-              if (SPIN) val = spin_f(myf, p, p1, spm[l - m], l, m, x, bw, t2, cw);
-              Pt[tid][s >> 1] = val;
-            }
-          } else if ((s & 1) == par) {
-            Pt[tid][s >> 1] = 0.0;
+        // generate 32 l-steps; keep the 16 values of the parity class
+        double keep = 0.0;
+        auto emit = [&](int s, double v) {
+          if ((s & 1) == 0) {
+            keep = v;
+          } else {
+            Pt[tid][s >> 1] = par ? v : keep;
           }
-          p2 = p1;
-          p1 = p;
-        }
+        };
+        const bool inj = (rec.st >= lc - 1) && (rec.st <= lc + 31);
+        if (__any(inj))
+          gen32<SPIN, true>(rec, cf + 32 * t, scf + (SPIN ? 32 * t : 0), lc, m, myf, emit);
+        else
+          gen32<SPIN, false>(rec, cf + 32 * t, scf + (SPIN ? 32 * t : 0), lc, m, myf, emit);
         __syncthreads();
         if (active) {
 #pragma unroll
@@ -261,10 +279,11 @@ __global__ __launch_bounds__(256) void k_analysis(
 // ---------------------------------------------------------------------------
 // Synthesis: G[class c][ring][cols] = sum_{l} F(l, ring) * alm(l, cols)
 // Workgroup = (m, class, 256 rings, 64 columns); 4 waves, each owning 64 rings
-// (its own recurrence, one ring per lane) and the 64 columns.
+// (its own recurrence, one ring per lane) and the 64 columns.  The alm
+// operands of chunk c+1 are loaded while chunk c's recurrence runs.
 // ---------------------------------------------------------------------------
 template <int SPIN>
-__global__ __launch_bounds__(256) void k_synthesis(
+__global__ __launch_bounds__(256, SPIN ? 1 : FSHT_SYN_MINBLK0) void k_synthesis(
     const LegItem* __restrict__ items, const RingGeo* __restrict__ geo, const int* __restrict__ i0tab,
     const double2* __restrict__ ab, const double2* __restrict__ spc, const long long* __restrict__ aboff,
     const long long* __restrict__ seedoff, const int* __restrict__ seed_st, const double2* __restrict__ seed_v,
@@ -274,6 +293,8 @@ __global__ __launch_bounds__(256) void k_synthesis(
   constexpr int NV = SPIN ? 32 : 16;  // values per lane per 32-step chunk
   constexpr int NKS = NV / 4;         // K-steps per chunk
   __shared__ double Pt[4][64][NV + 1];
+  __shared__ double2 cf[2][32];
+  __shared__ double2 scf[2][SPIN ? 32 : 1];
   const LegItem it = items[blockIdx.x];
   if (it.m < 0) return;
   const int m = it.m, cls = it.par;
@@ -286,35 +307,36 @@ __global__ __launch_bounds__(256) void k_synthesis(
   const int myring = rbase + lane;
   const bool ring_ok = myring <= nhr;
   const bool wave_ok = rbase <= nhr;
+  const double2* abm = ab + aboff[m];
+  const double2* spm = spc + aboff[m];
+  const long long L = lmax + 1;
 
-  // column tiles of this workgroup
+  // column tiles of this workgroup and their alm sources per function
   int otype[4], sim0[4];
   bool cok[4];
 #pragma unroll
   for (int u = 0; u < 4; ++u) {
     const int c0 = (it.colblk * 4 + u) * 16;
-    otype[u] = c0 / nsimp;
-    sim0[u] = c0 - otype[u] * nsimp;
     cok[u] = c0 < ncols;
+    otype[u] = cok[u] ? c0 / nsimp : 0;
+    sim0[u] = cok[u] ? c0 - otype[u] * nsimp : 0;
   }
 
-  int st = 0x3fffffff;
-  double2 sv = make_double2(0.0, 0.0);
-  double x = 0.0, bw = 0.0, t2 = 0.0, cw = 0.0;
-  if (ring_ok) {
-    const long long so = seedoff[m] + (myring - i0);
-    st = seed_st[so];
-    sv = seed_v[so];
-    const RingGeo g = geo[myring];
-    x = g.z;
+  Rec rec;
+  {
+    const int ic = ring_ok ? myring : nhr;
+    const long long so = seedoff[m] + (ic - i0);
+    rec.st = ring_ok ? seed_st[so] : 0x3fffffff;
+    rec.sv = seed_v[so];
+    const RingGeo g = geo[ic];
+    rec.x = g.z;
     const double s2 = g.s * g.s;
-    bw = x / s2;
-    t2 = 2.0 / s2;
-    cw = (double)m / s2;
+    rec.bw = g.z / s2;
+    rec.t2 = 2.0 / s2;
+    rec.cw = (double)m / s2;
+    rec.p1 = 0.0;
+    rec.p2 = 0.0;
   }
-  const double2* abm = ab + aboff[m];
-  const double2* spm = spc + aboff[m];
-  const long long L = lmax + 1;
 
   d4 D[4][4];
 #pragma unroll
@@ -322,66 +344,128 @@ __global__ __launch_bounds__(256) void k_synthesis(
 #pragma unroll
     for (int b = 0; b < 4; ++b) D[a][b] = d4{0.0, 0.0, 0.0, 0.0};
 
-  double p1 = 0.0, p2 = 0.0;
-  for (int lc = m; lc <= lmax; lc += 32) {
-    // ---- generate this lane's ring over 32 l-steps
-#pragma unroll 4
-    for (int s = 0; s < 32; ++s) {
-      const int l = lc + s;
-      double p = 0.0;
-      const int pclass = s & 1;  // (l - m) parity
-      if (l <= lmax) {
-        p = rec_step(abm[l - m], x, p1, p2, l, st, sv);
-        if (SPIN) {
-          // F1 in slots [0,16) for l in class cls, F2 in [16,32) for the other class
-          const int f = (pclass == cls) ? 0 : 1;
-          Pt[wave][lane][f * 16 + (s >> 1)] = spin_f(f, p, p1, spm[l - m], l, m, x, bw, t2, cw);
-        } else if (pclass == cls) {
-          Pt[wave][lane][s >> 1] = p;
-        }
-      } else {
-        if (SPIN)
-          Pt[wave][lane][((pclass == cls) ? 0 : 16) + (s >> 1)] = 0.0;
-        else if (pclass == cls)
-          Pt[wave][lane][s >> 1] = 0.0;
+  // branch-free alm operand load for K-step ks of the chunk starting at lc
+  auto load_b = [&](int lc, double (&bf)[NKS][4]) {
+#pragma unroll
+    for (int ks = 0; ks < NKS; ++ks) {
+      const int f = SPIN ? (ks / 4) : 0;
+      const int vi = 4 * (ks % 4) + q;
+      const int pc = (f == 0) ? cls : (cls ^ 1);
+      const int l = lc + pc + 2 * vi;
+      const bool lok = l <= lmax;
+      const int lcl = lok ? l : lmax;
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const Src s = syn_src(SPIN, otype[u], f);
+        const int sim = sim0[u] + li;
+        // Im at m = 0 has no storage: read the (valid) Re address and drop it
+        const bool im = s.reim && m > 0;
+        const bool ok = cok[u] && lok && sim < nsim && !(m == 0 && s.reim);
+        const int simc = sim < nsim ? sim : nsim - 1;
+        const double* alm = s.comp ? alm1 : alm0;
+        const long long idx = im ? (long long)simc + nsim * ((lcl - m) + L * (L - m))
+                                 : (long long)simc + nsim * (lcl + L * m);
+        const double v = alm[idx];
+        bf[ks][u] = ok ? s.sign * v : 0.0;
       }
-      p2 = p1;
-      p1 = p;
+    }
+  };
+  auto load_cf = [&](int lc, int buf) {
+    if (tid < 32) {
+      const int l = lc + tid;
+      cf[buf][tid] = (l <= lmax) ? abm[l - m] : make_double2(0.0, 0.0);
+      if (SPIN) scf[buf][tid] = (l <= lmax) ? spm[l - m] : make_double2(0.0, 0.0);
+    }
+  };
+
+  double bcur[NKS][4], bnext[NKS][4];
+  load_b(m, bcur);
+  load_cf(m, 0);
+  __syncthreads();
+  int buf = 0;
+  for (int lc = m; lc <= lmax; lc += 32, buf ^= 1) {
+    const bool more = lc + 32 <= lmax;
+    if (more) {
+      load_b(lc + 32, bnext);
+      load_cf(lc + 32, buf ^ 1);
+    }
+    // ---- recurrence for this lane's ring over 32 l-steps; step s has (l - m)
+    // parity s & 1, so each (even, odd) pair of steps yields one value per class
+    double keep = 0.0;
+    auto emit = [&](int s, double v) {
+      if ((s & 1) == 0)
+        keep = v;
+      else
+        Pt[wave][lane][s >> 1] = cls ? v : keep;
+    };
+    if (SPIN) {
+      // F1 uses class cls, F2 the other class: generate both functions
+      double k1 = 0.0, k2 = 0.0;
+      auto emit2 = [&](int s, double v1, double v2) {
+        if ((s & 1) == 0) {
+          k1 = v1;
+          k2 = v2;
+        } else {
+          Pt[wave][lane][(s >> 1)] = cls ? v1 : k1;            // F1, class cls
+          Pt[wave][lane][16 + (s >> 1)] = cls ? k2 : v2;       // F2, other class
+        }
+      };
+      const bool inj = (rec.st >= lc - 1) && (rec.st <= lc + 31);
+      const double2* c = cf[buf];
+      const double2* sc = scf[buf];
+      auto body = [&](auto injc) {
+        constexpr bool INJ = decltype(injc)::value;
+#pragma unroll
+        for (int s = 0; s < 32; ++s) {
+          const double2 cc = c[s];
+          double p = fma(cc.x * rec.x, rec.p1, -(cc.y * rec.p2));
+          if (INJ) {
+            const int l = lc + s;
+            p = (l == rec.st) ? rec.sv.x : p;
+            p = (l == rec.st + 1) ? rec.sv.y : p;
+          }
+          const double f1 = spin_f(0, p, rec.p1, sc[s], lc + s, m, rec.x, rec.bw, rec.t2, rec.cw);
+          const double f2 = spin_f(1, p, rec.p1, sc[s], lc + s, m, rec.x, rec.bw, rec.t2, rec.cw);
+          emit2(s, f1, f2);
+          rec.p2 = rec.p1;
+          rec.p1 = p;
+        }
+      };
+      if (__any(inj))
+        body(std::true_type{});
+      else
+        body(std::false_type{});
+    } else {
+      const bool inj = (rec.st >= lc - 1) && (rec.st <= lc + 31);
+      if (__any(inj))
+        gen32<0, true>(rec, cf[buf], nullptr, lc, m, 0, emit);
+      else
+        gen32<0, false>(rec, cf[buf], nullptr, lc, m, 0, emit);
     }
     __syncthreads();
     if (wave_ok) {
+      // one ring tile at a time: its 4 accumulators rotate through the chunk's
+      // K-steps.  FP64 MFMA throughput drops when many distinct accumulators
+      // are cycled (register-file reads of srcC); 4 in rotation keep it high.
 #pragma unroll
-      for (int ks = 0; ks < NKS; ++ks) {
-        const int f = SPIN ? (ks / 4) : 0;
-        const int vi = 4 * (ks % 4) + q;  // value index within the function's 16
-        // l of this K-row: class cls for f=0 (P or F1), the other class for F2
-        const int pc = (f == 0) ? cls : (cls ^ 1);
-        const int l = lc + pc + 2 * vi;
-        double bf[4];
+      for (int a = 0; a < 4; ++a) {
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          double val = 0.0;
-          if (cok[u] && l <= lmax) {
-            const Src s = syn_src(SPIN, otype[u], f);
-            const int sim = sim0[u] + li;
-            if (sim < nsim && !(m == 0 && s.reim)) {
-              const double* alm = s.comp ? alm1 : alm0;
-              const long long idx = s.reim ? (long long)sim + nsim * ((l - m) + L * (L - m))
-                                           : (long long)sim + nsim * (l + L * m);
-              val = s.sign * alm[idx];
-            }
-          }
-          bf[u] = val;
-        }
-#pragma unroll
-        for (int a = 0; a < 4; ++a) {
+        for (int ks = 0; ks < NKS; ++ks) {
+          const int f = SPIN ? (ks / 4) : 0;
           const double av = Pt[wave][16 * a + li][f * 16 + 4 * (ks % 4) + q];
 #pragma unroll
-          for (int u = 0; u < 4; ++u) D[a][u] = mfma64(av, bf[u], D[a][u]);
+          for (int u = 0; u < 4; ++u) D[a][u] = mfma64(av, bcur[ks][u], D[a][u]);
         }
+        __builtin_amdgcn_sched_barrier(0);
       }
     }
     __syncthreads();
+    if (more) {
+#pragma unroll
+      for (int ks = 0; ks < NKS; ++ks)
+#pragma unroll
+        for (int u = 0; u < 4; ++u) bcur[ks][u] = bnext[ks][u];
+    }
   }
 
   if (!wave_ok) return;
@@ -402,18 +486,18 @@ __global__ __launch_bounds__(256) void k_synthesis(
   }
 }
 
-hipError_t launch_analysis(Handle* h, const double* specT, double* alm0, double* alm1, double beta, int want0,
+hipError_t launch_analysis(Handle* h, const double* F, double* alm0, double* alm1, double beta, int want0,
                            int want1) {
   const int sp = h->spin ? 1 : 0;
   dim3 grid(h->n_ana_items[sp]);
   if (sp)
     hipLaunchKernelGGL(k_analysis<2>, grid, dim3(256), 0, h->stream, h->d_ana_items[1], h->d_geo, h->d_i0, h->d_ab,
-                       h->d_spin, h->d_aboff, h->d_tw, h->d_slot, h->d_seedoff, h->d_seed_st, h->d_seed_v, specT,
-                       alm0, alm1, h->nsim, h->nsimp, h->lmax, h->nhr, h->nnorth, h->pa, beta, want0, want1);
+                       h->d_spin, h->d_aboff, h->d_seedoff, h->d_seed_st, h->d_seed_v, F, h->d_goff, alm0, alm1,
+                       h->nsim, h->nsimp, h->lmax, h->nhr, h->pa, beta, want0, want1);
   else
     hipLaunchKernelGGL(k_analysis<0>, grid, dim3(256), 0, h->stream, h->d_ana_items[0], h->d_geo, h->d_i0, h->d_ab,
-                       h->d_spin, h->d_aboff, h->d_tw, h->d_slot, h->d_seedoff, h->d_seed_st, h->d_seed_v, specT,
-                       alm0, alm1, h->nsim, h->nsimp, h->lmax, h->nhr, h->nnorth, h->pa, beta, 1, 0);
+                       h->d_spin, h->d_aboff, h->d_seedoff, h->d_seed_st, h->d_seed_v, F, h->d_goff, alm0, alm1,
+                       h->nsim, h->nsimp, h->lmax, h->nhr, h->pa, beta, 1, 0);
   return hipGetLastError();
 }
 

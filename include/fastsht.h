@@ -98,6 +98,15 @@ int fsht_qu2b(fsht_handle* h, const double* Q, const double* U, double* B, int l
  * / SHT.eb2qu (SHT.py:179-190). */
 int fsht_eb2qu(fsht_handle* h, const double* E, const double* B, double* Q, double* U, int loc);
 
+/* Per-kernel-class timing with HIP events on the handle's stream.
+ * fsht_profile(h, 1) starts recording (and clears earlier records); classes:
+ * 0 ring FFT forward, 1 ring FFT inverse, 2 transposes, 3 Legendre analysis,
+ * 4 Legendre synthesis, 5 alias fold / Jacobi residual, 6 spectrum gather
+ * into the m-major analysis operand.
+ * fsht_profile_read fills out[2c] = launches, out[2c+1] = total ms (c < 7). */
+int fsht_profile(fsht_handle* h, int enable);
+int fsht_profile_read(fsht_handle* h, double* out, int n);
+
 /* alm utilities on device arrays (stream: hipStream_t or NULL). */
 /* out = in*fac(l) + shift(l), with an extra factor m0 on m = 0.
  * Replaces norm_alm1 (src/norm_alm.f90:14-33). */

@@ -353,28 +353,34 @@ class OracleSHT:
         return c1, c2
 
     def mat2fft(self, buf, m, c1, c2):
-        """fft_mapping.f90:84-128: scatter with set (first touch) / accumulate."""
+        """fft_mapping.f90:84-128: scatter with set (first touch) / accumulate.
+        Within one m every ring owns a distinct slot pair, so the ring loop is
+        vectorised."""
         i0, i1 = self._loc(m)
-        for i in range(i0, i1 + 1):
-            j = int(self.tab[i, m])
-            jf = min(i, self.nring - 1 - i)
-            fac = 1.0 if 2 * (m + 1) > self.nfft[i] else 0.0
-            v0 = c1[:, i] * self.fac1I[jf, m] + c2[:, i] * self.fac2I[jf, m]
-            v1 = c1[:, i] * self.fac3I[jf, m] + c2[:, i] * self.fac4I[jf, m]
-            buf[:, j] = buf[:, j] * fac + v0
-            buf[:, j + 1] = buf[:, j + 1] * fac + v1
+        rings = np.arange(i0, i1 + 1)
+        j = self.tab[rings, m]
+        jf = np.minimum(rings, self.nring - 1 - rings)
+        fac = np.where(2 * (m + 1) > self.nfft[rings], 1.0, 0.0)[None, :]
+        a, b = c1[:, rings], c2[:, rings]
+        v0 = a * self.fac1I[jf, m] + b * self.fac2I[jf, m]
+        v1 = a * self.fac3I[jf, m] + b * self.fac4I[jf, m]
+        buf[:, j] = buf[:, j] * fac + v0
+        buf[:, j + 1] = buf[:, j + 1] * fac + v1
 
     def mat2fft_iter(self, buf1, buf2, m, c1, c2):
         """fft_mapping.f90:142-190: residual X2 = X1 - nfft*synth (first touch)
-        or X2 -= nfft*synth (aliased)."""
+        or X2 -= nfft*synth (aliased); vectorised over rings as above."""
         i0, i1 = self._loc(m)
-        for i in range(i0, i1 + 1):
-            j = int(self.tab[i, m])
-            jf = min(i, self.nring - 1 - i)
-            nf = float(self.nfft[i])
-            src0 = buf2 if 2 * (m + 1) > self.nfft[i] else buf1
-            buf2[:, j] = src0[:, j] - c1[:, i] * self.fac1I[jf, m] * nf - c2[:, i] * self.fac2I[jf, m] * nf
-            buf2[:, j + 1] = src0[:, j + 1] - c1[:, i] * self.fac3I[jf, m] * nf - c2[:, i] * self.fac4I[jf, m] * nf
+        rings = np.arange(i0, i1 + 1)
+        j = self.tab[rings, m]
+        jf = np.minimum(rings, self.nring - 1 - rings)
+        nf = self.nfft[rings].astype(np.float64)
+        alias = (2 * (m + 1) > self.nfft[rings])[None, :]
+        a, b = c1[:, rings], c2[:, rings]
+        s0 = np.where(alias, buf2[:, j], buf1[:, j])
+        s1 = np.where(alias, buf2[:, j + 1], buf1[:, j + 1])
+        buf2[:, j] = s0 - a * self.fac1I[jf, m] * nf - b * self.fac2I[jf, m] * nf
+        buf2[:, j + 1] = s1 - a * self.fac3I[jf, m] * nf - b * self.fac4I[jf, m] * nf
 
     def zero_unused(self, buf):
         """sht_main.f90:137-146: zero PERM slots >= 2(lmax+1)."""
