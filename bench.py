@@ -66,38 +66,38 @@ def make_inputs(sht, seed):
     return alm, maps
 
 
-def cpu_baseline(seconds_budget=20.0):
-    """The oracle (CPU FP64 restatement of the reference algorithm: scipy/numpy
-    FFT, OpenBLAS dgemm per m, C Legendre recursion) on a bounded sample of the
-    same workload: nside=1024, lmax=2047, one map, one forward pass
-    (t2alm niter=0) and one backward pass (alm2t).  A t2alm(niter=3)+alm2t
-    round trip costs 4 forward + 4 backward Legendre passes, so maps/s =
-    1 / (4 * (t_fwd + t_bwd)) (the two ring-FFT passes of the round trip are
-    included in t_fwd and t_bwd once each; the 3 extra FFT-free passes are
-    counted at full pass cost, an upper bound on the CPU time)."""
+def cpu_baseline(nsim=16, mstride=32):
+    """The oracle (CPU FP64 restatement of the reference algorithm: numpy ring
+    FFTs, OpenBLAS dgemm per m, C Legendre recursion) on a bounded sample of the
+    same workload (nside=1024, lmax=2047): `nsim` maps, the full forward and
+    inverse ring FFTs, and the per-m analysis and synthesis passes for every
+    `mstride`-th order m.  The per-m pass times are scaled to all m by the
+    per-m work (lmax+1-m) * (4*nside-1 - 2*i0(m)); the round trip is
+    fft_fwd + 4 analysis passes + 4 synthesis passes + fft_inv."""
     import numpy as np
     from threadpoolctl import threadpool_info
 
     from oracle import oracle as O
 
     nside, lmax = 1024, 2047
-    S = O.OracleSHT(nside, lmax, 1)
+    S = O.OracleSHT(nside, lmax, nsim)
     rng = np.random.default_rng(0)
-    maps = np.asfortranarray(rng.standard_normal((12 * nside * nside, 1)))
-    t0 = time.perf_counter()
-    a = S.t2alm(maps, niter=0)
-    t1 = time.perf_counter()
-    S.alm2t(a)
-    t2 = time.perf_counter()
-    tf, tb = t1 - t0, t2 - t1
+    maps = np.asfortranarray(rng.standard_normal((12 * nside * nside, nsim)))
+    ms = list(range(0, lmax + 1, mstride))
+    t = S.timed_sample(maps, ms)
+    w = (lmax + 1 - np.arange(lmax + 1)) * (S.nring - 2 * S.rtab)
+    scale = float(w.sum() / w[ms].sum())
+    total = t["fft_fwd"] + t["fft_inv"] + 4.0 * scale * (t["analysis"] + t["synthesis"])
     blas = [p.get("num_threads", 1) for p in threadpool_info() if p.get("user_api") == "blas"]
     return {
-        "value": 1.0 / (4.0 * (tf + tb)),
+        "value": nsim / total,
         "unit": "maps/s",
         "cores": int(max(blas) if blas else 1),
         "kind": "port",
-        "sample": (f"oracle (numpy FFT + OpenBLAS dgemm + C recursion), nside=1024 lmax=2047, 1 map: "
-                   f"t2alm(niter=0) {tf:.1f}s + alm2t {tb:.1f}s; t2alm(niter=3)+alm2t = 4x(fwd+bwd) passes"),
+        "sample": (f"oracle (numpy FFT + OpenBLAS dgemm + C recursion) at nside=1024 lmax=2047, {nsim} maps: "
+                   f"ring FFT fwd {t['fft_fwd']:.1f}s + inv {t['fft_inv']:.1f}s, analysis {t['analysis']:.1f}s and "
+                   f"synthesis {t['synthesis']:.1f}s over every {mstride}th m scaled x{scale:.2f} by per-m work; "
+                   f"round trip = FFTs + 4 analysis + 4 synthesis passes"),
         "cpu": _cpu_model(),
     }
 
@@ -123,7 +123,24 @@ def main():
     ap.add_argument("--niter", type=int, default=3)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-only", action="store_true", help="only the CPU baseline leg")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a device copy of MIB MiB (HBM-counter calibration under rocprofv3)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+
+        n = args.copy_calibration * (1 << 20) // 8
+        src = torch.ones(n, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": n * 8, "copy_bytes_written": n * 8, "launches": 3}))
+        return
+    if args.cpu_only:
+        print(json.dumps({"cpu_baseline": cpu_baseline()}))
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); "
               "the transform shards over maps with no communication, see DESIGN.md", file=sys.stderr)

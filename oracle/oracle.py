@@ -478,6 +478,33 @@ class OracleSHT:
         self.zero_unused(b1)
         return self.fft2map(b1)
 
+    def timed_sample(self, maps, ms):
+        """Wall-clock parts of one temperature round trip on a sample of orders
+        (CPU-baseline helper): full forward ring FFT, per-m analysis
+        (get_plm + fft2mat + mat2alm) over `ms`, per-m synthesis (get_plm +
+        alm2mat + mat2fft) over `ms`, full inverse ring FFT.  Returns seconds."""
+        import time
+        nsim = maps.shape[1]
+        t0 = time.perf_counter()
+        b1 = self.map2fft(maps)
+        t1 = time.perf_counter()
+        alm, af = self._new_alm(nsim)
+        for m in ms:
+            (P,) = self.plm(m)
+            c1, c2 = self.fft2mat(b1, m)
+            self.mat2alm(af, P, m, 0, 0.0, c1, c2)
+        t2 = time.perf_counter()
+        b2 = np.zeros_like(b1)
+        c1, c2 = self._caches(nsim)
+        for m in ms:
+            (P,) = self.plm(m)
+            self.alm2mat(af, P, m, 0, 0.0, c1, c2)
+            self.mat2fft(b2, m, c1, c2)
+        t3 = time.perf_counter()
+        self.fft2map(b2)
+        t4 = time.perf_counter()
+        return {"fft_fwd": t1 - t0, "analysis": t2 - t1, "synthesis": t3 - t2, "fft_inv": t4 - t3}
+
     def _qu_forward(self, bq, bu, afE, afB, beta, want_e=True, want_b=True):
         """The per-m body shared by QU2EB (:409-430) and qu2eb_iter_old (:596-617, :645-666)."""
         for m in range(self.lmax + 1):
