@@ -127,7 +127,7 @@ __device__ inline void gen32(Rec& r, const double2* cf, const double2* scf, int 
 
 // Workgroups per CU the register allocation must allow (spin 0 / spin 2).
 #ifndef FSHT_ANA_MINBLK0
-#define FSHT_ANA_MINBLK0 2
+#define FSHT_ANA_MINBLK0 1
 #endif
 #ifndef FSHT_SYN_MINBLK0
 #define FSHT_SYN_MINBLK0 2
