@@ -43,7 +43,7 @@ static int fail(Handle* h, int code, const std::string& msg) {
 
 static void free_all(Handle* h) {
   void* ptrs[] = {h->d_geo, h->d_i0, h->d_ab, h->d_spin, h->d_aboff, h->d_tw, h->d_slot, h->d_seedoff,
-                  h->d_seed_st, h->d_seed_v, h->d_goff, h->d_ana_items[0], h->d_ana_items[1],
+                  h->d_seed_st, h->d_seed_v, h->d_synoff, h->d_syn_st, h->d_syn_v, h->d_goff, h->d_ana_items[0], h->d_ana_items[1],
                   h->d_syn_items[0], h->d_syn_items[1], h->d_fft_jobs, h->d_blue, h->d_chirp, h->d_filt,
                   h->d_twid, h->ws_spec, h->ws_specT, h->ws_specR, h->ws_G, h->ws_alm[0], h->ws_alm[1]};
   for (void* p : ptrs)
@@ -69,6 +69,9 @@ int fsht_create(int nside, int lmax, int nsim, int niter, int pol, fsht_handle**
   if (lmax < 0 || lmax > 4 * nside)
     return fail(nullptr, FSHT_ERR_ARG, "lmax must be in [0, 4*nside]");
   if (nsim < 1) return fail(nullptr, FSHT_ERR_ARG, "nsim must be >= 1");
+  // the Legendre kernels address one alm column (nsim x (lmax+1) doubles) with 32-bit byte offsets
+  if ((long long)nsim * (lmax + 1) >= (1LL << 28))
+    return fail(nullptr, FSHT_ERR_ARG, "nsim*(lmax+1) must be below 2^28");
   if (niter < 0) return fail(nullptr, FSHT_ERR_ARG, "niter must be >= 0");
   fsht_handle* H = new fsht_handle();
   Handle* h = &H->h;

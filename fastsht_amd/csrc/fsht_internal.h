@@ -38,7 +38,7 @@ struct LegItem {
   int m;
   int par;     // parity / symmetry class (0: l+m even, 1: odd)
   int blk;     // analysis: l-block; synthesis: ring block
-  int colblk;  // column block
+  int colblk;  // analysis: block of kAnaCols columns; synthesis: first 16-column tile
 };
 
 // Tunables (also used by the host to size work lists).
@@ -48,7 +48,8 @@ constexpr int kAnaRings2 = 128;   // analysis ring sub-block (spin 2, x2 functio
 constexpr int kAnaLB = 256;       // analysis l-block (l steps)
 constexpr int kAnaCols = 64;      // analysis columns per workgroup (4 waves x 16)
 constexpr int kSynRings = 256;    // synthesis rings per workgroup (4 waves x 64)
-constexpr int kSynCols = 64;      // synthesis columns per workgroup
+constexpr int kSynCols = 64;      // synthesis columns per workgroup (spin 0; spin 2 uses 32)
+constexpr int kSegL = 256;        // synthesis: recurrence coefficients staged per wave (l values)
 constexpr int kFlushExp = -899;   // seeds: a value p*2^e is kept iff frexp-exponent(p)+e >= this
 
 struct Handle;
@@ -85,6 +86,7 @@ struct Handle {
   std::vector<int> i0;          // ring_tab per m (0..lmax)
   std::vector<long long> aboff; // offset of order m in the (a,b) table
   std::vector<long long> seedoff;
+  std::vector<long long> synoff;  // offset of order m in the per-(m, ring) synthesis seeds
   std::vector<int> nblk;        // analysis l-blocks per m
   std::vector<long long> goff;  // offset (in doubles) of order m in the m-major synthesis buffer
 
@@ -99,6 +101,9 @@ struct Handle {
   long long* d_seedoff = nullptr;
   int* d_seed_st = nullptr;
   double2* d_seed_v = nullptr;
+  long long* d_synoff = nullptr;
+  int* d_syn_st = nullptr;      // first l whose P_lm is kept, per (m, ring): the synthesis start
+  double2* d_syn_v = nullptr;   // (P_st, P_st+1) at that start
   long long* d_goff = nullptr;
   LegItem* d_ana_items[2] = {nullptr, nullptr};  // per spin (0, 2)
   LegItem* d_syn_items[2] = {nullptr, nullptr};

@@ -129,10 +129,6 @@ __device__ inline void gen32(Rec& r, const double2* cf, const double2* scf, int 
 #ifndef FSHT_ANA_MINBLK0
 #define FSHT_ANA_MINBLK0 1
 #endif
-#ifndef FSHT_SYN_MINBLK0
-#define FSHT_SYN_MINBLK0 2
-#endif
-
 template <int SPIN>
 __global__ __launch_bounds__(256, SPIN ? 1 : FSHT_ANA_MINBLK0) void k_analysis(
     const LegItem* __restrict__ items, const RingGeo* __restrict__ geo, const int* __restrict__ i0tab,
@@ -277,57 +273,68 @@ __global__ __launch_bounds__(256, SPIN ? 1 : FSHT_ANA_MINBLK0) void k_analysis(
 }
 
 // ---------------------------------------------------------------------------
-// Synthesis: G[class c][ring][cols] = sum_{l} F(l, ring) * alm(l, cols)
-// Workgroup = (m, class, 256 rings, 64 columns); 4 waves, each owning 64 rings
-// (its own recurrence, one ring per lane) and the 64 columns.  The alm
-// operands of chunk c+1 are loaded while chunk c's recurrence runs.
+// Synthesis: G[class c][ring][cols] = sum_{l in class} F(l, ring) * alm(l, cols)
+//
+// Workgroup = (m, class, 256 rings, NT 16-column tiles); its 4 waves are
+// independent (64 rings each, one ring per lane) and never synchronise: each
+// wave stages its own recurrence coefficients and its own P tile in LDS, so
+// the two waves a SIMD holds drift apart and one's recurrence runs while the
+// other's MFMAs do.  Per 32-step l chunk a wave
+//   1. issues the global loads of the next chunk's alm operand (registers),
+//   2. runs the plain FP64 recurrence for its rings (started at the ring's
+//      first kept l from the resident seeds), keeping the values of its class
+//      (spin 2: F1 of its class and F2 of the other one) in LDS,
+//   3. contracts them with 4 x NT x NKS v_mfma_f64_16x16x4 (A = P from LDS,
+//      B = alm from registers), each accumulator's K chain back to back.
 // ---------------------------------------------------------------------------
+#ifndef FSHT_SYN_MINBLK0
+#define FSHT_SYN_MINBLK0 2
+#endif
+
+// Code-motion barrier for LDS traffic between the lanes of one wave: a
+// wave's LDS instructions execute in order, so no wait is needed, but the
+// compiler must not move accesses across it.
+__device__ inline void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 template <int SPIN>
-__global__ __launch_bounds__(256, SPIN ? 1 : FSHT_SYN_MINBLK0) void k_synthesis(
-    const LegItem* __restrict__ items, const RingGeo* __restrict__ geo, const int* __restrict__ i0tab,
-    const double2* __restrict__ ab, const double2* __restrict__ spc, const long long* __restrict__ aboff,
-    const long long* __restrict__ seedoff, const int* __restrict__ seed_st, const double2* __restrict__ seed_v,
-    const double* __restrict__ alm0, const double* __restrict__ alm1, double* __restrict__ G,
-    const long long* __restrict__ goff, int nsim, int nsimp, int lmax, int nhr) {
-  constexpr int NCOMP = SPIN ? 2 : 1;
-  constexpr int NV = SPIN ? 32 : 16;  // values per lane per 32-step chunk
-  constexpr int NKS = NV / 4;         // K-steps per chunk
-  __shared__ double Pt[4][64][NV + 1];
-  __shared__ double2 cf[2][32];
-  __shared__ double2 scf[2][SPIN ? 32 : 1];
-  const LegItem it = items[blockIdx.x];
-  if (it.m < 0) return;
-  const int m = it.m, cls = it.par;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+struct SynShared {
+  static constexpr int NV = SPIN ? 32 : 16;  // LDS values per ring per chunk
+  double Pt[4][64][NV + 1];
+  double2 cf[4][kSegL];
+  double2 sf[4][SPIN ? kSegL : 1];
+};
+
+template <int SPIN, int NT, int CLS>
+__device__ inline void synth_wave(SynShared<SPIN>& sh, const LegItem it, const RingGeo* __restrict__ geo,
+                                  const double2* __restrict__ abm, const double2* __restrict__ spm,
+                                  const int* __restrict__ syn_st, const double2* __restrict__ syn_v,
+                                  long long ys0, const double* __restrict__ alm0, const double* __restrict__ alm1,
+                                  double* __restrict__ G, long long gbase, int i0, int nsim, int nsimp, int lmax,
+                                  int nhr, int rbase) {
+  constexpr int NV = SynShared<SPIN>::NV;
+  constexpr int NKS = NV / 4;  // K-steps per chunk
+  constexpr int NF = SPIN ? 2 : 1;
+  const int m = it.m;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int li = lane & 15, q = lane >> 4;
-  const int ncols = 2 * NCOMP * nsimp;
-  const int i0 = i0tab[m];
-  const int nr = nhr + 1 - i0;
-  const int rbase = i0 + it.blk * kSynRings + wave * 64;
-  const int myring = rbase + lane;
-  const bool ring_ok = myring <= nhr;
-  const bool wave_ok = rbase <= nhr;
-  const double2* abm = ab + aboff[m];
-  const double2* spm = spc + aboff[m];
+  const int ncols = 2 * (SPIN ? 2 : 1) * nsimp;
   const long long L = lmax + 1;
+  double (*Pt)[NV + 1] = sh.Pt[wave];
+  double2* cf = sh.cf[wave];
+  double2* sf = sh.sf[wave];
 
-  // column tiles of this workgroup and their alm sources per function
-  int otype[4], sim0[4];
-  bool cok[4];
-#pragma unroll
-  for (int u = 0; u < 4; ++u) {
-    const int c0 = (it.colblk * 4 + u) * 16;
-    cok[u] = c0 < ncols;
-    otype[u] = cok[u] ? c0 / nsimp : 0;
-    sim0[u] = cok[u] ? c0 - otype[u] * nsimp : 0;
-  }
-
+  // ---- recurrence state of this lane's ring
   Rec rec;
   {
+    const int myring = rbase + lane;
+    const bool ring_ok = myring <= nhr;
     const int ic = ring_ok ? myring : nhr;
-    const long long so = seedoff[m] + (ic - i0);
-    rec.st = ring_ok ? seed_st[so] : 0x3fffffff;
-    rec.sv = seed_v[so];
+    rec.st = ring_ok ? syn_st[ys0 + (ic - i0)] : 0x3fffffff;
+    rec.sv = syn_v[ys0 + (ic - i0)];
     const RingGeo g = geo[ic];
     rec.x = g.z;
     const double s2 = g.s * g.s;
@@ -338,138 +345,132 @@ __global__ __launch_bounds__(256, SPIN ? 1 : FSHT_SYN_MINBLK0) void k_synthesis(
     rec.p2 = 0.0;
   }
 
-  d4 D[4][4];
+  // ---- alm operand sources.  Element (l, sim) of tile u, function f sits at
+  // byte bb[f][u] + 8*(sim + l*nsim): a uniform base (scalar registers) plus a
+  // 32-bit lane offset.  Rows are clamped to l <= lmax (P is 0 beyond lmax)
+  // and padding columns to sim < nsim, so every read is in bounds.  Spin 2
+  // scales by the sign of the term (0 for padding and for Im at m = 0); spin 0
+  // needs no scale: padding columns are never read back and Im at m = 0 is
+  // stored as 0.
+  const char* bb[NF][NT];
+  double bs[NF][NT];
+  int simc[NT];
 #pragma unroll
-  for (int a = 0; a < 4; ++a)
+  for (int u = 0; u < NT; ++u) {
+    const int c0 = (it.colblk + u) * 16;
+    const bool cok = c0 < ncols;
+    const int otype = cok ? c0 / nsimp : 0;
+    const int sim = (cok ? c0 - otype * nsimp : 0) + li;
+    simc[u] = sim < nsim ? sim : nsim - 1;
 #pragma unroll
-    for (int b = 0; b < 4; ++b) D[a][b] = d4{0.0, 0.0, 0.0, 0.0};
-
-  // branch-free alm operand load for K-step ks of the chunk starting at lc
-  auto load_b = [&](int lc, double (&bf)[NKS][4]) {
+    for (int f = 0; f < NF; ++f) {
+      const Src s = syn_src(SPIN, otype, f);
+      const bool im = s.reim && m > 0;
+      const bool ok = cok && sim < nsim && !(m == 0 && s.reim);
+      const double* a = s.comp ? alm1 : alm0;
+      bb[f][u] = reinterpret_cast<const char*>(a + (long long)nsim * (im ? (L * (L - m) - m) : (L * m)));
+      bs[f][u] = ok ? s.sign : 0.0;
+    }
+  }
+  // K-step ks of the chunk at lc holds l = lc + pc(f) + 2*(4*(ks%4) + q)
+  auto load_b = [&](int lc, double (&bf)[NKS][NT]) {
 #pragma unroll
     for (int ks = 0; ks < NKS; ++ks) {
       const int f = SPIN ? (ks / 4) : 0;
-      const int vi = 4 * (ks % 4) + q;
-      const int pc = (f == 0) ? cls : (cls ^ 1);
-      const int l = lc + pc + 2 * vi;
-      const bool lok = l <= lmax;
-      const int lcl = lok ? l : lmax;
+      const int pc = (f == 0) ? CLS : (CLS ^ 1);
+      const int l = lc + pc + 8 * (ks % 4) + 2 * q;
+      const unsigned row = (unsigned)nsim * (unsigned)(l <= lmax ? l : lmax);
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        const Src s = syn_src(SPIN, otype[u], f);
-        const int sim = sim0[u] + li;
-        // Im at m = 0 has no storage: read the (valid) Re address and drop it
-        const bool im = s.reim && m > 0;
-        const bool ok = cok[u] && lok && sim < nsim && !(m == 0 && s.reim);
-        const int simc = sim < nsim ? sim : nsim - 1;
-        const double* alm = s.comp ? alm1 : alm0;
-        const long long idx = im ? (long long)simc + nsim * ((lcl - m) + L * (L - m))
-                                 : (long long)simc + nsim * (lcl + L * m);
-        const double v = alm[idx];
-        bf[ks][u] = ok ? s.sign * v : 0.0;
+      for (int u = 0; u < NT; ++u) {
+        const double v = *reinterpret_cast<const double*>(bb[f][u] + 8u * (row + (unsigned)simc[u]));
+        bf[ks][u] = SPIN ? v * bs[f][u] : v;
       }
     }
   };
-  auto load_cf = [&](int lc, int buf) {
-    if (tid < 32) {
-      const int l = lc + tid;
-      cf[buf][tid] = (l <= lmax) ? abm[l - m] : make_double2(0.0, 0.0);
-      if (SPIN) scf[buf][tid] = (l <= lmax) ? spm[l - m] : make_double2(0.0, 0.0);
+  auto load_seg = [&](int l0) {
+#pragma unroll
+    for (int k = lane; k < kSegL; k += 64) {
+      const int l = l0 + k;
+      cf[k] = (l <= lmax) ? abm[l - m] : make_double2(0.0, 0.0);
+      if (SPIN) sf[k] = (l <= lmax) ? spm[l - m] : make_double2(0.0, 0.0);
     }
   };
 
-  double bcur[NKS][4], bnext[NKS][4];
+  d4 D[4][NT];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int u = 0; u < NT; ++u) D[a][u] = d4{0.0, 0.0, 0.0, 0.0};
+
+  double bcur[NKS][NT], bnx[NKS][NT];
   load_b(m, bcur);
-  load_cf(m, 0);
-  __syncthreads();
-  int buf = 0;
-  for (int lc = m; lc <= lmax; lc += 32, buf ^= 1) {
+  for (int lc = m; lc <= lmax; lc += 32) {
+    const int jseg = (lc - m) % kSegL;
+    if (jseg == 0) {
+      wave_lds_sync();
+      load_seg(lc);
+      wave_lds_sync();
+    }
     const bool more = lc + 32 <= lmax;
-    if (more) {
-      load_b(lc + 32, bnext);
-      load_cf(lc + 32, buf ^ 1);
-    }
-    // ---- recurrence for this lane's ring over 32 l-steps; step s has (l - m)
-    // parity s & 1, so each (even, odd) pair of steps yields one value per class
-    double keep = 0.0;
-    auto emit = [&](int s, double v) {
-      if ((s & 1) == 0)
-        keep = v;
-      else
-        Pt[wave][lane][s >> 1] = cls ? v : keep;
-    };
-    if (SPIN) {
-      // F1 uses class cls, F2 the other class: generate both functions
-      double k1 = 0.0, k2 = 0.0;
-      auto emit2 = [&](int s, double v1, double v2) {
-        if ((s & 1) == 0) {
-          k1 = v1;
-          k2 = v2;
-        } else {
-          Pt[wave][lane][(s >> 1)] = cls ? v1 : k1;            // F1, class cls
-          Pt[wave][lane][16 + (s >> 1)] = cls ? k2 : v2;       // F2, other class
+    if (more) load_b(lc + 32, bnx);
+
+    // ---- 32 recurrence steps; step s has l - m parity s & 1 (lc - m is even)
+    const double2* c = cf + jseg;
+    const double2* sc = sf + (SPIN ? jseg : 0);
+    auto steps = [&](auto injc) {
+      constexpr bool INJ = decltype(injc)::value;
+#pragma unroll
+      for (int s = 0; s < 32; ++s) {
+        const double2 cc = c[s];
+        double p = fma(cc.x * rec.x, rec.p1, -(cc.y * rec.p2));
+        if (INJ) {
+          const int l = lc + s;
+          p = (l == rec.st) ? rec.sv.x : p;
+          p = (l == rec.st + 1) ? rec.sv.y : p;
         }
-      };
-      const bool inj = (rec.st >= lc - 1) && (rec.st <= lc + 31);
-      const double2* c = cf[buf];
-      const double2* sc = scf[buf];
-      auto body = [&](auto injc) {
-        constexpr bool INJ = decltype(injc)::value;
-#pragma unroll
-        for (int s = 0; s < 32; ++s) {
-          const double2 cc = c[s];
-          double p = fma(cc.x * rec.x, rec.p1, -(cc.y * rec.p2));
-          if (INJ) {
-            const int l = lc + s;
-            p = (l == rec.st) ? rec.sv.x : p;
-            p = (l == rec.st + 1) ? rec.sv.y : p;
-          }
-          const double f1 = spin_f(0, p, rec.p1, sc[s], lc + s, m, rec.x, rec.bw, rec.t2, rec.cw);
-          const double f2 = spin_f(1, p, rec.p1, sc[s], lc + s, m, rec.x, rec.bw, rec.t2, rec.cw);
-          emit2(s, f1, f2);
-          rec.p2 = rec.p1;
-          rec.p1 = p;
+        if (SPIN) {
+          // F1 of this class at steps of its parity, F2 of the other class at the others
+          if ((s & 1) == CLS)
+            Pt[lane][s >> 1] = spin_f(0, p, rec.p1, sc[s], lc + s, m, rec.x, rec.bw, rec.t2, rec.cw);
+          else
+            Pt[lane][16 + (s >> 1)] = spin_f(1, p, rec.p1, sc[s], lc + s, m, rec.x, rec.bw, rec.t2, rec.cw);
+        } else if ((s & 1) == CLS) {
+          Pt[lane][s >> 1] = p;
         }
-      };
-      if (__any(inj))
-        body(std::true_type{});
-      else
-        body(std::false_type{});
-    } else {
-      const bool inj = (rec.st >= lc - 1) && (rec.st <= lc + 31);
-      if (__any(inj))
-        gen32<0, true>(rec, cf[buf], nullptr, lc, m, 0, emit);
-      else
-        gen32<0, false>(rec, cf[buf], nullptr, lc, m, 0, emit);
-    }
-    __syncthreads();
-    if (wave_ok) {
-      // one ring tile at a time: its 4 accumulators rotate through the chunk's
-      // K-steps.  FP64 MFMA throughput drops when many distinct accumulators
-      // are cycled (register-file reads of srcC); 4 in rotation keep it high.
-#pragma unroll
-      for (int a = 0; a < 4; ++a) {
-#pragma unroll
-        for (int ks = 0; ks < NKS; ++ks) {
-          const int f = SPIN ? (ks / 4) : 0;
-          const double av = Pt[wave][16 * a + li][f * 16 + 4 * (ks % 4) + q];
-#pragma unroll
-          for (int u = 0; u < 4; ++u) D[a][u] = mfma64(av, bcur[ks][u], D[a][u]);
-        }
-        __builtin_amdgcn_sched_barrier(0);
+        rec.p2 = rec.p1;
+        rec.p1 = p;
       }
+    };
+    const bool inj = (rec.st >= lc - 1) && (rec.st <= lc + 31);
+    if (__any(inj))
+      steps(std::true_type{});
+    else
+      steps(std::false_type{});
+    wave_lds_sync();
+
+    // ---- contraction: ring tile a (16 rings) x column tile u, K = NV values
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+      double av[NKS];
+#pragma unroll
+      for (int ks = 0; ks < NKS; ++ks) {
+        const int f = SPIN ? (ks / 4) : 0;
+        av[ks] = Pt[16 * a + li][f * 16 + 4 * (ks % 4) + q];
+      }
+#pragma unroll
+      for (int u = 0; u < NT; ++u)
+#pragma unroll
+        for (int ks = 0; ks < NKS; ++ks) D[a][u] = mfma64(av[ks], bcur[ks][u], D[a][u]);
     }
-    __syncthreads();
+    wave_lds_sync();
     if (more) {
 #pragma unroll
       for (int ks = 0; ks < NKS; ++ks)
 #pragma unroll
-        for (int u = 0; u < 4; ++u) bcur[ks][u] = bnext[ks][u];
+        for (int u = 0; u < NT; ++u) bcur[ks][u] = bnx[ks][u];
     }
   }
 
-  if (!wave_ok) return;
-  const long long gbase = goff[m] / (SPIN ? 1 : 2) + (long long)cls * nr * ncols;
 #pragma unroll
   for (int a = 0; a < 4; ++a) {
 #pragma unroll
@@ -477,11 +478,54 @@ __global__ __launch_bounds__(256, SPIN ? 1 : FSHT_SYN_MINBLK0) void k_synthesis(
       const int ring = rbase + 16 * a + q + 4 * v;
       if (ring > nhr) continue;
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        if (!cok[u]) continue;
-        const int col = (it.colblk * 4 + u) * 16 + li;
-        G[gbase + (long long)(ring - i0) * ncols + col] = D[a][u][v];
+      for (int u = 0; u < NT; ++u) {
+        const int col = (it.colblk + u) * 16 + li;
+        // spin 0: the Im columns of m = 0 carry nothing
+        const bool zero = !SPIN && m == 0 && col >= nsimp;
+        if (col < ncols) G[gbase + (long long)(ring - i0) * ncols + col] = zero ? 0.0 : D[a][u][v];
       }
+    }
+  }
+}
+
+template <int SPIN>
+__global__ __launch_bounds__(256, SPIN ? 1 : FSHT_SYN_MINBLK0) void k_synthesis(
+    const LegItem* __restrict__ items, const RingGeo* __restrict__ geo, const int* __restrict__ i0tab,
+    const double2* __restrict__ ab, const double2* __restrict__ spc, const long long* __restrict__ aboff,
+    const long long* __restrict__ synoff, const int* __restrict__ syn_st, const double2* __restrict__ syn_v,
+    const double* __restrict__ alm0, const double* __restrict__ alm1, double* __restrict__ G,
+    const long long* __restrict__ goff, int nsim, int nsimp, int lmax, int nhr) {
+  __shared__ SynShared<SPIN> sh;
+  const LegItem it = items[blockIdx.x];
+  if (it.m < 0) return;
+  const int m = it.m;
+  const int i0 = i0tab[m];
+  const int nr = nhr + 1 - i0;
+  const int rbase = i0 + it.blk * kSynRings + (threadIdx.x >> 6) * 64;
+  // waves never synchronise with each other, so an idle wave may leave
+  if (rbase > nhr) return;
+  const int ncols = 2 * (SPIN ? 2 : 1) * nsimp;
+  const long long gbase = goff[m] / (SPIN ? 1 : 2) + (long long)it.par * nr * ncols;
+  const double2* abm = ab + aboff[m];
+  const double2* spm = spc + aboff[m];
+  const long long ys0 = synoff[m];
+  auto run = [&](auto ntc, auto clsc) {
+    synth_wave<SPIN, decltype(ntc)::value, decltype(clsc)::value>(sh, it, geo, abm, spm, syn_st, syn_v, ys0, alm0,
+                                                                   alm1, G, gbase, i0, nsim, nsimp, lmax, nhr, rbase);
+  };
+  using I0 = std::integral_constant<int, 0>;
+  using I1 = std::integral_constant<int, 1>;
+  if (SPIN) {
+    using N2 = std::integral_constant<int, 2>;
+    if (it.par) run(N2{}, I1{}); else run(N2{}, I0{});
+  } else {
+    using N4 = std::integral_constant<int, 4>;
+    using N2 = std::integral_constant<int, 2>;
+    const bool full = (it.colblk + 4) * 16 <= ncols;
+    if (full) {
+      if (it.par) run(N4{}, I1{}); else run(N4{}, I0{});
+    } else {
+      if (it.par) run(N2{}, I1{}); else run(N2{}, I0{});
     }
   }
 }
@@ -506,11 +550,11 @@ hipError_t launch_synthesis(Handle* h, const double* alm0, const double* alm1, d
   dim3 grid(h->n_syn_items[sp]);
   if (sp)
     hipLaunchKernelGGL(k_synthesis<2>, grid, dim3(256), 0, h->stream, h->d_syn_items[1], h->d_geo, h->d_i0, h->d_ab,
-                       h->d_spin, h->d_aboff, h->d_seedoff, h->d_seed_st, h->d_seed_v, alm0, alm1, G, h->d_goff,
+                       h->d_spin, h->d_aboff, h->d_synoff, h->d_syn_st, h->d_syn_v, alm0, alm1, G, h->d_goff,
                        h->nsim, h->nsimp, h->lmax, h->nhr);
   else
     hipLaunchKernelGGL(k_synthesis<0>, grid, dim3(256), 0, h->stream, h->d_syn_items[0], h->d_geo, h->d_i0, h->d_ab,
-                       h->d_spin, h->d_aboff, h->d_seedoff, h->d_seed_st, h->d_seed_v, alm0, alm0, G, h->d_goff,
+                       h->d_spin, h->d_aboff, h->d_synoff, h->d_syn_st, h->d_syn_v, alm0, alm0, G, h->d_goff,
                        h->nsim, h->nsimp, h->lmax, h->nhr);
   return hipGetLastError();
 }

@@ -208,7 +208,9 @@ __device__ static void pow_scaled_dev(double x, int n, double* mant, long long* 
 __global__ void k_seeds(int lmax, int nhr, const RingGeo* __restrict__ geo, const int* __restrict__ i0,
                         const double* __restrict__ dfac, const double2* __restrict__ ab,
                         const long long* __restrict__ aboff, const long long* __restrict__ seedoff,
-                        int* __restrict__ seed_st, double2* __restrict__ seed_v) {
+                        int* __restrict__ seed_st, double2* __restrict__ seed_v,
+                        const long long* __restrict__ synoff, int* __restrict__ syn_st,
+                        double2* __restrict__ syn_v) {
   const int m = blockIdx.y;
   const int i = i0[m] + blockIdx.x * blockDim.x + threadIdx.x;
   if (i > nhr) return;
@@ -219,6 +221,11 @@ __global__ void k_seeds(int lmax, int nhr, const RingGeo* __restrict__ geo, cons
     seed_st[base + (long long)b * nr] = 0x3fffffff;
     seed_v[base + (long long)b * nr] = make_double2(0.0, 0.0);
   }
+  // synthesis start: the first kept l of the ring over the whole l range
+  const long long ys = synoff[m] + (i - i0[m]);
+  syn_st[ys] = 0x3fffffff;
+  syn_v[ys] = make_double2(0.0, 0.0);
+  double y1 = 0.0;
   const double z = geo[i].z, s = geo[i].s;
   const double2* abm = ab + aboff[m];
   double ms;
@@ -257,6 +264,13 @@ __global__ void k_seeds(int lmax, int nhr, const RingGeo* __restrict__ geo, cons
     }
     if (first < 0) continue;
     const double val = ldexp(p, (int)e);
+    if (l == first) {
+      y1 = val;
+      syn_st[ys] = first;
+      syn_v[ys] = make_double2(val, 0.0);
+    } else if (l == first + 1) {
+      syn_v[ys] = make_double2(y1, val);
+    }
     // second seed value of the previous start (may fall in the next block)
     if (cur_b >= 0 && l == cur_st + 1) {
       seed_v[base + (long long)cur_b * nr] = make_double2(v1, val);
@@ -285,7 +299,8 @@ hipError_t launch_seeds(Handle* h) {
   e = hipMemcpyAsync(d_dfac, dfac.data(), dfac.size() * sizeof(double), hipMemcpyHostToDevice, h->stream);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_seeds, grid, dim3(256), 0, h->stream, h->lmax, h->nhr, h->d_geo, h->d_i0, d_dfac,
-                     h->d_ab, h->d_aboff, h->d_seedoff, h->d_seed_st, h->d_seed_v);
+                     h->d_ab, h->d_aboff, h->d_seedoff, h->d_seed_st, h->d_seed_v, h->d_synoff, h->d_syn_st,
+                     h->d_syn_v);
   e = hipGetLastError();
   if (e != hipSuccess) return e;
   e = hipStreamSynchronize(h->stream);
@@ -366,16 +381,19 @@ static void fill_work(Handle* h) {
   const int lmax = h->lmax;
   h->nblk.resize(lmax + 1);
   h->seedoff.resize(lmax + 2);
+  h->synoff.resize(lmax + 2);
   h->goff.resize(lmax + 2);
-  long long so = 0, go = 0;
+  long long so = 0, yo = 0;
   for (int m = 0; m <= lmax; ++m) {
     const int nr = h->nhr + 1 - h->i0[m];
     h->nblk[m] = (lmax + 1 - m + kAnaLB - 1) / kAnaLB;
     h->seedoff[m] = so;
     so += (long long)h->nblk[m] * nr;
-    h->goff[m] = go;
+    h->synoff[m] = yo;
+    yo += nr;
   }
   h->seedoff[lmax + 1] = so;
+  h->synoff[lmax + 1] = yo;
 }
 
 static int ceil_div(int a, int b) { return (a + b - 1) / b; }
@@ -418,9 +436,9 @@ static hipError_t build_items(Handle* h) {
         for (int b = 0; b < h->nblk[m]; ++b)
           for (int cb = 0; cb < ncb; ++cb) ana.push_back(LegItem{m, par, b, cb});
         const int nrb = ceil_div(nr, kSynRings);
-        const int ncs = ceil_div(ncols, kSynCols);
+        const int ntile = ncols / 16, tpi = sp ? 2 : 4;
         for (int rb = 0; rb < nrb; ++rb)
-          for (int cb = 0; cb < ncs; ++cb) syn.push_back(LegItem{m, par, rb, cb});
+          for (int t0 = 0; t0 < ntile; t0 += tpi) syn.push_back(LegItem{m, par, rb, t0});
       }
     }
     hipError_t e = upload_items(ana, &h->d_ana_items[sp], &h->n_ana_items[sp]);
@@ -470,6 +488,10 @@ hipError_t setup_tables(Handle* h) {
   const long long nseed = h->seedoff[h->lmax + 1];
   if ((e = hipMalloc(&h->d_seed_st, std::max(1LL, nseed) * sizeof(int))) != hipSuccess) return e;
   if ((e = hipMalloc(&h->d_seed_v, std::max(1LL, nseed) * sizeof(double2))) != hipSuccess) return e;
+  if ((e = upload(&h->d_synoff, h->synoff)) != hipSuccess) return e;
+  const long long nsyn = h->synoff[h->lmax + 1];
+  if ((e = hipMalloc(&h->d_syn_st, std::max(1LL, nsyn) * sizeof(int))) != hipSuccess) return e;
+  if ((e = hipMalloc(&h->d_syn_v, std::max(1LL, nsyn) * sizeof(double2))) != hipSuccess) return e;
   if ((e = launch_seeds(h)) != hipSuccess) return e;
   synth_offsets(h, 2);  // sized for the largest (spin-2) layout; spin-0 uses a prefix
   if ((e = upload(&h->d_goff, h->goff)) != hipSuccess) return e;

@@ -163,3 +163,65 @@ def test_alm_utilities_vs_oracle(torch_cuda):
     _lib.check(lib.fsht_convert_alm_healpy_inv(p(hpd), p(back), lmax, nsim, None))
     torch.cuda.synchronize()
     np.testing.assert_array_equal(back.cpu().numpy().transpose(2, 1, 0), alm)
+
+
+def _single_order_closed_form(nside, lmax, m, a):
+    """Map of the field whose only alm are a_lm (l = m..lmax, complex a) for
+    one order m: T(theta_r, phi) = g * Re(sum_l P_lm(theta_r) a_l e^{i m phi}),
+    g = 1 at m = 0 (a real) else 2; south rings carry (-1)^(l+m).  Also returns
+    the per-ring direct sums S_r = sum_{p in ring r} T_p e^{-i m phi_p} used to
+    check the analysis."""
+    nr = 4 * nside - 1
+    P, _ = O.plm_north(nside, lmax, m)              # (2*nside rings, lmax+1-m)
+    sgn = (-1.0) ** np.arange(lmax + 1 - m)         # (-1)^(l+m) relative to l = m
+    gN = P @ a
+    gS = P @ (sgn * a)
+    g_ring = np.concatenate([gN, gS[:2 * nside - 1][::-1]])
+    # exact phases: phi_p = pi*(2j + h)/n on a ring of n pixels (h = 1 when
+    # the ring starts half a pixel off phi = 0), so m*phi_p = pi*k/n with the
+    # integer k = (2mj + mh) mod 2n; float64 m*phi would be off by ~m*1e-16
+    i1, nfft = O.ring_info(nside)
+    nfft = np.asarray(nfft, dtype=np.int64)
+    h = np.rint(O.ring_phi0(nside) * nfft / np.pi).astype(np.int64)
+    ring_of_pix = np.repeat(np.arange(nr), nfft)
+    j = np.arange(12 * nside * nside, dtype=np.int64) - np.repeat(np.asarray(i1, dtype=np.int64), nfft)
+    n_p = nfft[ring_of_pix]
+    k = (2 * m * j + m * h[ring_of_pix]) % (2 * n_p)
+    eph = np.exp(1j * np.pi * k / n_p)
+    fac = 1.0 if m == 0 else 2.0
+    T = fac * np.real(g_ring[ring_of_pix] * eph)
+    S = np.add.reduceat(T * np.conj(eph), np.asarray(i1, dtype=np.int64))
+    # north-folded analysis operands: P^T (S_north +/- S_south)
+    return T, S, P, sgn
+
+
+@pytest.mark.parametrize("m", [0, 393, 1200, 3000])
+def test_single_order_lmax_4nside(torch_cuda, m):
+    """nside=1024, lmax=4*nside: for these orders some kept polar rings only
+    rise above the 2^-900 flush level hundreds of l-steps after m (more than
+    one 256-l block), and reach O(1) before lmax.  Synthesis must start each
+    ring's recurrence at its first kept l; checked against the closed form,
+    and the analysis against the direct per-ring sums."""
+    from fastsht_amd import SHT
+    nside, lmax, nsim = 1024, 4096, 1
+    L = lmax + 1
+    print(f"case single order nside={nside} lmax={lmax} m={m}", flush=True)
+    rng = np.random.default_rng(100 + m)
+    n = L - m
+    a = rng.standard_normal(n) + (1j * rng.standard_normal(n) if m > 0 else 0.0)
+    alm = np.zeros((nsim, L, L), order="F")
+    alm[0, m:, m] = a.real
+    if m > 0:
+        alm[0, :n, L - m] = a.imag
+    T, S, P, sgn = _single_order_closed_form(nside, lmax, m, a)
+    sht = SHT(nside, lmax, nsim)
+    maps = sht.alm2t(alm)
+    assert maxrel(maps[:, 0], T) < TOL
+    # analysis of the closed-form map, order m only
+    out = sht.t2alm(np.asfortranarray(T[:, None]))
+    npix = 12 * nside * nside
+    SN = S[:2 * nside]
+    SS = np.concatenate([S[2 * nside:][::-1], [0.0]])  # mirror of north ring i (equator has none)
+    ad = (4 * np.pi / npix) * (P.T @ SN + sgn * (P.T @ SS))
+    got = out[0, m:, m] + (1j * out[0, :n, L - m] if m > 0 else 0.0)
+    assert maxrel(got, ad) < TOL
