@@ -398,23 +398,46 @@ static void fill_work(Handle* h) {
 
 static int ceil_div(int a, int b) { return (a + b - 1) / b; }
 
-// XCD-aware ordering: the hardware deals workgroups round-robin to 8 XCDs, so
-// item k of the launch runs on XCD k%8.  Lay the list out so consecutive items
-// of the natural order (same m, sharing gathered data and seeds) land on one XCD.
-static std::vector<LegItem> xcd_order(const std::vector<LegItem>& v) {
-  const size_t n = v.size();
-  const size_t per = (n + 7) / 8;
-  std::vector<LegItem> out(per * 8);
-  for (size_t k = 0; k < per * 8; ++k) {
-    const size_t x = k % 8, j = k / 8;
-    const size_t src = x * per + j;
-    out[k] = src < n ? v[src] : LegItem{-1, 0, 0, 0};
+// XCD-aware ordering.  The hardware deals workgroups round-robin to the 8
+// XCDs (workgroup k runs on XCD k % 8), each with its own L2.  All items of
+// one order m read the same operands and seeds, so they are kept on one XCD;
+// the m-groups are spread over the XCDs by estimated cost (longest group to
+// the least loaded XCD), each XCD's list runs longest groups first, and the
+// eight lists are interleaved for the round-robin dispatch (short lists are
+// padded with empty items, m = -1).
+static std::vector<LegItem> xcd_order(const std::vector<LegItem>& v, const std::vector<double>& cost) {
+  constexpr int kXcd = 8;
+  struct Group {
+    size_t begin, end;
+    double cost;
+  };
+  std::vector<Group> groups;
+  for (size_t i = 0; i < v.size();) {
+    size_t j = i;
+    double c = 0.0;
+    while (j < v.size() && v[j].m == v[i].m) c += cost[j++];
+    groups.push_back(Group{i, j, c});
+    i = j;
   }
+  std::stable_sort(groups.begin(), groups.end(), [](const Group& a, const Group& b) { return a.cost > b.cost; });
+  std::vector<std::vector<LegItem>> lists(kXcd);
+  std::vector<double> load(kXcd, 0.0);
+  for (const Group& g : groups) {
+    const int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+    load[x] += g.cost;
+    for (size_t i = g.begin; i < g.end; ++i) lists[x].push_back(v[i]);
+  }
+  size_t per = 0;
+  for (const auto& l : lists) per = std::max(per, l.size());
+  std::vector<LegItem> out(per * kXcd, LegItem{-1, 0, 0, 0});
+  for (int x = 0; x < kXcd; ++x)
+    for (size_t j = 0; j < lists[x].size(); ++j) out[j * kXcd + x] = lists[x][j];
   return out;
 }
 
-static hipError_t upload_items(const std::vector<LegItem>& v, LegItem** d, int* n) {
-  std::vector<LegItem> o = xcd_order(v);
+static hipError_t upload_items(const std::vector<LegItem>& v, const std::vector<double>& cost, LegItem** d,
+                               int* n) {
+  std::vector<LegItem> o = xcd_order(v, cost);
   *n = (int)o.size();
   if (o.empty()) return hipSuccess;
   hipError_t e = hipMalloc(d, o.size() * sizeof(LegItem));
@@ -428,22 +451,33 @@ static hipError_t build_items(Handle* h) {
     const int ncomp = sp ? 2 : 1;
     const int ncols = 2 * ncomp * h->nsimp;
     std::vector<LegItem> ana, syn;
+    std::vector<double> ana_cost, syn_cost;  // estimated run time of each item
     for (int m = 0; m <= lmax; ++m) {
       const int nr = h->nhr + 1 - h->i0[m];
-      // columns: at m = 0 the imaginary columns carry nothing
       for (int par = 0; par < 2; ++par) {
+        // analysis: every ring of m against one l-block
         const int ncb = ceil_div(ncols, kAnaCols);
-        for (int b = 0; b < h->nblk[m]; ++b)
-          for (int cb = 0; cb < ncb; ++cb) ana.push_back(LegItem{m, par, b, cb});
+        for (int b = 0; b < h->nblk[m]; ++b) {
+          const int nl = std::min(kAnaLB, lmax + 1 - m - b * kAnaLB);
+          for (int cb = 0; cb < ncb; ++cb) {
+            ana.push_back(LegItem{m, par, b, cb});
+            ana_cost.push_back((double)nr * (nl + 32));
+          }
+        }
+        // synthesis: one ring block against all l of m (first 16-column tile t0)
         const int nrb = ceil_div(nr, kSynRings);
         const int ntile = ncols / 16, tpi = sp ? 2 : 4;
+        const double chunks = (double)ceil_div(lmax + 1 - m, 32) + 1.0;
         for (int rb = 0; rb < nrb; ++rb)
-          for (int t0 = 0; t0 < ntile; t0 += tpi) syn.push_back(LegItem{m, par, rb, t0});
+          for (int t0 = 0; t0 < ntile; t0 += tpi) {
+            syn.push_back(LegItem{m, par, rb, t0});
+            syn_cost.push_back(chunks * std::min(tpi, ntile - t0));
+          }
       }
     }
-    hipError_t e = upload_items(ana, &h->d_ana_items[sp], &h->n_ana_items[sp]);
+    hipError_t e = upload_items(ana, ana_cost, &h->d_ana_items[sp], &h->n_ana_items[sp]);
     if (e != hipSuccess) return e;
-    e = upload_items(syn, &h->d_syn_items[sp], &h->n_syn_items[sp]);
+    e = upload_items(syn, syn_cost, &h->d_syn_items[sp], &h->n_syn_items[sp]);
     if (e != hipSuccess) return e;
   }
   return hipSuccess;
