@@ -129,8 +129,11 @@ __device__ inline void gen32(Rec& r, const double2* cf, const double2* scf, int 
 #ifndef FSHT_ANA_MINBLK0
 #define FSHT_ANA_MINBLK0 1
 #endif
+#ifndef FSHT_ANA_MINBLK2
+#define FSHT_ANA_MINBLK2 2
+#endif
 template <int SPIN>
-__global__ __launch_bounds__(256, SPIN ? 1 : FSHT_ANA_MINBLK0) void k_analysis(
+__global__ __launch_bounds__(256, SPIN ? FSHT_ANA_MINBLK2 : FSHT_ANA_MINBLK0) void k_analysis(
     const LegItem* __restrict__ items, const RingGeo* __restrict__ geo, const int* __restrict__ i0tab,
     const double2* __restrict__ ab, const double2* __restrict__ spc, const long long* __restrict__ aboff,
     const long long* __restrict__ seedoff, const int* __restrict__ seed_st, const double2* __restrict__ seed_v,
@@ -300,12 +303,22 @@ __device__ inline void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+#ifndef FSHT_SYN_SEG2
+#define FSHT_SYN_SEG2 64
+#endif
+#ifndef FSHT_SYN_MINBLK2
+#define FSHT_SYN_MINBLK2 2
+#endif
 template <int SPIN>
 struct SynShared {
   static constexpr int NV = SPIN ? 32 : 16;  // LDS values per ring per chunk
+  // l values of recurrence coefficients staged per wave (spin 2 stages two
+  // tables and a twice larger P tile, so it keeps a shorter segment to fit
+  // two workgroups per CU)
+  static constexpr int SEG = SPIN ? FSHT_SYN_SEG2 : kSegL;
   double Pt[4][64][NV + 1];
-  double2 cf[4][kSegL];
-  double2 sf[4][SPIN ? kSegL : 1];
+  double2 cf[4][SEG];
+  double2 sf[4][SPIN ? SEG : 1];
 };
 
 template <int SPIN, int NT, int CLS>
@@ -387,9 +400,11 @@ __device__ inline void synth_wave(SynShared<SPIN>& sh, const LegItem it, const R
       }
     }
   };
+  constexpr int SEG = SynShared<SPIN>::SEG;
+  static_assert(SEG % 32 == 0 && SEG >= 32, "segment must hold whole chunks");
   auto load_seg = [&](int l0) {
 #pragma unroll
-    for (int k = lane; k < kSegL; k += 64) {
+    for (int k = lane; k < SEG; k += 64) {
       const int l = l0 + k;
       cf[k] = (l <= lmax) ? abm[l - m] : make_double2(0.0, 0.0);
       if (SPIN) sf[k] = (l <= lmax) ? spm[l - m] : make_double2(0.0, 0.0);
@@ -405,7 +420,7 @@ __device__ inline void synth_wave(SynShared<SPIN>& sh, const LegItem it, const R
   double bcur[NKS][NT], bnx[NKS][NT];
   load_b(m, bcur);
   for (int lc = m; lc <= lmax; lc += 32) {
-    const int jseg = (lc - m) % kSegL;
+    const int jseg = (lc - m) % SEG;
     if (jseg == 0) {
       wave_lds_sync();
       load_seg(lc);
@@ -489,7 +504,7 @@ __device__ inline void synth_wave(SynShared<SPIN>& sh, const LegItem it, const R
 }
 
 template <int SPIN>
-__global__ __launch_bounds__(256, SPIN ? 1 : FSHT_SYN_MINBLK0) void k_synthesis(
+__global__ __launch_bounds__(256, SPIN ? FSHT_SYN_MINBLK2 : FSHT_SYN_MINBLK0) void k_synthesis(
     const LegItem* __restrict__ items, const RingGeo* __restrict__ geo, const int* __restrict__ i0tab,
     const double2* __restrict__ ab, const double2* __restrict__ spc, const long long* __restrict__ aboff,
     const long long* __restrict__ synoff, const int* __restrict__ syn_st, const double2* __restrict__ syn_v,
