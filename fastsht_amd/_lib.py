@@ -24,7 +24,7 @@ FSHT_DEVICE = 1
 EXPORTS = (
     "fsht_version", "fsht_last_error", "fsht_create", "fsht_destroy", "fsht_get_stream",
     "fsht_set_stream", "fsht_synchronize", "fsht_info", "fsht_ring_tab", "fsht_t2alm",
-    "fsht_get_alm", "fsht_alm2t", "fsht_qu2eb", "fsht_qu2e", "fsht_qu2b", "fsht_eb2qu",
+    "fsht_get_alm", "fsht_alm2t", "fsht_qu2eb", "fsht_qu2e", "fsht_qu2b", "fsht_eb2qu", "fsht_fix_eb",
     "fsht_profile", "fsht_profile_read", "fsht_norm_alm", "fsht_filt_alm", "fsht_convert_alm_healpy", "fsht_convert_alm_healpy_inv",
 )
 
@@ -69,6 +69,7 @@ def load():
     lib.fsht_qu2e.argtypes = [H, dp, dp, dp, i]
     lib.fsht_qu2b.argtypes = [H, dp, dp, dp, i]
     lib.fsht_eb2qu.argtypes = [H, dp, dp, dp, dp, i]
+    lib.fsht_fix_eb.argtypes = [H, dp, dp, dp, dp, dp, dp, dp, i, i]
     lib.fsht_profile.argtypes = [H, i]
     lib.fsht_profile_read.argtypes = [H, ctypes.POINTER(ctypes.c_double), i]
     lib.fsht_norm_alm.argtypes = [dp, dp, dp, dp, d, i, i, vp]

@@ -9,6 +9,8 @@
 #include <cstring>
 #include <string>
 
+#include <algorithm>
+
 #include "fastsht.h"
 #include "fsht_internal.h"
 
@@ -384,6 +386,102 @@ int fsht_eb2qu(fsht_handle* H, const double* E, const double* B, double* Q, doub
   if ((rc = inverse_spectra(h, Q, U, loc))) return rc;
   if (!loc) HIPCHK(h, hipStreamSynchronize(h->stream));
   return FSHT_OK;
+}
+
+// ---------------------------------------------------------------------------
+// EB-leakage cleaning on a masked sky (src/sht_main.f90:801-941):
+//   Q, U *= mask (in place, as the reference does)
+//   (E, B)  = qu2eb(Q, U, niter)
+//   (Q', U') = E-only synthesis of E, masked again
+//   (., B') = qu2eb(Q', U', niter)           -- harmonic leakage template
+//   Bmap0 = alm2t(B), Btpl = alm2t(B')
+//   per map, over the pixels with mask == 1: fit Bmap0 = a0 + a1*Btpl and
+//   return the residual there (0 elsewhere).
+// Temporaries are allocated for the call and released at its end.
+// ---------------------------------------------------------------------------
+namespace {
+struct DevBuf {
+  double* p = nullptr;
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+};
+}  // namespace
+
+int fsht_fix_eb(fsht_handle* H, double* Q, double* U, const double* mask, double* Bmap, double* almB, double* Btpl,
+                double* coef, int niter, int loc) {
+  int rc = check_common(H, loc);
+  if (rc) return rc;
+  Handle* h = &H->h;
+  if (!h->pol) return fail(h, FSHT_ERR_STATE, "fix_eb needs a handle created with pol=1");
+  if (!Q || !U || !mask || !Bmap) return fail(h, FSHT_ERR_ARG, "NULL argument");
+  if (niter < 0) return fail(h, FSHT_ERR_ARG, "niter must be >= 0");
+  const long long nm = (long long)h->nsim * h->npix;
+  const size_t mbytes = (size_t)nm * sizeof(double), abytes = (size_t)alm_count(h) * sizeof(double);
+  // temporaries outlive the work: the stream is drained before they are freed,
+  // on the error paths too
+  DevBuf dQ, dU, dmask, dB, dE, dB2, dZ, dQ2, dU2, dBmap, dBtpl, dcoef;
+  rc = [&]() -> int {
+  auto alloc = [&](DevBuf& b, size_t bytes) -> hipError_t { return hipMalloc((void**)&b.p, std::max<size_t>(bytes, 8)); };
+  double *q = Q, *u = U, *bm = Bmap, *bt = Btpl, *ab = almB, *cf = coef;
+  const double* mk = mask;
+  if (!loc) {
+    HIPCHK(h, alloc(dQ, mbytes));
+    HIPCHK(h, alloc(dU, mbytes));
+    HIPCHK(h, alloc(dmask, (size_t)h->npix * sizeof(double)));
+    HIPCHK(h, alloc(dBmap, mbytes));
+    HIPCHK(h, hipMemcpyAsync(dQ.p, Q, mbytes, hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, hipMemcpyAsync(dU.p, U, mbytes, hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, hipMemcpyAsync(dmask.p, mask, (size_t)h->npix * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    q = dQ.p;
+    u = dU.p;
+    mk = dmask.p;
+    bm = dBmap.p;
+    bt = nullptr;
+    ab = nullptr;
+    cf = nullptr;
+  }
+  if (!bt) {
+    HIPCHK(h, alloc(dBtpl, mbytes));
+    bt = dBtpl.p;
+  }
+  if (!ab) {
+    HIPCHK(h, alloc(dB, abytes));
+    ab = dB.p;
+  }
+  if (!cf) {
+    HIPCHK(h, alloc(dcoef, 2 * (size_t)h->nsim * sizeof(double)));
+    cf = dcoef.p;
+  }
+  HIPCHK(h, alloc(dE, abytes));
+  HIPCHK(h, alloc(dB2, abytes));
+  HIPCHK(h, alloc(dZ, abytes));
+  HIPCHK(h, alloc(dQ2, mbytes));
+  HIPCHK(h, alloc(dU2, mbytes));
+  HIPCHK(h, hipMemsetAsync(dZ.p, 0, abytes, h->stream));
+  HIPCHK(h, fsht::launch_mask_maps(q, mk, h->npix, h->nsim, h->stream));
+  HIPCHK(h, fsht::launch_mask_maps(u, mk, h->npix, h->nsim, h->stream));
+  if ((rc = fsht_qu2eb(H, q, u, dE.p, ab, niter, FSHT_DEVICE))) return rc;
+  if ((rc = fsht_eb2qu(H, dE.p, dZ.p, dQ2.p, dU2.p, FSHT_DEVICE))) return rc;
+  HIPCHK(h, fsht::launch_mask_maps(dQ2.p, mk, h->npix, h->nsim, h->stream));
+  HIPCHK(h, fsht::launch_mask_maps(dU2.p, mk, h->npix, h->nsim, h->stream));
+  if ((rc = fsht_qu2eb(H, dQ2.p, dU2.p, dE.p, dB2.p, niter, FSHT_DEVICE))) return rc;
+  if ((rc = fsht_alm2t(H, ab, dQ2.p, FSHT_DEVICE))) return rc;  // B map with leakage
+  if ((rc = fsht_alm2t(H, dB2.p, bt, FSHT_DEVICE))) return rc;  // leakage template
+  HIPCHK(h, fsht::launch_linfit_resi(bt, dQ2.p, mk, bm, cf, h->npix, h->nsim, h->stream));
+  if (!loc) {
+    HIPCHK(h, hipMemcpyAsync(Q, dQ.p, mbytes, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipMemcpyAsync(U, dU.p, mbytes, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipMemcpyAsync(Bmap, bm, mbytes, hipMemcpyDeviceToHost, h->stream));
+    if (Btpl) HIPCHK(h, hipMemcpyAsync(Btpl, bt, mbytes, hipMemcpyDeviceToHost, h->stream));
+    if (almB) HIPCHK(h, hipMemcpyAsync(almB, ab, abytes, hipMemcpyDeviceToHost, h->stream));
+    if (coef) HIPCHK(h, hipMemcpyAsync(coef, cf, 2 * (size_t)h->nsim * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  }
+  return FSHT_OK;
+  }();
+  const hipError_t se = hipStreamSynchronize(h->stream);
+  if (rc == FSHT_OK && se != hipSuccess) return fail(h, FSHT_ERR_HIP, std::string("fix_eb: ") + hipGetErrorString(se));
+  return rc;
 }
 
 int fsht_profile(fsht_handle* H, int enable) {

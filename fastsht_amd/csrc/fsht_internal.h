@@ -67,6 +67,10 @@ hipError_t launch_analysis(Handle* h, const double* F, double* alm0, double* alm
 hipError_t launch_synthesis(Handle* h, const double* alm0, const double* alm1, double* G);
 hipError_t launch_fold(Handle* h, const double* G, const double* specT_t, double* specT_out,
                        int residual);
+// map-domain helpers (k_maps.hip)
+hipError_t launch_mask_maps(double* maps, const double* mask, long long npix, int nmaps, hipStream_t s);
+hipError_t launch_linfit_resi(const double* x, const double* y, const double* mask, double* resi, double* coef,
+                              long long npix, int nmaps, hipStream_t s);
 
 // -------------------------------------------------------------------------
 // Handle: one transform configuration with its resident tables and workspace.

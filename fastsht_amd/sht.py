@@ -242,9 +242,45 @@ class SHT:
         self._call(lib.fsht_eb2qu, ep, bp, qp, up, loc)
         return (Q, U)
 
-    def fix_eb(self, Q, U, mask):
-        """EB-leakage cleaning (SHT.py:192-202) is not provided by this library."""
-        raise NotImplementedError("fix_eb is not provided by fastsht_amd")
+    def fix_eb(self, Q, U, mask, Btpl_out=None, almB_out=None):
+        """EB-leakage cleaning on a masked sky (SHT.py:192-202 ->
+        src/sht_main.f90:801-941): returns the cleaned B map (npix, nsim).
+
+        As in the reference, Q and U are masked in place (when they are
+        Fortran-ordered float64 NumPy arrays or GPU tensors; other inputs are
+        converted and the masked copies are discarded).  The fit uses the
+        pixels where mask == 1.  Optional outputs: Btpl_out receives the
+        leakage template map, almB_out the B alm of the masked maps."""
+        self._need_pol()
+        lib = _lib.load()
+        dev = _is_torch(Q)
+        qp, loc, qk = self._arg(Q, self._mshape, "Q", out=dev or (isinstance(Q, np.ndarray) and Q.flags.f_contiguous
+                                                                  and Q.dtype == np.float64))
+        up, uloc, uk = self._arg(U, self._mshape, "U", out=dev or (isinstance(U, np.ndarray) and U.flags.f_contiguous
+                                                                   and U.dtype == np.float64))
+        if dev:
+            torch = _torch()
+            mk = mask if _is_torch(mask) else torch.as_tensor(np.asarray(mask, dtype=np.float64), device="cuda")
+            mk = mk.to(torch.float64).contiguous()
+            if mk.numel() != self.npix:
+                raise ValueError(f"mask must have npix={self.npix} entries")
+            mp = ctypes.c_void_p(mk.data_ptr())
+        else:
+            mk = np.ascontiguousarray(np.asarray(mask, dtype=np.float64).reshape(-1))
+            if mk.size != self.npix:
+                raise ValueError(f"mask must have npix={self.npix} entries")
+            mp = ctypes.c_void_p(mk.ctypes.data)
+        Bmap = self._like(Q, "map")
+        bp, bloc, bk = self._arg(Bmap, self._mshape, "Bmap", out=True)
+        tp = ap = None
+        if Btpl_out is not None:
+            tp = self._arg(Btpl_out, self._mshape, "Btpl_out", out=True)[0]
+        if almB_out is not None:
+            ap = self._arg(almB_out, self._ashape, "almB_out", out=True)[0]
+        if len({loc, uloc, bloc}) != 1:
+            raise ValueError("all arrays must be host arrays or all GPU tensors")
+        self._call(lib.fsht_fix_eb, qp, up, mp, bp, ap, tp, None, self.niter, loc)
+        return Bmap
 
     def convert_alm_healpy(self, alms):
         """Square packing -> healpy complex (nalm, nsim) (SHT.py:204-210,

@@ -98,6 +98,21 @@ int fsht_qu2b(fsht_handle* h, const double* Q, const double* U, double* B, int l
  * / SHT.eb2qu (SHT.py:179-190). */
 int fsht_eb2qu(fsht_handle* h, const double* E, const double* B, double* Q, double* U, int loc);
 
+/* EB-leakage cleaning of polarised maps on a masked sky.  Replaces fix_eb
+ * (src/sht_main.f90:801-941) behind SHT.fix_eb (SHT.py:192-202).
+ * Q and U are masked in place (Q *= mask, U *= mask), as the reference does;
+ * mask has npix entries.  Steps: (E, B) = qu2eb(masked Q, U, niter); the
+ * E-only synthesis of E, masked again, analysed again gives the leakage
+ * template B'; Bmap = alm2t(B) - a0 - a1 * alm2t(B') with (a0, a1) the
+ * least-squares fit per map over the pixels where mask == 1 (Bmap is 0
+ * elsewhere).  Optional outputs (NULL skips): almB (nsim, lmax+1, lmax+1) the
+ * B alm of the masked maps, Btpl (npix, nsim) the template map alm2t(B'),
+ * coef (2, nsim) the fit (a0, a1) per map.  Needs pol != 0.  Allocates its
+ * temporaries (4 alm sets, 2 map sets; host calls 4 more map sets) for the
+ * duration of the call and returns when done. */
+int fsht_fix_eb(fsht_handle* h, double* Q, double* U, const double* mask, double* Bmap, double* almB, double* Btpl,
+                double* coef, int niter, int loc);
+
 /* Per-kernel-class timing with HIP events on the handle's stream.
  * fsht_profile(h, 1) starts recording (and clears earlier records); classes:
  * 0 ring FFT forward, 1 ring FFT inverse, 2 transposes, 3 Legendre analysis,

@@ -595,6 +595,54 @@ class OracleSHT:
 # --------------------------------------------------------------------------
 # alm utilities
 # --------------------------------------------------------------------------
+def fix_eb(sht, Q, U, mask, niter=0, remask=True):
+    """EB-leakage cleaning on a masked sky, src/sht_main.f90:801-941 (with
+    copy_with_mask_mini / apply_mask_mini / lin_resi_mini,
+    src/sht_core.f90:209-258), on an OracleSHT `sht`.
+
+    Returns dict(Bmap, Qm, Um, almB, Btpl, coef): the cleaned B map (0 outside
+    mask == 1), the masked inputs, the B alm of the masked maps, the template
+    map and the per-map fit (a0, a1).  remask=False drops the second masking
+    (only for the planted-mistake test)."""
+    mask = np.asarray(mask, dtype=np.float64)
+    Qm = np.asfortranarray(Q * mask[:, None])                 # :846-849
+    Um = np.asfortranarray(U * mask[:, None])
+    E, B = sht.qu2eb(Qm, Um, niter=niter)                    # :853-860
+    Q2, U2 = sht.eb2qu(E, np.zeros_like(E))                   # :866 (E2QU = the E part of EB2QU)
+    if remask:
+        Q2 = np.asfortranarray(Q2 * mask[:, None])            # :870-871
+        U2 = np.asfortranarray(U2 * mask[:, None])
+    _, B2 = sht.qu2eb(Q2, U2, niter=niter)                    # :876-881
+    y = sht.alm2t(B)                                          # :884-919 (scalar synthesis of B)
+    x = sht.alm2t(B2)
+    vid = np.nonzero(mask == 1.0)[0]                          # SHT.py:193
+    Bmap = np.zeros_like(y)
+    coef = np.zeros((2, y.shape[1]))
+    for k in range(y.shape[1]):                               # lin_resi_mini, sht_core.f90:239-258
+        xs, ys = x[vid, k], y[vid, k]
+        mx, my = xs.sum() / len(vid), ys.sum() / len(vid)
+        cxx = ((xs - mx) ** 2).sum()
+        cxy = ((ys - my) * (xs - mx)).sum()
+        a1 = cxy / cxx
+        a0 = my - mx * a1
+        Bmap[vid, k] = ys - a0 - a1 * xs
+        coef[:, k] = (a0, a1)
+    return dict(Bmap=Bmap, Qm=Qm, Um=Um, almB=B, Btpl=x, coef=coef)
+
+
+def galactic_cut_mask(nside, zcut=0.2, holes=0, seed=0):
+    """0/1 mask: 0 for |cos theta| < zcut and, optionally, `holes` random
+    discs of 0; 1 elsewhere."""
+    th, ph = pixel_angles(nside)
+    mask = (np.abs(np.cos(th)) >= zcut).astype(np.float64)
+    rng = np.random.default_rng(seed)
+    for _ in range(holes):
+        t0, p0 = np.arccos(rng.uniform(-1, 1)), rng.uniform(0, 2 * np.pi)
+        cosd = np.cos(th) * np.cos(t0) + np.sin(th) * np.sin(t0) * np.cos(ph - p0)
+        mask[cosd > np.cos(0.3)] = 0.0
+    return mask
+
+
 def convert_alm_healpy(alm):
     """src/convert_alm.f90:34-56 (convert_alm_healpy1) + SHT.py:204-210:
     square-packed real alm -> healpy m-major complex, shape (nalm, nsim)."""

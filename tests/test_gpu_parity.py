@@ -225,3 +225,47 @@ def test_single_order_lmax_4nside(torch_cuda, m):
     ad = (4 * np.pi / npix) * (P.T @ SN + sgn * (P.T @ SS))
     got = out[0, m:, m] + (1j * out[0, :n, L - m] if m > 0 else 0.0)
     assert maxrel(got, ad) < TOL
+
+
+@pytest.mark.parametrize("ns,lmax,nsim,niter", [(8, 16, 3, 0), (16, 32, 2, 2), (32, 80, 3, 1)])
+def test_fix_eb_vs_oracle(torch_cuda, ns, lmax, nsim, niter):
+    """EB-leakage cleaning through the C ABI (host arrays) against the oracle:
+    cleaned B map, template, B alm, and the in-place masking of Q and U."""
+    from fastsht_amd import SHT, empty_alms, empty_maps
+    print(f"case fix_eb nside={ns} lmax={lmax} nsim={nsim} niter={niter}", flush=True)
+    R = O.OracleSHT(ns, lmax, nsim)
+    rng = np.random.default_rng(ns + niter)
+    E, B = O.random_alm(lmax, nsim, seed=7), 0.05 * O.random_alm(lmax, nsim, seed=8)
+    Q, U = R.eb2qu(E, B)
+    Q = np.asfortranarray(Q + 1e-3 * rng.standard_normal(Q.shape))
+    U = np.asfortranarray(U + 1e-3 * rng.standard_normal(U.shape))
+    mask = O.galactic_cut_mask(ns, 0.2, holes=2, seed=ns)
+    ref = O.fix_eb(R, Q, U, mask, niter=niter)
+    sht = SHT(ns, lmax, nsim, niter=niter, pol=True)
+    tpl, ab = empty_maps(sht.npix, nsim), empty_alms(lmax, nsim)
+    Bmap = sht.fix_eb(Q, U, mask, Btpl_out=tpl, almB_out=ab)
+    assert np.array_equal(Q, ref["Qm"]) and np.array_equal(U, ref["Um"])
+    assert maxrel(ab, ref["almB"]) < TOL
+    assert maxrel(tpl, ref["Btpl"]) < TOL
+    assert maxrel(Bmap, ref["Bmap"]) < TOL
+    assert np.all(Bmap[mask != 1.0] == 0.0)
+
+
+def test_fix_eb_device_tensors(torch_cuda):
+    """Same cleaning on GPU tensors: Q and U masked in place on the device."""
+    torch = torch_cuda
+    from fastsht_amd import SHT, empty_maps
+    ns, lmax, nsim = 16, 40, 2
+    print(f"case fix_eb device nside={ns} lmax={lmax} nsim={nsim}", flush=True)
+    R = O.OracleSHT(ns, lmax, nsim)
+    Q, U = R.eb2qu(O.random_alm(lmax, nsim, seed=9), O.random_alm(lmax, nsim, seed=10))
+    mask = O.galactic_cut_mask(ns, 0.25, holes=1, seed=2)
+    ref = O.fix_eb(R, Q, U, mask, niter=1)
+    sht = SHT(ns, lmax, nsim, niter=1, pol=True)
+    Qd, Ud = empty_maps(sht.npix, nsim, device=True), empty_maps(sht.npix, nsim, device=True)
+    Qd.copy_(torch.from_numpy(np.asarray(Q)))
+    Ud.copy_(torch.from_numpy(np.asarray(U)))
+    Bd = sht.fix_eb(Qd, Ud, torch.from_numpy(mask).cuda())
+    torch.cuda.synchronize()
+    assert maxrel(Bd.cpu().numpy(), ref["Bmap"]) < TOL
+    assert maxrel(Qd.cpu().numpy(), ref["Qm"]) < 1e-15

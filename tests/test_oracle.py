@@ -293,3 +293,58 @@ def test_norm_and_filt():
         for l in range(m, lmax + 1):
             idx = m * (2 * lmax + 1 - m) // 2 + l
             np.testing.assert_allclose(f2[idx], hp_in[idx] * win[l], rtol=1e-15)
+
+
+# --------------------------------------------------------------------------
+# EB-leakage cleaning (src/sht_main.f90:801-941)
+# --------------------------------------------------------------------------
+def _pure_e_case(ns=16, lmax=32, nsim=3, seed=1):
+    R = O.OracleSHT(ns, lmax, nsim)
+    E = O.random_alm(lmax, nsim, seed=seed)
+    Q, U = R.eb2qu(E, np.zeros_like(E))
+    mask = O.galactic_cut_mask(ns, 0.2, holes=2, seed=3)
+    return R, Q, U, mask
+
+
+def test_fix_eb_removes_leakage_of_pure_e():
+    """A pure-E sky seen through a mask leaks into B; the cleaned B map keeps
+    only a small part of the leaked power on the observed pixels."""
+    R, Q, U, mask = _pure_e_case()
+    r = O.fix_eb(R, Q, U, mask, niter=3)
+    vid = mask == 1
+    naive = R.alm2t(r["almB"])
+    ratio = (r["Bmap"][vid] ** 2).sum() / (naive[vid] ** 2).sum()
+    assert ratio < 0.15
+    assert np.all(r["Bmap"][~vid] == 0.0)
+
+
+def test_fix_eb_fit_is_least_squares():
+    """The residual of a least-squares line fit sums to zero and is
+    orthogonal to the regressor over the fitted pixels."""
+    R, Q, U, mask = _pure_e_case(nsim=2, seed=4)
+    r = O.fix_eb(R, Q, U, mask, niter=1)
+    vid = mask == 1
+    res, x = r["Bmap"][vid], r["Btpl"][vid]
+    scale = np.abs(res).max() * np.abs(x).max() * vid.sum()
+    assert np.abs(res.sum(axis=0)).max() < 1e-12 * np.abs(res).max() * vid.sum()
+    assert np.abs((res * x).sum(axis=0)).max() < 1e-12 * scale
+    naive = R.alm2t(r["almB"])
+    a0, a1 = r["coef"]
+    assert np.allclose(res, naive[vid] - a0 - a1 * x, rtol=0, atol=1e-14 * np.abs(naive).max())
+
+
+def test_fix_eb_masks_inputs():
+    R, Q, U, mask = _pure_e_case(ns=8, lmax=16, nsim=2)
+    r = O.fix_eb(R, Q, U, mask)
+    assert np.array_equal(r["Qm"], Q * mask[:, None]) and np.array_equal(r["Um"], U * mask[:, None])
+
+
+def test_fix_eb_planted_mistake_is_caught():
+    """Skipping the second masking (a plausible slip) leaves the template
+    without the leakage, and the leakage criterion above fails."""
+    R, Q, U, mask = _pure_e_case()
+    r_bad = O.fix_eb(R, Q, U, mask, niter=3, remask=False)
+    vid = mask == 1
+    naive = R.alm2t(r_bad["almB"])
+    ratio = (r_bad["Bmap"][vid] ** 2).sum() / (naive[vid] ** 2).sum()
+    assert ratio > 0.15
