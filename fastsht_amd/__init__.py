@@ -5,5 +5,6 @@ hand-written gfx950 HIP kernels behind the C ABI of include/fastsht.h.
 """
 from .sht import SHT, empty_alms, empty_maps  # noqa: F401
 from ._lib import FastSHTError  # noqa: F401
+from .spectra import alm2cl, cl2alm  # noqa: F401
 
 __version__ = "0.1.0"

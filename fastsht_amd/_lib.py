@@ -26,6 +26,7 @@ EXPORTS = (
     "fsht_set_stream", "fsht_synchronize", "fsht_info", "fsht_ring_tab", "fsht_t2alm",
     "fsht_get_alm", "fsht_alm2t", "fsht_qu2eb", "fsht_qu2e", "fsht_qu2b", "fsht_eb2qu", "fsht_fix_eb",
     "fsht_profile", "fsht_profile_read", "fsht_norm_alm", "fsht_filt_alm", "fsht_convert_alm_healpy", "fsht_convert_alm_healpy_inv",
+    "fsht_alm2cl", "fsht_cl2alm",
 )
 
 _lib = None
@@ -76,6 +77,8 @@ def load():
     lib.fsht_filt_alm.argtypes = [dp, dp, i, i, vp]
     lib.fsht_convert_alm_healpy.argtypes = [dp, dp, i, i, vp]
     lib.fsht_convert_alm_healpy_inv.argtypes = [dp, dp, i, i, vp]
+    lib.fsht_alm2cl.argtypes = [dp, dp, dp, dp, i, i, i, vp]
+    lib.fsht_cl2alm.argtypes = [dp, dp, dp, dp, i, i, i, ctypes.c_ulonglong, vp]
     for name in EXPORTS:
         if name not in ("fsht_version", "fsht_last_error"):
             getattr(lib, name).restype = ctypes.c_int

@@ -19,6 +19,8 @@ hipError_t setup_tables(Handle* h);
 hipError_t launch_norm_alm(const double*, double*, const double*, const double*, double, int, int, hipStream_t);
 hipError_t launch_filt_alm(double*, const double*, int, int, hipStream_t);
 hipError_t launch_convert_healpy(const double*, double*, int, int, int, hipStream_t);
+hipError_t launch_alm2cl(const double*, const double*, const double*, double*, int, int, int, int, hipStream_t);
+hipError_t launch_cl2alm(const double*, double*, double*, double*, int, int, int, int, unsigned long long, hipStream_t);
 void synth_offsets(Handle* h, int ncomp);
 }  // namespace fsht
 
@@ -535,6 +537,22 @@ int fsht_convert_alm_healpy(const double* alm, double* alm_hp, int lmax, int nsi
 int fsht_convert_alm_healpy_inv(const double* alm_hp, double* alm, int lmax, int nsim, void* stream) {
   if (!alm || !alm_hp || lmax < 0 || nsim < 1) return fail(nullptr, FSHT_ERR_ARG, "bad argument");
   hipError_t e = fsht::launch_convert_healpy(alm_hp, alm, lmax, nsim, 1, (hipStream_t)stream);
+  return e == hipSuccess ? FSHT_OK : fail(nullptr, FSHT_ERR_HIP, hipGetErrorString(e));
+}
+
+int fsht_alm2cl(const double* almT, const double* almE, const double* almB, double* cl, int lmin, int lmax, int nsim,
+                void* stream) {
+  if (!almT || !cl || lmax < 0 || nsim < 1 || lmin < 0) return fail(nullptr, FSHT_ERR_ARG, "bad argument");
+  if ((almE == nullptr) != (almB == nullptr)) return fail(nullptr, FSHT_ERR_ARG, "almE and almB go together");
+  hipError_t e = fsht::launch_alm2cl(almT, almE, almB, cl, lmin, lmax, nsim, almE ? 6 : 1, (hipStream_t)stream);
+  return e == hipSuccess ? FSHT_OK : fail(nullptr, FSHT_ERR_HIP, hipGetErrorString(e));
+}
+
+int fsht_cl2alm(const double* cl, double* almT, double* almE, double* almB, int lmin, int lmax, int nsim,
+                unsigned long long seed, void* stream) {
+  if (!cl || !almT || lmax < 0 || nsim < 1 || lmin < 0) return fail(nullptr, FSHT_ERR_ARG, "bad argument");
+  if ((almE == nullptr) != (almB == nullptr)) return fail(nullptr, FSHT_ERR_ARG, "almE and almB go together");
+  hipError_t e = fsht::launch_cl2alm(cl, almT, almE, almB, lmin, lmax, nsim, almE ? 3 : 1, seed, (hipStream_t)stream);
   return e == hipSuccess ? FSHT_OK : fail(nullptr, FSHT_ERR_HIP, hipGetErrorString(e));
 }
 

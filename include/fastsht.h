@@ -136,6 +136,23 @@ int fsht_convert_alm_healpy(const double* alm, double* alm_hp, int lmax, int nsi
 /* Inverse of the above.  Replaces convert_alm_healpy_inv1 (src/convert_alm.f90:87-107). */
 int fsht_convert_alm_healpy_inv(const double* alm_hp, double* alm, int lmax, int nsim, void* stream);
 
+/* Angular power spectra of packed alm (device arrays): T only (almE = almB =
+ * NULL): cl (nsim, lmax+1); or all of TT EE BB TE TB EB: cl (nsim, lmax+1, 6),
+ * Fortran order.  cl(k, l, c) += sum_m fac_m * (Re x Re y + Im x Im y)/(2l+1)
+ * for l >= lmin (fac_0 = 1, fac_m = 2); like the reference it accumulates into
+ * cl, so pass zeros for a plain spectrum.  Replaces alm2cl_T1 / alm2cl_TEB1
+ * (src/alm2cl_TEB.f90:22-110). */
+int fsht_alm2cl(const double* almT, const double* almE, const double* almB, double* cl, int lmin, int lmax, int nsim,
+                void* stream);
+/* Seeded Gaussian alm with spectrum cl (device arrays): T only (almE = almB =
+ * NULL, cl has lmax+1 entries) or T, E, B with cl (lmax+1, 6) = TT EE BB TE TB
+ * EB; elements with l < lmin are 0.  Replaces cl2alm_T1 / cl2alm_TEB1
+ * (src/cl2alm_TEB.f90:25-170) with the same per-l linear combination, but the
+ * random stream is Philox4x32-10 + Box-Muller keyed by seed (MKL VSL's stream
+ * is not reproduced); each element depends only on (seed, sim, element). */
+int fsht_cl2alm(const double* cl, double* almT, double* almE, double* almB, int lmin, int lmax, int nsim,
+                unsigned long long seed, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

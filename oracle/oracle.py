@@ -643,6 +643,116 @@ def galactic_cut_mask(nside, zcut=0.2, holes=0, seed=0):
     return mask
 
 
+# --------------------------------------------------------------------------
+# Spectra and seeded alm (src/alm2cl_TEB.f90:22-110, src/cl2alm_TEB.f90:25-170)
+# --------------------------------------------------------------------------
+def alm2cl(almT, almE=None, almB=None, lmin=0, cl=None):
+    """cl(k, l, c) += sum_m fac_m (x_lm y_lm)/(2l+1) over Re and Im parts for
+    l >= lmin, in the reference's order (m ascending, Re term then Im term).
+    T alone: (nsim, lmax+1); with E and B: (nsim, lmax+1, 6) = TT EE BB TE TB EB."""
+    nsim, L, _ = almT.shape
+    pol = almE is not None
+    if cl is None:
+        cl = np.zeros((nsim, L, 6) if pol else (nsim, L), order="F")
+    comps = [almT, almE, almB] if pol else [almT]
+    pairs = [(0, 0), (1, 1), (2, 2), (0, 1), (0, 2), (1, 2)] if pol else [(0, 0)]
+    for m in range(L):
+        fac = 1.0 if m == 0 else 2.0
+        l1 = max(m, lmin)
+        if l1 > L - 1:
+            continue
+        ls = np.arange(l1, L)
+        d = 2.0 * ls + 1.0
+        for part in range(2 if m > 0 else 1):
+            rows = ls - m if part else ls
+            col = L - m if part else m
+            vals = [c[:, rows, col] for c in comps]
+            for ci, (a, b) in enumerate(pairs):
+                term = vals[a] * vals[b] * fac / d
+                if pol:
+                    cl[:, l1:, ci] = cl[:, l1:, ci] + term
+                else:
+                    cl[:, l1:] = cl[:, l1:] + term
+    return cl
+
+
+_PHILOX_M = (0xD2511F53, 0xCD9E8D57)
+_PHILOX_W = (0x9E3779B9, 0xBB67AE85)
+
+
+def philox4x32_10(ctr, key):
+    """Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC 2011) on uint32 arrays:
+    ctr (4, n), key (2,) -> (4, n)."""
+    c = [np.asarray(x, dtype=np.uint64) & 0xFFFFFFFF for x in ctr]
+    k0, k1 = int(key[0]) & 0xFFFFFFFF, int(key[1]) & 0xFFFFFFFF
+    mask = np.uint64(0xFFFFFFFF)
+    for _ in range(10):
+        p0 = np.uint64(_PHILOX_M[0]) * c[0]
+        p1 = np.uint64(_PHILOX_M[1]) * c[2]
+        c = [((p1 >> np.uint64(32)) ^ c[1] ^ np.uint64(k0)) & mask, p1 & mask,
+             ((p0 >> np.uint64(32)) ^ c[3] ^ np.uint64(k1)) & mask, p0 & mask]
+        k0 = (k0 + _PHILOX_W[0]) & 0xFFFFFFFF
+        k1 = (k1 + _PHILOX_W[1]) & 0xFFFFFFFF
+    return c
+
+
+def std_normal(seed, k, r, col, comp):
+    """Standard normal of storage element (sim k, row r, column col) of
+    component comp: Philox4x32-10 on the counter (k, r, col, comp) with key
+    (seed lo, seed hi), Box-Muller on two 53-bit uniforms (u1 in (0, 1])."""
+    k, r, col = np.broadcast_arrays(np.asarray(k), np.asarray(r), np.asarray(col))
+    comp = np.full(k.shape, comp)
+    c = philox4x32_10([k.ravel(), r.ravel(), col.ravel(), comp.ravel()],
+                      (seed & 0xFFFFFFFF, (seed >> 32) & 0xFFFFFFFF))
+    u1 = ((c[0] >> np.uint64(5)).astype(np.float64) * 67108864.0 + (c[1] >> np.uint64(6)).astype(np.float64)
+          + 1.0) / 9007199254740992.0
+    u2 = ((c[2] >> np.uint64(5)).astype(np.float64) * 67108864.0
+          + (c[3] >> np.uint64(6)).astype(np.float64)) / 9007199254740992.0
+    return (np.sqrt(-2.0 * np.log(u1)) * np.cos(6.283185307179586 * u2)).reshape(k.shape)
+
+
+def teb_factor(cl_l):
+    """Lower-triangular factor rows (a; b, u; c, v, w) of the TEB covariance at
+    one l, src/cl2alm_TEB.f90:49-56.  cl_l = (TT, EE, BB, TE, TB, EB)."""
+    TT, EE, BB, TE, TB, EB = cl_l
+    a = np.sqrt(TT)
+    b = TE / a
+    c = TB / a
+    u = np.sqrt(EE - b ** 2)
+    v = (TT * EB - TE * TB) / (u * TT)
+    w = np.sqrt(BB - TB / TT / u ** 2 * (EE * TB - TE * EB) - v * EB / u)
+    return a, b, c, u, v, w
+
+
+def cl2alm(cl, lmax, nsim, seed=0, lmin=0):
+    """Seeded Gaussian alm: cl (lmax+1,) -> almT; cl (lmax+1, 6) -> (T, E, B).
+    Variance 1 at m = 0 and 1/2 per part at m > 0, combined per l by
+    teb_factor (src/cl2alm_TEB.f90:58-100); l < lmin zeroed (:103-116)."""
+    L = lmax + 1
+    cl = np.asarray(cl, dtype=np.float64)
+    pol = cl.ndim == 2
+    kk, rr, cc = np.meshgrid(np.arange(nsim), np.arange(L), np.arange(L), indexing="ij")
+    re = rr >= cc
+    m = np.where(re, cc, L - cc)
+    ll = np.where(re, rr, rr + m)
+    fac = np.where(cc == 0, 1.0, 0.7071067811865476)
+    out = []
+    g = [std_normal(seed, kk, rr, cc, comp) * fac for comp in range(3 if pol else 1)]
+    if not pol:
+        aT = g[0] * np.sqrt(cl[ll])
+        aT[ll < lmin] = 0.0
+        return np.asfortranarray(aT)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        a, b, c, u, v, w = teb_factor([cl[ll, j] for j in range(6)])
+    aB = g[0] * c + g[1] * v + g[2] * w
+    aE = g[0] * b + g[1] * u
+    aT = g[0] * a
+    for x in (aT, aE, aB):
+        x[ll < lmin] = 0.0
+        out.append(np.asfortranarray(x))
+    return tuple(out)
+
+
 def convert_alm_healpy(alm):
     """src/convert_alm.f90:34-56 (convert_alm_healpy1) + SHT.py:204-210:
     square-packed real alm -> healpy m-major complex, shape (nalm, nsim)."""
