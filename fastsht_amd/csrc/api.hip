@@ -49,7 +49,7 @@ static void free_all(Handle* h) {
   void* ptrs[] = {h->d_geo, h->d_i0, h->d_ab, h->d_spin, h->d_aboff, h->d_tw, h->d_slot, h->d_seedoff,
                   h->d_seed_st, h->d_seed_v, h->d_synoff, h->d_syn_st, h->d_syn_v, h->d_goff, h->d_ana_items[0], h->d_ana_items[1],
                   h->d_syn_items[0], h->d_syn_items[1], h->d_fft_jobs, h->d_blue, h->d_chirp, h->d_filt,
-                  h->d_twid, h->ws_spec, h->ws_specT, h->ws_specR, h->ws_G, h->ws_alm[0], h->ws_alm[1]};
+                  h->d_twid, h->ws_spec, h->ws_specT, h->ws_specR, h->ws_G, h->ws_F, h->ws_alm[0], h->ws_alm[1]};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
 }
@@ -254,23 +254,43 @@ static int check_common(fsht_handle* H, int loc) {
   return FSHT_OK;
 }
 
-// analysis with optional Jacobi refinement; alm outputs are device pointers
-static int analysis_iter(Handle* h, double* a0, double* a1, int niter, int want0, int want1) {
+// analysis with optional refinement; alm outputs are device pointers.
+// method 0 (Jacobi, t2alm_iter_old): every order's residual has all synthesised
+// orders subtracted.  method 1 (immediate update, t2alm_iter): order m's
+// residual has only the aliases m' <= m of its slot subtracted.
+static int analysis_iter(Handle* h, double* a0, double* a1, int niter, int want0, int want1, int method = 0) {
   HIPCHK(h, timed(h, fsht::KC_GATHER, [&] { return fsht::launch_gather(h, h->ws_specT, h->ws_G); }));
   HIPCHK(h, timed(h, fsht::KC_ANALYSIS, [&] { return fsht::launch_analysis(h, h->ws_G, a0, a1, 0.0, want0, want1); }));
-  if (niter > 0 && !h->ws_specR) HIPCHK(h, hipMalloc((void**)&h->ws_specR, (size_t)h->npix * (h->spin ? 2 : 1) * h->nsimp * sizeof(double)));
+  if (niter > 0 && method == 0 && !h->ws_specR)
+    HIPCHK(h, hipMalloc((void**)&h->ws_specR, (size_t)h->npix * (h->spin ? 2 : 1) * h->nsimp * sizeof(double)));
+  if (niter > 0 && method == 1 && !h->ws_F) HIPCHK(h, hipMalloc((void**)&h->ws_F, (size_t)h->G_size * sizeof(double)));
   for (int it = 0; it < niter; ++it) {
     HIPCHK(h, timed(h, fsht::KC_SYNTHESIS, [&] { return fsht::launch_synthesis(h, a0, a1, h->ws_G); }));
-    HIPCHK(h, timed(h, fsht::KC_FOLD, [&] { return fsht::launch_fold(h, h->ws_G, h->ws_specT, h->ws_specR, 1); }));
-    HIPCHK(h, timed(h, fsht::KC_GATHER, [&] { return fsht::launch_gather(h, h->ws_specR, h->ws_G); }));
-    HIPCHK(h, timed(h, fsht::KC_ANALYSIS, [&] { return fsht::launch_analysis(h, h->ws_G, a0, a1, 1.0, want0, want1); }));
+    if (method == 0) {
+      HIPCHK(h, timed(h, fsht::KC_FOLD, [&] { return fsht::launch_fold(h, h->ws_G, h->ws_specT, h->ws_specR, 1); }));
+      HIPCHK(h, timed(h, fsht::KC_GATHER, [&] { return fsht::launch_gather(h, h->ws_specR, h->ws_G); }));
+      HIPCHK(h, timed(h, fsht::KC_ANALYSIS, [&] { return fsht::launch_analysis(h, h->ws_G, a0, a1, 1.0, want0, want1); }));
+    } else {
+      HIPCHK(h, timed(h, fsht::KC_GATHER, [&] { return fsht::launch_gather_prefix(h, h->ws_G, h->ws_specT, h->ws_F); }));
+      HIPCHK(h, timed(h, fsht::KC_ANALYSIS, [&] { return fsht::launch_analysis(h, h->ws_F, a0, a1, 1.0, want0, want1); }));
+    }
   }
   return FSHT_OK;
 }
 
 extern "C" {
 
+static int t2alm_impl(fsht_handle* H, const double* maps, double* alms, int niter, int loc, int method);
+
 int fsht_t2alm(fsht_handle* H, const double* maps, double* alms, int niter, int loc) {
+  return t2alm_impl(H, maps, alms, niter, loc, 0);
+}
+
+int fsht_t2alm_iter(fsht_handle* H, const double* maps, double* alms, int niter, int loc) {
+  return t2alm_impl(H, maps, alms, niter, loc, 1);
+}
+
+static int t2alm_impl(fsht_handle* H, const double* maps, double* alms, int niter, int loc, int method) {
   int rc = check_common(H, loc);
   if (rc) return rc;
   Handle* h = &H->h;
@@ -284,7 +304,7 @@ int fsht_t2alm(fsht_handle* H, const double* maps, double* alms, int niter, int 
     if ((rc = ensure_dev_alm(h, 0))) return rc;
     dst = h->ws_alm[0];
   }
-  if ((rc = analysis_iter(h, dst, nullptr, niter, 1, 0))) return rc;
+  if ((rc = analysis_iter(h, dst, nullptr, niter, 1, 0, method))) return rc;
   if (!keep && !loc) {
     HIPCHK(h, hipMemcpyAsync(alms, dst, alm_count(h) * sizeof(double), hipMemcpyDeviceToHost, h->stream));
     HIPCHK(h, hipStreamSynchronize(h->stream));

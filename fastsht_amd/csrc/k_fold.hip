@@ -159,6 +159,120 @@ __global__ __launch_bounds__(256) void k_gather(const RingGeo* __restrict__ geo,
   }
 }
 
+// ---------------------------------------------------------------------------
+// Residual analysis operand of the "immediate update" iteration
+// (t2alm_iter, src/sht_main.f90:161-264).  There the reference synthesises,
+// scatters (mat2fft_fast_iter) and re-analyses one order at a time, m
+// ascending, so the residual that order m reads on ring i has only the
+// synthesised aliases m' <= m of its slot subtracted:
+//   R(m, i) = Xt(slot) - n * sum_{m' <= m, m' ~ slot} X_{m'}(i)
+// (every synthesis uses the alm from before the sweep).  One thread computes
+// that prefix for its (m, ring, column) from the synthesis buffer G, then
+// applies the gather of k_gather.  G and F are distinct buffers.
+// ---------------------------------------------------------------------------
+template <int NCOMP>
+__global__ __launch_bounds__(256) void k_gather_prefix(const RingGeo* __restrict__ geo, const int* __restrict__ i0tab,
+                                                       const double2* __restrict__ tw, const int* __restrict__ slot,
+                                                       const double* __restrict__ G, const double* __restrict__ specT,
+                                                       double* __restrict__ F, const long long* __restrict__ goff,
+                                                       int nhr, int nnorth, int nsimp) {
+  const int m = blockIdx.x;
+  const int i0 = i0tab[m];
+  const int ldT = NCOMP * nsimp;
+  const int col = blockIdx.z * 64 + threadIdx.x;
+  if (col >= ldT) return;
+  const int comp = col / nsimp, sim = col - comp * nsimp;
+  const int nr = nhr + 1 - i0;
+  const int ncols = 2 * NCOMP * nsimp;
+  const long long fb = goff[m] / (NCOMP == 2 ? 1 : 2);
+  const long long odd = (long long)nr * ncols;
+  const int cre = (comp * 2) * nsimp + sim, cim = cre + nsimp;
+  const long long mrow = (long long)m * nnorth;
+  for (int i = i0 + blockIdx.y * 16 + threadIdx.y; i <= nhr && i < i0 + (blockIdx.y + 1) * 16; i += 4) {
+    const RingGeo g = geo[i];
+    const int n = g.nfft;
+    const int pos = m % n;
+    const int kk = pos <= n / 2 ? pos : n - pos;
+    const bool edge = (kk == 0) || (kk == n / 2);
+    // prefix of the alias chain of slot kk up to m (k_fold's sum, stopped at m)
+    double XNr = 0.0, XNi = 0.0, XSr = 0.0, XSi = 0.0;
+    for (int base = 0; base <= m; base += n) {
+      for (int side = 0; side < 2; ++side) {
+        int mp;
+        if (side == 0) {
+          mp = base + kk;
+        } else {
+          if (edge) break;
+          mp = base + n - kk;
+        }
+        if (mp > m) break;
+        const int i0p = i0tab[mp];  // <= i0(m) <= i: the ring is kept for mp
+        const int nrp = nhr + 1 - i0p;
+        const long long gb = goff[mp] / (NCOMP == 2 ? 1 : 2) + (long long)(i - i0p) * ncols;
+        const long long go = (long long)nrp * ncols;
+        const double er = G[gb + cre], ei = G[gb + cim];
+        const double orr = G[gb + go + cre], oi = G[gb + go + cim];
+        const double2 cs = tw[(long long)mp * nnorth + i];
+        const double nr_ = er + orr, ni_ = ei + oi;
+        const double WNr = nr_ * cs.x - ni_ * cs.y, WNi = nr_ * cs.y + ni_ * cs.x;
+        const double sr_ = er - orr, si_ = ei - oi;
+        const double WSr = sr_ * cs.x - si_ * cs.y, WSi = sr_ * cs.y + si_ * cs.x;
+        if (edge) {
+          const double f = (mp == 0) ? 1.0 : 2.0;
+          XNr += f * WNr;
+          XSr += f * WSr;
+        } else if (side == 0) {
+          XNr += WNr;
+          XNi += WNi;
+          XSr += WSr;
+          XSi += WSi;
+        } else {
+          XNr += WNr;
+          XNi -= WNi;
+          XSr += WSr;
+          XSi -= WSi;
+        }
+      }
+    }
+    const int sinfo = slot[mrow + i];
+    const int rel = sinfo >> 2, cls = sinfo & 3;
+    const double sg = cls == 1 ? 1.0 : (cls == 2 ? -1.0 : 0.0);
+    const double nf = (double)n;
+    const bool pair = g.i1S != g.i1N;
+    const double* pN = specT + (g.i1N + rel) * (long long)ldT + col;
+    const double* pS = specT + (g.i1S + rel) * (long long)ldT + col;
+    // residual slot pair (the Im slot does not exist on DC / Nyquist: sg = 0 there)
+    const double xr = pN[0] - nf * XNr;
+    const double xi = edge ? 0.0 : pN[ldT] - nf * XNi;
+    const double yr = pair ? pS[0] - nf * XSr : 0.0;
+    const double yi = (pair && !edge) ? pS[ldT] - nf * XSi : 0.0;
+    const double2 cs = tw[mrow + i];
+    const double Nr = xr * cs.x + sg * xi * cs.y;
+    const double Ni = sg * xi * cs.x - xr * cs.y;
+    const double Sr = yr * cs.x + sg * yi * cs.y;
+    const double Si = sg * yi * cs.x - yr * cs.y;
+    const double fs = pair ? 1.0 : 0.0;
+    const long long row = fb + (long long)(i - i0) * ncols;
+    F[row + cre] = fma(fs, Sr, Nr);
+    F[row + cim] = fma(fs, Si, Ni);
+    F[row + odd + cre] = fma(-fs, Sr, Nr);
+    F[row + odd + cim] = fma(-fs, Si, Ni);
+  }
+}
+
+hipError_t launch_gather_prefix(Handle* h, const double* G, const double* specT, double* F) {
+  const int ncomp = h->spin ? 2 : 1;
+  dim3 block(64, 4);
+  dim3 grid(h->lmax + 1, (h->nnorth + 15) / 16, (ncomp * h->nsimp + 63) / 64);
+  if (ncomp == 2)
+    hipLaunchKernelGGL(k_gather_prefix<2>, grid, block, 0, h->stream, h->d_geo, h->d_i0, h->d_tw, h->d_slot, G,
+                       specT, F, h->d_goff, h->nhr, h->nnorth, h->nsimp);
+  else
+    hipLaunchKernelGGL(k_gather_prefix<1>, grid, block, 0, h->stream, h->d_geo, h->d_i0, h->d_tw, h->d_slot, G,
+                       specT, F, h->d_goff, h->nhr, h->nnorth, h->nsimp);
+  return hipGetLastError();
+}
+
 hipError_t launch_gather(Handle* h, const double* specT, double* F) {
   const int ncomp = h->spin ? 2 : 1;
   dim3 block(64, 4);

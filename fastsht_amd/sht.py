@@ -145,25 +145,24 @@ class SHT:
         self.alms = alms_in
 
     def t2alm(self, maps_in=None, alms_in=None, method="old", get_alm=True):
-        """Temperature analysis (SHT.py:72-107): niter = 0 -> t2alm, niter > 0 ->
-        t2alm_iter_old (Jacobi).  get_alm=False keeps the result on the GPU
-        for get_alms()."""
+        """Temperature analysis (SHT.py:72-107): niter = 0 -> t2alm; niter > 0
+        -> t2alm_iter_old (Jacobi) for method='old', else t2alm_iter
+        (immediate update).  get_alm=False keeps the result on the GPU for
+        get_alms()."""
         maps = self.maps if maps_in is None else maps_in
         if maps is None:
             raise ValueError("No input maps AND no internal maps! Check the parameters or call set_maps(maps_in)!")
-        if self.niter > 0 and method != "old":
-            raise NotImplementedError("t2alm(method != 'old') selects t2alm_iter (Gauss-Seidel), "
-                                      "which is not provided; use method='old'")
         lib = _lib.load()
+        fn = lib.fsht_t2alm if (self.niter == 0 or method == "old") else lib.fsht_t2alm_iter
         mp, loc, mkeep = self._arg(maps, self._mshape, "maps")
         if not get_alm:
-            self._call(lib.fsht_t2alm, mp, None, self.niter, loc)
+            self._call(fn, mp, None, self.niter, loc)
             return None
         alms = self._like(maps, "alm") if alms_in is None else alms_in
         ap, aloc, akeep = self._arg(alms, self._ashape, "alms", out=True)
         if aloc != loc:
             raise ValueError("maps and alms must both be host arrays or both be GPU tensors")
-        self._call(lib.fsht_t2alm, mp, ap, self.niter, loc)
+        self._call(fn, mp, ap, self.niter, loc)
         return alms if alms_in is None else 0
 
     def alm2t(self, alms, maps_in=None):

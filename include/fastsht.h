@@ -76,6 +76,15 @@ int fsht_ring_tab(const fsht_handle* h, int* out);
  * fsht_get_alm (the reference's get_alm=False path, SHT.py:88-95). */
 int fsht_t2alm(fsht_handle* h, const double* maps, double* alms, int niter, int loc);
 
+/* Temperature analysis with the "immediate update" refinement: replaces
+ * t2alm_iter (src/sht_main.f90:161-264), selected in SHT.t2alm by
+ * method != 'old' (SHT.py:100-101).  Each of the niter sweeps re-analyses,
+ * for every order m, the residual of the maps with only the synthesised
+ * aliases m' <= m of its ring frequency subtracted (the reference's order-m
+ * scatter, m ascending); niter = 0 equals fsht_t2alm.  alms = NULL keeps the
+ * result on the device as in fsht_t2alm. */
+int fsht_t2alm_iter(fsht_handle* h, const double* maps, double* alms, int niter, int loc);
+
 /* Fetch the alms kept by fsht_t2alm(..., alms=NULL).  Replaces get_alm
  * (src/sht_main.f90:1294-1302) / SHT.get_alms (SHT.py:214-227). */
 int fsht_get_alm(fsht_handle* h, double* alms, int loc);

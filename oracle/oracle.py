@@ -464,6 +464,29 @@ class OracleSHT:
                 self.mat2alm(af, P, m, 0, 1.0, c1, c2)
         return alm
 
+    def t2alm_iter(self, maps, niter=0):
+        """Immediate-update refinement, sht_main.f90:161-264: within each sweep
+        every order m is synthesised, scattered into the residual buffer
+        (mat2fft_iter: set on first touch, accumulate on aliases) and
+        re-analysed at once, m ascending."""
+        nsim = maps.shape[1]
+        b1 = self.map2fft(maps)
+        alm, af = self._new_alm(nsim)
+        for m in range(self.lmax + 1):
+            (P,) = self.plm(m)
+            c1, c2 = self.fft2mat(b1, m)
+            self.mat2alm(af, P, m, 0, 0.0, c1, c2)
+        for _ in range(niter):
+            b2 = np.zeros_like(b1)
+            c1, c2 = self._caches(nsim)
+            for m in range(self.lmax + 1):
+                (P,) = self.plm(m)
+                self.alm2mat(af, P, m, 0, 0.0, c1, c2)
+                self.mat2fft_iter(b1, b2, m, c1, c2)
+                d1, d2 = self.fft2mat(b2, m)
+                self.mat2alm(af, P, m, 0, 1.0, d1, d2)
+        return alm
+
     def alm2t(self, alm):
         """sht_main.f90:96-155."""
         alm = np.asarray(alm)

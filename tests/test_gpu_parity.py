@@ -306,3 +306,17 @@ def test_cl2alm_vs_oracle(torch_cuda):
     assert maxrel(gT, rT) < 1e-13
     gd = cl2alm(cl[:, 0], lmax, nsim, seed=5, device=True)
     assert gd.is_cuda and maxrel(gd.cpu().numpy(), O.cl2alm(cl[:, 0], lmax, nsim, seed=5)) < 1e-13
+
+
+@pytest.mark.parametrize("ns,lmax,nsim,niter", [(4, 10, 3, 1), (8, 20, 2, 3), (16, 64, 5, 2), (32, 95, 4, 2)])
+def test_t2alm_iter_vs_oracle(torch_cuda, ns, lmax, nsim, niter):
+    """method != 'old' (immediate update, t2alm_iter) through the C ABI."""
+    from fastsht_amd import SHT
+    print(f"case t2alm_iter nside={ns} lmax={lmax} nsim={nsim} niter={niter}", flush=True)
+    R = O.OracleSHT(ns, lmax, nsim)
+    maps = _maps(np.random.default_rng(ns * 31 + lmax), ns, nsim)
+    sht = SHT(ns, lmax, nsim, niter=niter)
+    got = sht.t2alm(maps, method="new")
+    assert maxrel(got, R.t2alm_iter(maps, niter=niter)) < TOL
+    # method='old' on the same handle still runs the Jacobi refinement
+    assert maxrel(sht.t2alm(maps), R.t2alm(maps, niter=niter)) < TOL
