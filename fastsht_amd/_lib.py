@@ -26,7 +26,7 @@ EXPORTS = (
     "fsht_set_stream", "fsht_synchronize", "fsht_info", "fsht_ring_tab", "fsht_t2alm", "fsht_t2alm_iter",
     "fsht_get_alm", "fsht_alm2t", "fsht_qu2eb", "fsht_qu2e", "fsht_qu2b", "fsht_eb2qu", "fsht_fix_eb",
     "fsht_profile", "fsht_profile_read", "fsht_norm_alm", "fsht_filt_alm", "fsht_convert_alm_healpy", "fsht_convert_alm_healpy_inv",
-    "fsht_alm2cl", "fsht_cl2alm",
+    "fsht_alm2cl", "fsht_cl2alm", "fsht_host_alloc", "fsht_host_free",
 )
 
 _lib = None
@@ -79,6 +79,8 @@ def load():
     lib.fsht_convert_alm_healpy.argtypes = [dp, dp, i, i, vp]
     lib.fsht_convert_alm_healpy_inv.argtypes = [dp, dp, i, i, vp]
     lib.fsht_alm2cl.argtypes = [dp, dp, dp, dp, i, i, i, vp]
+    lib.fsht_host_alloc.argtypes = [ctypes.c_ulonglong, ctypes.POINTER(ctypes.c_void_p)]
+    lib.fsht_host_free.argtypes = [vp]
     lib.fsht_cl2alm.argtypes = [dp, dp, dp, dp, i, i, i, ctypes.c_ulonglong, vp]
     for name in EXPORTS:
         if name not in ("fsht_version", "fsht_last_error"):

@@ -10,6 +10,8 @@
 #include <string>
 
 #include <algorithm>
+#include <vector>
+#include <thread>
 
 #include "fastsht.h"
 #include "fsht_internal.h"
@@ -52,6 +54,10 @@ static void free_all(Handle* h) {
                   h->d_twid, h->ws_spec, h->ws_specT, h->ws_specR, h->ws_G, h->ws_F, h->ws_alm[0], h->ws_alm[1]};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
+  for (int b = 0; b < 2; ++b) {
+    if (h->stage[b]) (void)hipHostFree(h->stage[b]);
+    if (h->stage_ev[b]) (void)hipEventDestroy(h->stage_ev[b]);
+  }
 }
 
 static bool is_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
@@ -213,36 +219,177 @@ static int ensure_dev_alm(Handle* h, int k) {
   return FSHT_OK;
 }
 
-// forward ring FFT of ncomp map sets + transpose into ws_specT
+// ---------------------------------------------------------------------------
+// Host transfers.  Host arrays move in chunks of whole maps through two pinned
+// staging buffers: host threads copy chunk k into one buffer while the DMA of
+// chunk k-1 and the ring FFTs of the chunks already on the device run.
+// Pinned host arrays (hipHostMalloc, e.g. torch pin_memory) skip the staging
+// copy and are transferred chunk by chunk directly.
+// ---------------------------------------------------------------------------
+constexpr size_t kStageBytes = 128u << 20;
+
+static bool host_pinned(const void* p) {
+  hipPointerAttribute_t a;
+  if (hipPointerGetAttributes(&a, p) != hipSuccess) {
+    (void)hipGetLastError();  // pageable memory is reported as an error by some runtimes
+    return false;
+  }
+  return a.type == hipMemoryTypeHost;
+}
+
+static void par_memcpy(void* dst, const void* src, size_t bytes) {
+  const size_t part = 16u << 20;
+  const int nt = (int)std::min<size_t>(8, std::max<size_t>(1, bytes / part));
+  if (nt == 1) {
+    std::memcpy(dst, src, bytes);
+    return;
+  }
+  std::vector<std::thread> th;
+  const size_t per = (bytes / nt + 63) & ~size_t(63);
+  for (int t = 0; t < nt; ++t) {
+    const size_t a = std::min(bytes, (size_t)t * per), b = std::min(bytes, a + per);
+    if (b > a) th.emplace_back([=] { std::memcpy((char*)dst + a, (const char*)src + a, b - a); });
+  }
+  for (auto& x : th) x.join();
+}
+
+static int ensure_stage(Handle* h, size_t bytes) {
+  bytes = std::max(bytes, kStageBytes);
+  if (h->stage[0] && h->stage_bytes >= bytes) return FSHT_OK;
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  for (int b = 0; b < 2; ++b) {
+    if (h->stage[b]) HIPCHK(h, hipHostFree(h->stage[b]));
+    h->stage[b] = nullptr;
+    HIPCHK(h, hipHostMalloc((void**)&h->stage[b], bytes, hipHostMallocDefault));
+    if (!h->stage_ev[b]) HIPCHK(h, hipEventCreateWithFlags(&h->stage_ev[b], hipEventDisableTiming));
+  }
+  h->stage_bytes = bytes;
+  return FSHT_OK;
+}
+
+// host -> device of n doubles in chunks of `chunk`; after(off, len) enqueues
+// work on each chunk once its copy is enqueued.
+template <class F>
+static int h2d_chunked(Handle* h, double* dev, const double* host, size_t n, size_t chunk, F&& after) {
+  const bool pinned = host_pinned(host);
+  if (!pinned) {
+    int rc = ensure_stage(h, chunk * sizeof(double));
+    if (rc) return rc;
+  }
+  for (size_t off = 0, k = 0; off < n; off += chunk, ++k) {
+    const size_t len = std::min(chunk, n - off);
+    if (pinned) {
+      HIPCHK(h, hipMemcpyAsync(dev + off, host + off, len * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    } else {
+      const int b = (int)(k & 1);
+      HIPCHK(h, hipEventSynchronize(h->stage_ev[b]));  // the buffer's previous DMA is done
+      par_memcpy(h->stage[b], host + off, len * sizeof(double));
+      HIPCHK(h, hipMemcpyAsync(dev + off, h->stage[b], len * sizeof(double), hipMemcpyHostToDevice, h->stream));
+      HIPCHK(h, hipEventRecord(h->stage_ev[b], h->stream));
+    }
+    int rc = after(off, len);
+    if (rc) return rc;
+  }
+  return FSHT_OK;
+}
+
+// device -> host of n doubles in chunks; before(off, len) enqueues the work
+// that produces each chunk.  Returns when the host array holds the data.
+template <class F>
+static int d2h_chunked(Handle* h, double* host, const double* dev, size_t n, size_t chunk, F&& before) {
+  const bool pinned = host_pinned(host);
+  if (!pinned) {
+    int rc = ensure_stage(h, chunk * sizeof(double));
+    if (rc) return rc;
+  }
+  size_t poff = 0, plen = 0;
+  for (size_t off = 0, k = 0; off < n; off += chunk, ++k) {
+    const size_t len = std::min(chunk, n - off);
+    int rc = before(off, len);
+    if (rc) return rc;
+    if (pinned) {
+      HIPCHK(h, hipMemcpyAsync(host + off, dev + off, len * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+      continue;
+    }
+    const int b = (int)(k & 1);
+    // buffer b was last used by chunk k-2, copied out in the previous iteration
+    HIPCHK(h, hipMemcpyAsync(h->stage[b], dev + off, len * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipEventRecord(h->stage_ev[b], h->stream));
+    if (k > 0) {
+      HIPCHK(h, hipEventSynchronize(h->stage_ev[b ^ 1]));
+      par_memcpy(host + poff, h->stage[b ^ 1], plen * sizeof(double));
+    }
+    poff = off;
+    plen = len;
+  }
+  if (pinned) {
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+  } else if (plen) {
+    const int b = (int)(((n + chunk - 1) / chunk - 1) & 1);
+    HIPCHK(h, hipEventSynchronize(h->stage_ev[b]));
+    par_memcpy(host + poff, h->stage[b], plen * sizeof(double));
+  }
+  return FSHT_OK;
+}
+
+static size_t map_chunk(const Handle* h) {
+  const size_t per = std::max<size_t>(1, kStageBytes / ((size_t)h->npix * sizeof(double)));
+  return per * (size_t)h->npix;
+}
+
+static int h2d_alm(Handle* h, double* dev, const double* host) {
+  return h2d_chunked(h, dev, host, (size_t)alm_count(h), kStageBytes / sizeof(double),
+                     [](size_t, size_t) { return FSHT_OK; });
+}
+static int d2h_alm(Handle* h, double* host, const double* dev) {
+  return d2h_chunked(h, host, dev, (size_t)alm_count(h), kStageBytes / sizeof(double),
+                     [](size_t, size_t) { return FSHT_OK; });
+}
+
+// forward ring FFT of ncomp map sets + transpose into ws_specT.  Host maps
+// are uploaded chunk by chunk, each chunk's FFTs starting once it is on the GPU.
 static int forward_spectra(Handle* h, const double* m0, const double* m1, int loc) {
   const long long cnt = (long long)h->nsim * h->npix;
-  const double* in[2] = {m0, m1};
-  if (!loc) {  // stage host maps into ws_spec, transform in place
-    HIPCHK(h, hipMemcpyAsync(h->ws_spec, m0, cnt * sizeof(double), hipMemcpyHostToDevice, h->stream));
-    if (h->spin)
-      HIPCHK(h, hipMemcpyAsync(h->ws_spec + cnt, m1, cnt * sizeof(double), hipMemcpyHostToDevice, h->stream));
-    in[0] = h->ws_spec;
-    in[1] = h->ws_spec + cnt;
+  if (loc) {
+    const double* in[2] = {m0, m1};
+    HIPCHK(h, timed(h, fsht::KC_FFT_FWD, [&] { return fsht::launch_ring_fft(h, in, h->ws_spec, 0, nullptr); }));
+  } else {
+    const double* in[2] = {h->ws_spec, h->ws_spec + cnt};
+    const double* host[2] = {m0, m1};
+    for (int c = 0; c < (h->spin ? 2 : 1); ++c) {
+      int rc = h2d_chunked(h, h->ws_spec + c * cnt, host[c], (size_t)cnt, map_chunk(h), [&](size_t off, size_t len) {
+        const int map0 = c * h->nsim + (int)(off / h->npix), nm = (int)(len / h->npix);
+        HIPCHK(h, timed(h, fsht::KC_FFT_FWD,
+                        [&] { return fsht::launch_ring_fft(h, in, h->ws_spec, 0, nullptr, map0, nm); }));
+        return FSHT_OK;
+      });
+      if (rc) return rc;
+    }
   }
-  HIPCHK(h, timed(h, fsht::KC_FFT_FWD, [&] { return fsht::launch_ring_fft(h, in, h->ws_spec, 0, nullptr); }));
   HIPCHK(h, timed(h, fsht::KC_TRANSPOSE, [&] { return fsht::launch_transpose_to_T(h, h->ws_spec, h->ws_specT); }));
   return FSHT_OK;
 }
 
-// spectrum in ws_specT -> maps
+// spectrum in ws_specT -> maps.  Host maps are downloaded chunk by chunk
+// behind the inverse FFTs of the following chunks.
 static int inverse_spectra(Handle* h, double* m0, double* m1, int loc) {
   const long long cnt = (long long)h->nsim * h->npix;
   HIPCHK(h, timed(h, fsht::KC_TRANSPOSE, [&] { return fsht::launch_transpose_from_T(h, h->ws_specT, h->ws_spec); }));
-  double* out[2] = {m0, m1};
-  if (!loc) {
-    out[0] = h->ws_spec;
-    out[1] = h->ws_spec + cnt;
+  if (loc) {
+    double* out[2] = {m0, m1};
+    HIPCHK(h, timed(h, fsht::KC_FFT_INV, [&] { return fsht::launch_ring_fft(h, nullptr, h->ws_spec, 1, out); }));
+    return FSHT_OK;
   }
-  HIPCHK(h, timed(h, fsht::KC_FFT_INV, [&] { return fsht::launch_ring_fft(h, nullptr, h->ws_spec, 1, out); }));
-  if (!loc) {
-    HIPCHK(h, hipMemcpyAsync(m0, h->ws_spec, cnt * sizeof(double), hipMemcpyDeviceToHost, h->stream));
-    if (h->spin)
-      HIPCHK(h, hipMemcpyAsync(m1, h->ws_spec + cnt, cnt * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  double* out[2] = {h->ws_spec, h->ws_spec + cnt};
+  double* host[2] = {m0, m1};
+  for (int c = 0; c < (h->spin ? 2 : 1); ++c) {
+    int rc = d2h_chunked(h, host[c], h->ws_spec + c * cnt, (size_t)cnt, map_chunk(h), [&](size_t off, size_t len) {
+      const int map0 = c * h->nsim + (int)(off / h->npix), nm = (int)(len / h->npix);
+      HIPCHK(h, timed(h, fsht::KC_FFT_INV,
+                      [&] { return fsht::launch_ring_fft(h, nullptr, h->ws_spec, 1, out, map0, nm); }));
+      return FSHT_OK;
+    });
+    if (rc) return rc;
   }
   return FSHT_OK;
 }
@@ -306,8 +453,7 @@ static int t2alm_impl(fsht_handle* H, const double* maps, double* alms, int nite
   }
   if ((rc = analysis_iter(h, dst, nullptr, niter, 1, 0, method))) return rc;
   if (!keep && !loc) {
-    HIPCHK(h, hipMemcpyAsync(alms, dst, alm_count(h) * sizeof(double), hipMemcpyDeviceToHost, h->stream));
-    HIPCHK(h, hipStreamSynchronize(h->stream));
+    if ((rc = d2h_alm(h, alms, dst))) return rc;
   }
   h->have_dev_alm = keep ? 1 : 0;
   return FSHT_OK;
@@ -318,9 +464,12 @@ int fsht_get_alm(fsht_handle* H, double* alms, int loc) {
   if (rc) return rc;
   Handle* h = &H->h;
   if (!h->have_dev_alm || !h->ws_alm[0]) return fail(h, FSHT_ERR_STATE, "no device-resident alm: call t2alm with alms=NULL first");
-  HIPCHK(h, hipMemcpyAsync(alms, h->ws_alm[0], alm_count(h) * sizeof(double),
-                           loc ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, h->stream));
-  HIPCHK(h, hipStreamSynchronize(h->stream));
+  if (loc) {
+    HIPCHK(h, hipMemcpyAsync(alms, h->ws_alm[0], alm_count(h) * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+  } else if ((rc = d2h_alm(h, alms, h->ws_alm[0]))) {
+    return rc;
+  }
   return FSHT_OK;
 }
 
@@ -333,7 +482,7 @@ int fsht_alm2t(fsht_handle* H, const double* alms, double* maps, int loc) {
   const double* a = alms;
   if (!loc) {
     if ((rc = ensure_dev_alm(h, 0))) return rc;
-    HIPCHK(h, hipMemcpyAsync(h->ws_alm[0], alms, alm_count(h) * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    if ((rc = h2d_alm(h, h->ws_alm[0], alms))) return rc;
     a = h->ws_alm[0];
     h->have_dev_alm = 0;
   }
@@ -368,9 +517,8 @@ static int qu_forward(fsht_handle* H, const double* Q, const double* U, double* 
   if (!want1) b = e;
   if ((rc = analysis_iter(h, e, b, niter, want0, want1))) return rc;
   if (!loc) {
-    if (want0) HIPCHK(h, hipMemcpyAsync(E, e, alm_count(h) * sizeof(double), hipMemcpyDeviceToHost, h->stream));
-    if (want1) HIPCHK(h, hipMemcpyAsync(B, b, alm_count(h) * sizeof(double), hipMemcpyDeviceToHost, h->stream));
-    HIPCHK(h, hipStreamSynchronize(h->stream));
+    if (want0 && (rc = d2h_alm(h, E, e))) return rc;
+    if (want1 && (rc = d2h_alm(h, B, b))) return rc;
   }
   return FSHT_OK;
 }
@@ -397,8 +545,8 @@ int fsht_eb2qu(fsht_handle* H, const double* E, const double* B, double* Q, doub
   if (!loc) {
     if ((rc = ensure_dev_alm(h, 0))) return rc;
     if ((rc = ensure_dev_alm(h, 1))) return rc;
-    HIPCHK(h, hipMemcpyAsync(h->ws_alm[0], E, alm_count(h) * sizeof(double), hipMemcpyHostToDevice, h->stream));
-    HIPCHK(h, hipMemcpyAsync(h->ws_alm[1], B, alm_count(h) * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    if ((rc = h2d_alm(h, h->ws_alm[0], E))) return rc;
+    if ((rc = h2d_alm(h, h->ws_alm[1], B))) return rc;
     e = h->ws_alm[0];
     b = h->ws_alm[1];
     h->have_dev_alm = 0;
@@ -504,6 +652,20 @@ int fsht_fix_eb(fsht_handle* H, double* Q, double* U, const double* mask, double
   const hipError_t se = hipStreamSynchronize(h->stream);
   if (rc == FSHT_OK && se != hipSuccess) return fail(h, FSHT_ERR_HIP, std::string("fix_eb: ") + hipGetErrorString(se));
   return rc;
+}
+
+int fsht_host_alloc(unsigned long long bytes, void** out) {
+  if (!out) return fail(nullptr, FSHT_ERR_ARG, "out is NULL");
+  *out = nullptr;
+  hipError_t e = hipHostMalloc(out, std::max<unsigned long long>(bytes, 8), hipHostMallocDefault);
+  if (e != hipSuccess) return fail(nullptr, FSHT_ERR_NOMEM, std::string("hipHostMalloc: ") + hipGetErrorString(e));
+  return FSHT_OK;
+}
+
+int fsht_host_free(void* p) {
+  if (!p) return FSHT_OK;
+  hipError_t e = hipHostFree(p);
+  return e == hipSuccess ? FSHT_OK : fail(nullptr, FSHT_ERR_HIP, hipGetErrorString(e));
 }
 
 int fsht_profile(fsht_handle* H, int enable) {

@@ -57,8 +57,9 @@ struct Handle;
 // Kernel launchers (k_*.hip). All enqueue on h->stream.
 hipError_t launch_seeds(Handle* h);
 hipError_t build_bluestein(Handle* h);
+// maps [map0, map0 + nmap) of the comp*nsim + sim index space (nmap <= 0: all)
 hipError_t launch_ring_fft(Handle* h, const double* const* maps_comp, double* spec, int inverse,
-                           double* const* out_comp);
+                           double* const* out_comp, int map0 = 0, int nmap = 0);
 hipError_t launch_transpose_to_T(Handle* h, const double* spec, double* specT);
 hipError_t launch_transpose_from_T(Handle* h, const double* specT, double* spec);
 hipError_t launch_gather(Handle* h, const double* specT, double* F);
@@ -138,6 +139,11 @@ struct Handle {
   double* ws_alm[2] = {nullptr, nullptr};  // device alm (nsim,(lmax+1)^2) for host-pointer calls
   double* ws_map[2] = {nullptr, nullptr};  // device maps for host-pointer calls
   int have_dev_alm = 0;  // result of the last forward transform kept on device
+
+  // pinned staging of host arrays (two buffers, allocated on first host call)
+  double* stage[2] = {nullptr, nullptr};
+  hipEvent_t stage_ev[2] = {nullptr, nullptr};
+  size_t stage_bytes = 0;
 
   // current call's spin (0 or 2)
   int spin = 0;

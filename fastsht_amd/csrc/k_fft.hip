@@ -213,10 +213,11 @@ __global__ __launch_bounds__(256) void k_ring_fft(const FftJob* __restrict__ job
                                                   const double2* __restrict__ chirp, const double2* __restrict__ filt,
                                                   const double2* __restrict__ tw, int Mmax, long long npix, int nsim,
                                                   const double* __restrict__ in0, const double* __restrict__ in1,
-                                                  double* __restrict__ out0, double* __restrict__ out1, int inverse) {
+                                                  double* __restrict__ out0, double* __restrict__ out1, int inverse,
+                                                  int map0) {
   __shared__ double2 buf[MAXM];
   const FftJob jb = jobs[job0 + blockIdx.x];
-  const int mapi = blockIdx.y;
+  const int mapi = map0 + blockIdx.y;
   const int comp = mapi / nsim, k = mapi - comp * nsim;
   const double* in = (comp ? in1 : in0) + (long long)k * npix;
   double* out = (comp ? out1 : out0) + (long long)k * npix;
@@ -320,7 +321,7 @@ __global__ __launch_bounds__(256) void k_ring_fft(const FftJob* __restrict__ job
 }
 
 hipError_t launch_ring_fft(Handle* h, const double* const* in_comp, double* spec, int inverse,
-                           double* const* out_comp) {
+                           double* const* out_comp, int map0, int nmap) {
   // forward: in_comp = user maps per component, out = spec (+ comp offset)
   // inverse: in = spec, out_comp = user maps per component
   const int ncomp = h->spin ? 2 : 1;
@@ -329,7 +330,8 @@ hipError_t launch_ring_fft(Handle* h, const double* const* in_comp, double* spec
   const double* in1 = inverse ? spec + cstride : (ncomp > 1 ? in_comp[1] : in_comp[0]);
   double* out0 = inverse ? out_comp[0] : spec;
   double* out1 = inverse ? (ncomp > 1 ? out_comp[1] : out_comp[0]) : spec + cstride;
-  const int nmaps = ncomp * h->nsim;
+  const int nmaps = nmap > 0 ? nmap : ncomp * h->nsim;
+  if (nmap <= 0) map0 = 0;
   for (size_t c = 0; c + 1 < h->fft_class_begin.size(); ++c) {
     const int b = h->fft_class_begin[c], e = h->fft_class_begin[c + 1];
     if (e <= b) continue;
@@ -337,7 +339,8 @@ hipError_t launch_ring_fft(Handle* h, const double* const* in_comp, double* spec
     const int M = h->fft_class_M[c];
 #define FSHT_LAUNCH_FFT(MM)                                                                                      \
   hipLaunchKernelGGL(k_ring_fft<MM>, grid, dim3(256), 0, h->stream, h->d_fft_jobs, b, h->d_geo, h->d_blue,      \
-                     h->d_chirp, h->d_filt, h->d_twid, h->Mmax, h->npix, h->nsim, in0, in1, out0, out1, inverse)
+                     h->d_chirp, h->d_filt, h->d_twid, h->Mmax, h->npix, h->nsim, in0, in1, out0, out1, inverse,   \
+                     map0)
     switch (M) {
       case 256: FSHT_LAUNCH_FFT(256); break;
       case 512: FSHT_LAUNCH_FFT(512); break;

@@ -11,6 +11,7 @@ allocate either kind.
 from __future__ import annotations
 
 import ctypes
+import weakref
 
 import numpy as np
 
@@ -29,20 +30,38 @@ def _is_torch(x):
     return type(x).__module__.startswith("torch") and hasattr(x, "data_ptr")
 
 
-def empty_maps(npix, nsim, device=False):
-    """(npix, nsim) Fortran-order float64 maps on the host or the GPU."""
+def _pinned_empty(shape):
+    """Fortran-order float64 host array in page-locked memory (freed when the
+    array is collected); host-pointer transforms move it by direct DMA."""
+    lib = _lib.load()
+    n = int(np.prod(shape))
+    p = ctypes.c_void_p()
+    _lib.check(lib.fsht_host_alloc(n * 8, ctypes.byref(p)))
+    buf = (ctypes.c_double * max(n, 1)).from_address(p.value)
+    weakref.finalize(buf, lib.fsht_host_free, ctypes.c_void_p(p.value))
+    return np.frombuffer(buf, dtype=np.float64, count=n).reshape(shape, order="F")
+
+
+def empty_maps(npix, nsim, device=False, pinned=True):
+    """(npix, nsim) Fortran-order float64 maps on the GPU, or on the host
+    (page-locked unless pinned=False)."""
     if device:
         torch = _torch()
         return torch.empty((nsim, npix), dtype=torch.float64, device="cuda").t()
+    if pinned:
+        return _pinned_empty((npix, nsim))
     return np.empty((npix, nsim), dtype=np.float64, order="F")
 
 
-def empty_alms(lmax, nsim, device=False):
-    """(nsim, lmax+1, lmax+1) Fortran-order float64 alms on the host or the GPU."""
+def empty_alms(lmax, nsim, device=False, pinned=True):
+    """(nsim, lmax+1, lmax+1) Fortran-order float64 alms on the GPU, or on the
+    host (page-locked unless pinned=False)."""
     L = lmax + 1
     if device:
         torch = _torch()
         return torch.empty((L, L, nsim), dtype=torch.float64, device="cuda").permute(2, 1, 0)
+    if pinned:
+        return _pinned_empty((nsim, L, L))
     return np.empty((nsim, L, L), dtype=np.float64, order="F")
 
 

@@ -122,6 +122,13 @@ int fsht_eb2qu(fsht_handle* h, const double* E, const double* B, double* Q, doub
 int fsht_fix_eb(fsht_handle* h, double* Q, double* U, const double* mask, double* Bmap, double* almB, double* Btpl,
                 double* coef, int niter, int loc);
 
+/* Page-locked host memory for map / alm arrays: host-pointer calls move
+ * such arrays by DMA directly (other host arrays go through the handle's
+ * pinned staging buffers).  Replaces the numba.cuda.pinned_array outputs of
+ * SHT.py (SHT.py:80-84, 110-114). */
+int fsht_host_alloc(unsigned long long bytes, void** out);
+int fsht_host_free(void* p);
+
 /* Per-kernel-class timing with HIP events on the handle's stream.
  * fsht_profile(h, 1) starts recording (and clears earlier records); classes:
  * 0 ring FFT forward, 1 ring FFT inverse, 2 transposes, 3 Legendre analysis,
