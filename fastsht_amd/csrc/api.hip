@@ -51,7 +51,7 @@ static void free_all(Handle* h) {
   void* ptrs[] = {h->d_geo, h->d_i0, h->d_ab, h->d_spin, h->d_aboff, h->d_tw, h->d_slot, h->d_seedoff,
                   h->d_seed_st, h->d_seed_v, h->d_synoff, h->d_syn_st, h->d_syn_v, h->d_goff, h->d_ana_items[0], h->d_ana_items[1],
                   h->d_syn_items[0], h->d_syn_items[1], h->d_fft_jobs, h->d_blue, h->d_chirp, h->d_filt,
-                  h->d_twid, h->ws_spec, h->ws_specT, h->ws_specR, h->ws_G, h->ws_F, h->ws_alm[0], h->ws_alm[1]};
+                  h->d_twid, h->ws_spec, h->ws_specT, h->ws_G, h->ws_F, h->ws_alm[0], h->ws_alm[1]};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (int b = 0; b < 2; ++b) {
@@ -118,8 +118,8 @@ int fsht_create(int nside, int lmax, int nsim, int niter, int pol, fsht_handle**
   auto alloc = [&](double** p, long long n) -> hipError_t { return hipMalloc((void**)p, (size_t)n * sizeof(double)); };
   e = alloc(&h->ws_spec, nspec);
   if (e == hipSuccess) e = alloc(&h->ws_specT, nspecT);
-  if (e == hipSuccess && niter > 0) e = alloc(&h->ws_specR, nspecT);
   if (e == hipSuccess) e = alloc(&h->ws_G, nG);
+  if (e == hipSuccess && niter > 0) e = alloc(&h->ws_F, nG);
   if (e != hipSuccess) {
     free_all(h);
     (void)hipStreamDestroy(h->own_stream);
@@ -408,19 +408,14 @@ static int check_common(fsht_handle* H, int loc) {
 static int analysis_iter(Handle* h, double* a0, double* a1, int niter, int want0, int want1, int method = 0) {
   HIPCHK(h, timed(h, fsht::KC_GATHER, [&] { return fsht::launch_gather(h, h->ws_specT, h->ws_G); }));
   HIPCHK(h, timed(h, fsht::KC_ANALYSIS, [&] { return fsht::launch_analysis(h, h->ws_G, a0, a1, 0.0, want0, want1); }));
-  if (niter > 0 && method == 0 && !h->ws_specR)
-    HIPCHK(h, hipMalloc((void**)&h->ws_specR, (size_t)h->npix * (h->spin ? 2 : 1) * h->nsimp * sizeof(double)));
-  if (niter > 0 && method == 1 && !h->ws_F) HIPCHK(h, hipMalloc((void**)&h->ws_F, (size_t)h->G_size * sizeof(double)));
+  if (niter > 0 && !h->ws_F)
+    HIPCHK(h, hipMalloc((void**)&h->ws_F, (size_t)(h->G_size / (h->pol ? 1 : 2)) * sizeof(double)));
   for (int it = 0; it < niter; ++it) {
     HIPCHK(h, timed(h, fsht::KC_SYNTHESIS, [&] { return fsht::launch_synthesis(h, a0, a1, h->ws_G); }));
-    if (method == 0) {
-      HIPCHK(h, timed(h, fsht::KC_FOLD, [&] { return fsht::launch_fold(h, h->ws_G, h->ws_specT, h->ws_specR, 1); }));
-      HIPCHK(h, timed(h, fsht::KC_GATHER, [&] { return fsht::launch_gather(h, h->ws_specR, h->ws_G); }));
-      HIPCHK(h, timed(h, fsht::KC_ANALYSIS, [&] { return fsht::launch_analysis(h, h->ws_G, a0, a1, 1.0, want0, want1); }));
-    } else {
-      HIPCHK(h, timed(h, fsht::KC_GATHER, [&] { return fsht::launch_gather_prefix(h, h->ws_G, h->ws_specT, h->ws_F); }));
-      HIPCHK(h, timed(h, fsht::KC_ANALYSIS, [&] { return fsht::launch_analysis(h, h->ws_F, a0, a1, 1.0, want0, want1); }));
-    }
+    // residual + gather in one pass (the residual spectrum is never stored)
+    HIPCHK(h, timed(h, fsht::KC_FOLD,
+                    [&] { return fsht::launch_gather_prefix(h, h->ws_G, h->ws_specT, h->ws_F, method == 0 ? 1 : 0); }));
+    HIPCHK(h, timed(h, fsht::KC_ANALYSIS, [&] { return fsht::launch_analysis(h, h->ws_F, a0, a1, 1.0, want0, want1); }));
   }
   return FSHT_OK;
 }

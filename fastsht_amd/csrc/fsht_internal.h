@@ -63,7 +63,9 @@ hipError_t launch_ring_fft(Handle* h, const double* const* maps_comp, double* sp
 hipError_t launch_transpose_to_T(Handle* h, const double* spec, double* specT);
 hipError_t launch_transpose_from_T(Handle* h, const double* specT, double* spec);
 hipError_t launch_gather(Handle* h, const double* specT, double* F);
-hipError_t launch_gather_prefix(Handle* h, const double* G, const double* specT, double* F);
+// residual analysis operand straight from the synthesis buffer G: aliases m' <= m
+// (full = 0, immediate update) or all aliases (full = 1, Jacobi)
+hipError_t launch_gather_prefix(Handle* h, const double* G, const double* specT, double* F, int full);
 hipError_t launch_analysis(Handle* h, const double* F, double* alm0, double* alm1, double beta,
                            int want0, int want1);
 hipError_t launch_synthesis(Handle* h, const double* alm0, const double* alm1, double* G);
@@ -132,9 +134,8 @@ struct Handle {
   int ncomp_ws = 0;            // components the workspace is sized for (1 or 2)
   double* ws_spec = nullptr;   // ncomp*nsim maps of npix (sim-major)
   double* ws_specT = nullptr;  // npix rows of ncomp*nsimp (sim-fastest)
-  double* ws_specR = nullptr;  // same shape: residual / second spectrum
   double* ws_G = nullptr;      // m-major synthesis output
-  double* ws_F = nullptr;      // second m-major buffer (immediate-update iteration), allocated on first use
+  double* ws_F = nullptr;      // m-major residual analysis operand of the refinement sweeps
   long long G_size = 0;
   double* ws_alm[2] = {nullptr, nullptr};  // device alm (nsim,(lmax+1)^2) for host-pointer calls
   double* ws_map[2] = {nullptr, nullptr};  // device maps for host-pointer calls

@@ -169,13 +169,17 @@ __global__ __launch_bounds__(256) void k_gather(const RingGeo* __restrict__ geo,
 // (every synthesis uses the alm from before the sweep).  One thread computes
 // that prefix for its (m, ring, column) from the synthesis buffer G, then
 // applies the gather of k_gather.  G and F are distinct buffers.
+//
+// full = 1 sums the whole alias chain instead (every m' <= lmax whose ring
+// set includes ring i): the Jacobi residual of t2alm_iter_old, i.e. k_fold's
+// residual followed by k_gather, without writing the residual spectrum.
 // ---------------------------------------------------------------------------
 template <int NCOMP>
 __global__ __launch_bounds__(256) void k_gather_prefix(const RingGeo* __restrict__ geo, const int* __restrict__ i0tab,
                                                        const double2* __restrict__ tw, const int* __restrict__ slot,
                                                        const double* __restrict__ G, const double* __restrict__ specT,
                                                        double* __restrict__ F, const long long* __restrict__ goff,
-                                                       int nhr, int nnorth, int nsimp) {
+                                                       int nhr, int nnorth, int nsimp, int lmax, int full) {
   const int m = blockIdx.x;
   const int i0 = i0tab[m];
   const int ldT = NCOMP * nsimp;
@@ -194,9 +198,12 @@ __global__ __launch_bounds__(256) void k_gather_prefix(const RingGeo* __restrict
     const int pos = m % n;
     const int kk = pos <= n / 2 ? pos : n - pos;
     const bool edge = (kk == 0) || (kk == n / 2);
-    // prefix of the alias chain of slot kk up to m (k_fold's sum, stopped at m)
+    // alias chain of slot kk in ascending order (k_fold's sum), up to m
+    // (prefix) or through every order whose ring set includes ring i (full)
+    const int mlast = full ? lmax : m;
     double XNr = 0.0, XNi = 0.0, XSr = 0.0, XSi = 0.0;
-    for (int base = 0; base <= m; base += n) {
+    bool stop = false;
+    for (int base = 0; base <= mlast && !stop; base += n) {
       for (int side = 0; side < 2; ++side) {
         int mp;
         if (side == 0) {
@@ -205,8 +212,15 @@ __global__ __launch_bounds__(256) void k_gather_prefix(const RingGeo* __restrict
           if (edge) break;
           mp = base + n - kk;
         }
-        if (mp > m) break;
-        const int i0p = i0tab[mp];  // <= i0(m) <= i: the ring is kept for mp
+        if (mp > mlast) {
+          stop = true;
+          break;
+        }
+        const int i0p = i0tab[mp];  // i0 is non-decreasing in m
+        if (i < i0p) {              // ring dropped for mp and every larger order
+          stop = true;
+          break;
+        }
         const int nrp = nhr + 1 - i0p;
         const long long gb = goff[mp] / (NCOMP == 2 ? 1 : 2) + (long long)(i - i0p) * ncols;
         const long long go = (long long)nrp * ncols;
@@ -260,16 +274,16 @@ __global__ __launch_bounds__(256) void k_gather_prefix(const RingGeo* __restrict
   }
 }
 
-hipError_t launch_gather_prefix(Handle* h, const double* G, const double* specT, double* F) {
+hipError_t launch_gather_prefix(Handle* h, const double* G, const double* specT, double* F, int full) {
   const int ncomp = h->spin ? 2 : 1;
   dim3 block(64, 4);
   dim3 grid(h->lmax + 1, (h->nnorth + 15) / 16, (ncomp * h->nsimp + 63) / 64);
   if (ncomp == 2)
     hipLaunchKernelGGL(k_gather_prefix<2>, grid, block, 0, h->stream, h->d_geo, h->d_i0, h->d_tw, h->d_slot, G,
-                       specT, F, h->d_goff, h->nhr, h->nnorth, h->nsimp);
+                       specT, F, h->d_goff, h->nhr, h->nnorth, h->nsimp, h->lmax, full);
   else
     hipLaunchKernelGGL(k_gather_prefix<1>, grid, block, 0, h->stream, h->d_geo, h->d_i0, h->d_tw, h->d_slot, G,
-                       specT, F, h->d_goff, h->nhr, h->nnorth, h->nsimp);
+                       specT, F, h->d_goff, h->nhr, h->nnorth, h->nsimp, h->lmax, full);
   return hipGetLastError();
 }
 
