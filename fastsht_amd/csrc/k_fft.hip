@@ -118,12 +118,13 @@ hipError_t build_bluestein(Handle* h) {
     for (auto& x : th) x.join();
   }
   // group jobs by LDS class
-  const int classes[] = {256, 512, 1024, 2048, 4096, 8192};
+  // one class per transform length: kernels are compiled for their exact M
+  const int classes[] = {4, 8, 16, 32, 64, 128, 256, 512, 1024, 2048, 4096, 8192};
   h->fft_jobs.clear();
   h->fft_class_begin.clear();
   h->fft_class_M.clear();
   int lastM = 0;
-  for (int ci = 0; ci < 6; ++ci) {
+  for (int ci = 0; ci < (int)(sizeof(classes) / sizeof(classes[0])); ++ci) {
     const int cM = classes[ci];
     h->fft_class_begin.push_back((int)h->fft_jobs.size());
     h->fft_class_M.push_back(cM);
@@ -170,34 +171,65 @@ __device__ inline double2 cconj(double2 a) { return make_double2(a.x, -a.y); }
 __device__ inline double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
 __device__ inline double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
 
+// The transform length of an LDS class is a compile-time constant (every job
+// of the class has M = the class size), so each pass is unrolled and issues
+// all its twiddle loads up front instead of one dependent load per butterfly.
+// 256 threads; the twiddle table holds exp(-2 pi i k / Mmax), k < Mmax/2.
+
 // Decimation in frequency, forward (e^{-2 pi i jk/M}): natural in, bit-reversed out.
-__device__ void lds_dif(double2* buf, int M, int Mmax, const double2* __restrict__ tw) {
+template <int M>
+__device__ void lds_dif(double2* buf, int Mmax, const double2* __restrict__ tw) {
+  constexpr int NB = (M / 2 + 255) / 256;  // butterflies per thread per pass
+#pragma unroll
   for (int len = M; len >= 2; len >>= 1) {
     const int half = len >> 1;
     const int stride = Mmax / len;
-    for (int t = threadIdx.x; t < (M >> 1); t += blockDim.x) {
-      const int pos = t & (half - 1);
-      const int i = (t - pos) * 2 + pos;
-      const double2 u = buf[i], v = buf[i + half];
-      buf[i] = cadd(u, v);
-      buf[i + half] = cmul(csub(u, v), tw[pos * stride]);
+    double2 w[NB];
+#pragma unroll
+    for (int r = 0; r < NB; ++r) {
+      const int t = threadIdx.x + 256 * r;
+      w[r] = (t < M / 2) ? tw[(t & (half - 1)) * stride] : make_double2(1.0, 0.0);
+    }
+#pragma unroll
+    for (int r = 0; r < NB; ++r) {
+      const int t = threadIdx.x + 256 * r;
+      if (t < M / 2) {
+        const int pos = t & (half - 1);
+        const int i = (t - pos) * 2 + pos;
+        const double2 u = buf[i], v = buf[i + half];
+        buf[i] = cadd(u, v);
+        buf[i + half] = cmul(csub(u, v), w[r]);
+      }
     }
     __syncthreads();
   }
 }
 
 // Decimation in time, inverse (e^{+2 pi i jk/M}, unnormalised): bit-reversed in, natural out.
-__device__ void lds_dit_inv(double2* buf, int M, int Mmax, const double2* __restrict__ tw) {
+template <int M>
+__device__ void lds_dit_inv(double2* buf, int Mmax, const double2* __restrict__ tw) {
+  constexpr int NB = (M / 2 + 255) / 256;
+#pragma unroll
   for (int len = 2; len <= M; len <<= 1) {
     const int half = len >> 1;
     const int stride = Mmax / len;
-    for (int t = threadIdx.x; t < (M >> 1); t += blockDim.x) {
-      const int pos = t & (half - 1);
-      const int i = (t - pos) * 2 + pos;
-      const double2 u = buf[i];
-      const double2 v = cmul(buf[i + half], cconj(tw[pos * stride]));
-      buf[i] = cadd(u, v);
-      buf[i + half] = csub(u, v);
+    double2 w[NB];
+#pragma unroll
+    for (int r = 0; r < NB; ++r) {
+      const int t = threadIdx.x + 256 * r;
+      w[r] = (t < M / 2) ? cconj(tw[(t & (half - 1)) * stride]) : make_double2(1.0, 0.0);
+    }
+#pragma unroll
+    for (int r = 0; r < NB; ++r) {
+      const int t = threadIdx.x + 256 * r;
+      if (t < M / 2) {
+        const int pos = t & (half - 1);
+        const int i = (t - pos) * 2 + pos;
+        const double2 u = buf[i];
+        const double2 v = cmul(buf[i + half], w[r]);
+        buf[i] = cadd(u, v);
+        buf[i + half] = csub(u, v);
+      }
     }
     __syncthreads();
   }
@@ -223,7 +255,8 @@ __global__ __launch_bounds__(256) void k_ring_fft(const FftJob* __restrict__ job
   double* out = (comp ? out1 : out0) + (long long)k * npix;
   const RingGeo g = geo[jb.ring];
   const bool pair = g.i1S != g.i1N;
-  const int n = jb.n, M = jb.M, lg = ilog2(M);
+  constexpr int M = MAXM;  // every job of a class has M equal to the class size
+  const int n = jb.n, lg = ilog2(M);
   const bool isblue = jb.blue >= 0;
   const double2* w = isblue ? chirp + blue[jb.blue].chirp_off : nullptr;
   const double2* fl = isblue ? filt + blue[jb.blue].filt_off : nullptr;
@@ -240,11 +273,11 @@ __global__ __launch_bounds__(256) void k_ring_fft(const FftJob* __restrict__ job
       buf[j] = z;
     }
     __syncthreads();
-    lds_dif(buf, M, Mmax, tw);
+    lds_dif<M>(buf, Mmax, tw);
     if (isblue) {
       for (int r = threadIdx.x; r < M; r += blockDim.x) buf[r] = cmul(buf[r], fl[brev(r, lg)]);
       __syncthreads();
-      lds_dit_inv(buf, M, Mmax, tw);
+      lds_dit_inv<M>(buf, Mmax, tw);
     }
     // ---- separate the two real spectra, write PERM
     for (int kk = threadIdx.x; kk <= n / 2; kk += blockDim.x) {
@@ -306,11 +339,11 @@ __global__ __launch_bounds__(256) void k_ring_fft(const FftJob* __restrict__ job
     }
     __syncthreads();
     if (isblue) {
-      lds_dif(buf, M, Mmax, tw);
+      lds_dif<M>(buf, Mmax, tw);
       for (int r = threadIdx.x; r < M; r += blockDim.x) buf[r] = cmul(buf[r], fl[brev(r, lg)]);
       __syncthreads();
     }
-    lds_dit_inv(buf, M, Mmax, tw);
+    lds_dit_inv<M>(buf, Mmax, tw);
     for (int j = threadIdx.x; j < n; j += blockDim.x) {
       double2 z = buf[j];
       if (isblue) z = cconj(cmul(z, w[j]));
@@ -342,6 +375,12 @@ hipError_t launch_ring_fft(Handle* h, const double* const* in_comp, double* spec
                      h->d_chirp, h->d_filt, h->d_twid, h->Mmax, h->npix, h->nsim, in0, in1, out0, out1, inverse,   \
                      map0)
     switch (M) {
+      case 4: FSHT_LAUNCH_FFT(4); break;
+      case 8: FSHT_LAUNCH_FFT(8); break;
+      case 16: FSHT_LAUNCH_FFT(16); break;
+      case 32: FSHT_LAUNCH_FFT(32); break;
+      case 64: FSHT_LAUNCH_FFT(64); break;
+      case 128: FSHT_LAUNCH_FFT(128); break;
       case 256: FSHT_LAUNCH_FFT(256); break;
       case 512: FSHT_LAUNCH_FFT(512); break;
       case 1024: FSHT_LAUNCH_FFT(1024); break;
