@@ -125,7 +125,7 @@ struct Handle {
   BlueInfo* d_blue = nullptr;
   double2* d_chirp = nullptr;
   double2* d_filt = nullptr;
-  double2* d_twid = nullptr;  // exp(-2 pi i k / Mmax), k < Mmax/2
+  double2* d_twid = nullptr;  // exp(-2 pi i k / Mmax), k < Mmax
   int Mmax = 0;
   std::vector<BlueInfo> blue_host;
   std::vector<int> blue_n, blue_M;

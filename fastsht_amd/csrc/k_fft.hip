@@ -140,8 +140,8 @@ hipError_t build_bluestein(Handle* h) {
     return hipErrorInvalidValue;
   }
   h->Mmax = Mmax;
-  std::vector<double2> tw(Mmax / 2);
-  for (int k = 0; k < Mmax / 2; ++k) {
+  std::vector<double2> tw(Mmax);  // full circle: radix-4 passes read w^{3k}
+  for (int k = 0; k < Mmax; ++k) {
     const long double ang = -2.0L * 3.141592653589793238462643383279502884L * k / Mmax;
     tw[k] = make_double2((double)cosl(ang), (double)sinl(ang));
   }
@@ -173,65 +173,170 @@ __device__ inline double2 csub(double2 a, double2 b) { return make_double2(a.x -
 
 // The transform length of an LDS class is a compile-time constant (every job
 // of the class has M = the class size), so each pass is unrolled and issues
-// all its twiddle loads up front instead of one dependent load per butterfly.
-// 256 threads; the twiddle table holds exp(-2 pi i k / Mmax), k < Mmax/2.
+// all its twiddle loads up front.  Passes are radix 4 (two radix-2 stages
+// fused, half the LDS round trips and barriers), with one radix-2 stage when
+// log2(M) is odd.  256 threads; tw[k] = exp(-2 pi i k / Mmax) for k < Mmax
+// (the full circle, so w^{3k} is a table entry).
+
+__device__ inline double2 mul_mi(double2 a) { return make_double2(a.y, -a.x); }  // a * (-i)
+__device__ inline double2 mul_pi(double2 a) { return make_double2(-a.y, a.x); }  // a * (+i)
+
+template <int M>
+struct FftShape {
+  static constexpr int LG = (M >= 2) ? __builtin_ctz(M) : 0;
+  static constexpr bool ODD = (LG & 1) != 0;
+  static constexpr int NB2 = (M / 2 + 255) / 256;  // radix-2 butterflies per thread
+  static constexpr int NB4 = (M / 4 + 255) / 256;  // radix-4 butterflies per thread
+};
+
+// DIF radix-2 stage of span len (natural order positions)
+template <int M>
+__device__ inline void dif2(double2* buf, int len, int Mmax, const double2* __restrict__ tw) {
+  constexpr int NB = FftShape<M>::NB2;
+  const int half = len >> 1, stride = Mmax / len;
+  double2 w[NB];
+#pragma unroll
+  for (int r = 0; r < NB; ++r) {
+    const int t = threadIdx.x + 256 * r;
+    w[r] = (t < M / 2) ? tw[(t & (half - 1)) * stride] : make_double2(1.0, 0.0);
+  }
+#pragma unroll
+  for (int r = 0; r < NB; ++r) {
+    const int t = threadIdx.x + 256 * r;
+    if (t < M / 2) {
+      const int pos = t & (half - 1);
+      const int i = (t - pos) * 2 + pos;
+      const double2 u = buf[i], v = buf[i + half];
+      buf[i] = cadd(u, v);
+      buf[i + half] = cmul(csub(u, v), w[r]);
+    }
+  }
+  __syncthreads();
+}
+
+// DIF radix-4 pass = stages of span len and len/2
+template <int M>
+__device__ inline void dif4(double2* buf, int len, int Mmax, const double2* __restrict__ tw) {
+  constexpr int NB = FftShape<M>::NB4;
+  const int q = len >> 2, stride = Mmax / len;
+  double2 w1[NB], w2[NB], w3[NB];
+#pragma unroll
+  for (int r = 0; r < NB; ++r) {
+    const int t = threadIdx.x + 256 * r;
+    const int pos = t & (q - 1);
+    const bool ok = t < M / 4;
+    w1[r] = ok ? tw[pos * stride] : make_double2(1.0, 0.0);
+    w2[r] = ok ? tw[2 * pos * stride] : make_double2(1.0, 0.0);
+    w3[r] = ok ? tw[3 * pos * stride] : make_double2(1.0, 0.0);
+  }
+#pragma unroll
+  for (int r = 0; r < NB; ++r) {
+    const int t = threadIdx.x + 256 * r;
+    if (t < M / 4) {
+      const int pos = t & (q - 1);
+      const int i0 = (t - pos) * 4 + pos;
+      const double2 a0 = buf[i0], a1 = buf[i0 + q], a2 = buf[i0 + 2 * q], a3 = buf[i0 + 3 * q];
+      const double2 s02 = cadd(a0, a2), d02 = csub(a0, a2);
+      const double2 s13 = cadd(a1, a3), d13 = mul_mi(csub(a1, a3));
+      buf[i0] = cadd(s02, s13);
+      buf[i0 + q] = cmul(csub(s02, s13), w2[r]);
+      buf[i0 + 2 * q] = cmul(cadd(d02, d13), w1[r]);
+      buf[i0 + 3 * q] = cmul(csub(d02, d13), w3[r]);
+    }
+  }
+  __syncthreads();
+}
 
 // Decimation in frequency, forward (e^{-2 pi i jk/M}): natural in, bit-reversed out.
 template <int M>
 __device__ void lds_dif(double2* buf, int Mmax, const double2* __restrict__ tw) {
-  constexpr int NB = (M / 2 + 255) / 256;  // butterflies per thread per pass
-#pragma unroll
-  for (int len = M; len >= 2; len >>= 1) {
-    const int half = len >> 1;
-    const int stride = Mmax / len;
-    double2 w[NB];
-#pragma unroll
-    for (int r = 0; r < NB; ++r) {
-      const int t = threadIdx.x + 256 * r;
-      w[r] = (t < M / 2) ? tw[(t & (half - 1)) * stride] : make_double2(1.0, 0.0);
-    }
-#pragma unroll
-    for (int r = 0; r < NB; ++r) {
-      const int t = threadIdx.x + 256 * r;
-      if (t < M / 2) {
-        const int pos = t & (half - 1);
-        const int i = (t - pos) * 2 + pos;
-        const double2 u = buf[i], v = buf[i + half];
-        buf[i] = cadd(u, v);
-        buf[i + half] = cmul(csub(u, v), w[r]);
-      }
-    }
-    __syncthreads();
+  int len = M;
+  if (FftShape<M>::ODD) {
+    dif2<M>(buf, len, Mmax, tw);
+    len >>= 1;
   }
+#pragma unroll
+  for (int k = 0; k < FftShape<M>::LG / 2; ++k) {
+    dif4<M>(buf, len, Mmax, tw);
+    len >>= 2;
+  }
+}
+
+// DIT radix-2 stage of span len, inverse twiddles
+template <int M>
+__device__ inline void dit2(double2* buf, int len, int Mmax, const double2* __restrict__ tw) {
+  constexpr int NB = FftShape<M>::NB2;
+  const int half = len >> 1, stride = Mmax / len;
+  double2 w[NB];
+#pragma unroll
+  for (int r = 0; r < NB; ++r) {
+    const int t = threadIdx.x + 256 * r;
+    w[r] = (t < M / 2) ? cconj(tw[(t & (half - 1)) * stride]) : make_double2(1.0, 0.0);
+  }
+#pragma unroll
+  for (int r = 0; r < NB; ++r) {
+    const int t = threadIdx.x + 256 * r;
+    if (t < M / 2) {
+      const int pos = t & (half - 1);
+      const int i = (t - pos) * 2 + pos;
+      const double2 u = buf[i];
+      const double2 v = cmul(buf[i + half], w[r]);
+      buf[i] = cadd(u, v);
+      buf[i + half] = csub(u, v);
+    }
+  }
+  __syncthreads();
+}
+
+// DIT radix-4 pass = stages of span len/2 then len (inverse twiddles)
+template <int M>
+__device__ inline void dit4(double2* buf, int len, int Mmax, const double2* __restrict__ tw) {
+  constexpr int NB = FftShape<M>::NB4;
+  const int q = len >> 2, stride = Mmax / len;
+  double2 w1[NB], w2[NB], w3[NB];
+#pragma unroll
+  for (int r = 0; r < NB; ++r) {
+    const int t = threadIdx.x + 256 * r;
+    const int pos = t & (q - 1);
+    const bool ok = t < M / 4;
+    w1[r] = ok ? cconj(tw[pos * stride]) : make_double2(1.0, 0.0);
+    w2[r] = ok ? cconj(tw[2 * pos * stride]) : make_double2(1.0, 0.0);
+    w3[r] = ok ? cconj(tw[3 * pos * stride]) : make_double2(1.0, 0.0);
+  }
+#pragma unroll
+  for (int r = 0; r < NB; ++r) {
+    const int t = threadIdx.x + 256 * r;
+    if (t < M / 4) {
+      const int pos = t & (q - 1);
+      const int i0 = (t - pos) * 4 + pos;
+      const double2 a0 = buf[i0], a1 = cmul(buf[i0 + q], w2[r]);
+      const double2 a2 = cmul(buf[i0 + 2 * q], w1[r]), a3 = cmul(buf[i0 + 3 * q], w3[r]);
+      // span len/2: (a0, a1) and (a2, a3); span len: pairs (0, 2) and (1, 3)
+      const double2 b0 = cadd(a0, a1), b1 = csub(a0, a1);
+      const double2 b2 = cadd(a2, a3), b3 = mul_pi(csub(a2, a3));
+      buf[i0] = cadd(b0, b2);
+      buf[i0 + q] = cadd(b1, b3);
+      buf[i0 + 2 * q] = csub(b0, b2);
+      buf[i0 + 3 * q] = csub(b1, b3);
+    }
+  }
+  __syncthreads();
 }
 
 // Decimation in time, inverse (e^{+2 pi i jk/M}, unnormalised): bit-reversed in, natural out.
 template <int M>
 __device__ void lds_dit_inv(double2* buf, int Mmax, const double2* __restrict__ tw) {
-  constexpr int NB = (M / 2 + 255) / 256;
+  int len = 2;
+  if (FftShape<M>::ODD) {
+    dit2<M>(buf, len, Mmax, tw);
+    len = 8;
+  } else {
+    len = 4;
+  }
 #pragma unroll
-  for (int len = 2; len <= M; len <<= 1) {
-    const int half = len >> 1;
-    const int stride = Mmax / len;
-    double2 w[NB];
-#pragma unroll
-    for (int r = 0; r < NB; ++r) {
-      const int t = threadIdx.x + 256 * r;
-      w[r] = (t < M / 2) ? cconj(tw[(t & (half - 1)) * stride]) : make_double2(1.0, 0.0);
-    }
-#pragma unroll
-    for (int r = 0; r < NB; ++r) {
-      const int t = threadIdx.x + 256 * r;
-      if (t < M / 2) {
-        const int pos = t & (half - 1);
-        const int i = (t - pos) * 2 + pos;
-        const double2 u = buf[i];
-        const double2 v = cmul(buf[i + half], w[r]);
-        buf[i] = cadd(u, v);
-        buf[i + half] = csub(u, v);
-      }
-    }
-    __syncthreads();
+  for (int k = 0; k < FftShape<M>::LG / 2; ++k) {
+    dit4<M>(buf, len, Mmax, tw);
+    len <<= 2;
   }
 }
 
