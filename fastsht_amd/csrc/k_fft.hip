@@ -175,7 +175,9 @@ __device__ inline double2 csub(double2 a, double2 b) { return make_double2(a.x -
 // of the class has M = the class size), so each pass is unrolled and issues
 // all its twiddle loads up front.  Passes are radix 4 (two radix-2 stages
 // fused, half the LDS round trips and barriers), with one radix-2 stage when
-// log2(M) is odd.  256 threads; tw[k] = exp(-2 pi i k / Mmax) for k < Mmax
+// log2(M) is odd.  FftShape<M>::TPB threads (512 for the long transforms,
+// whose LDS footprint leaves one workgroup per CU); tw[k] = exp(-2 pi i k /
+// Mmax) for k < Mmax
 // (the full circle, so w^{3k} is a table entry).
 
 __device__ inline double2 mul_mi(double2 a) { return make_double2(a.y, -a.x); }  // a * (-i)
@@ -185,8 +187,9 @@ template <int M>
 struct FftShape {
   static constexpr int LG = (M >= 2) ? __builtin_ctz(M) : 0;
   static constexpr bool ODD = (LG & 1) != 0;
-  static constexpr int NB2 = (M / 2 + 255) / 256;  // radix-2 butterflies per thread
-  static constexpr int NB4 = (M / 4 + 255) / 256;  // radix-4 butterflies per thread
+  static constexpr int TPB = (M >= 4096) ? 512 : 256;  // threads per workgroup
+  static constexpr int NB2 = (M / 2 + TPB - 1) / TPB;    // radix-2 butterflies per thread
+  static constexpr int NB4 = (M / 4 + TPB - 1) / TPB;    // radix-4 butterflies per thread
 };
 
 // DIF radix-2 stage of span len (natural order positions)
@@ -197,12 +200,12 @@ __device__ inline void dif2(double2* buf, int len, int Mmax, const double2* __re
   double2 w[NB];
 #pragma unroll
   for (int r = 0; r < NB; ++r) {
-    const int t = threadIdx.x + 256 * r;
+    const int t = threadIdx.x + FftShape<M>::TPB * r;
     w[r] = (t < M / 2) ? tw[(t & (half - 1)) * stride] : make_double2(1.0, 0.0);
   }
 #pragma unroll
   for (int r = 0; r < NB; ++r) {
-    const int t = threadIdx.x + 256 * r;
+    const int t = threadIdx.x + FftShape<M>::TPB * r;
     if (t < M / 2) {
       const int pos = t & (half - 1);
       const int i = (t - pos) * 2 + pos;
@@ -222,7 +225,7 @@ __device__ inline void dif4(double2* buf, int len, int Mmax, const double2* __re
   double2 w1[NB], w2[NB], w3[NB];
 #pragma unroll
   for (int r = 0; r < NB; ++r) {
-    const int t = threadIdx.x + 256 * r;
+    const int t = threadIdx.x + FftShape<M>::TPB * r;
     const int pos = t & (q - 1);
     const bool ok = t < M / 4;
     w1[r] = ok ? tw[pos * stride] : make_double2(1.0, 0.0);
@@ -231,7 +234,7 @@ __device__ inline void dif4(double2* buf, int len, int Mmax, const double2* __re
   }
 #pragma unroll
   for (int r = 0; r < NB; ++r) {
-    const int t = threadIdx.x + 256 * r;
+    const int t = threadIdx.x + FftShape<M>::TPB * r;
     if (t < M / 4) {
       const int pos = t & (q - 1);
       const int i0 = (t - pos) * 4 + pos;
@@ -270,12 +273,12 @@ __device__ inline void dit2(double2* buf, int len, int Mmax, const double2* __re
   double2 w[NB];
 #pragma unroll
   for (int r = 0; r < NB; ++r) {
-    const int t = threadIdx.x + 256 * r;
+    const int t = threadIdx.x + FftShape<M>::TPB * r;
     w[r] = (t < M / 2) ? cconj(tw[(t & (half - 1)) * stride]) : make_double2(1.0, 0.0);
   }
 #pragma unroll
   for (int r = 0; r < NB; ++r) {
-    const int t = threadIdx.x + 256 * r;
+    const int t = threadIdx.x + FftShape<M>::TPB * r;
     if (t < M / 2) {
       const int pos = t & (half - 1);
       const int i = (t - pos) * 2 + pos;
@@ -296,7 +299,7 @@ __device__ inline void dit4(double2* buf, int len, int Mmax, const double2* __re
   double2 w1[NB], w2[NB], w3[NB];
 #pragma unroll
   for (int r = 0; r < NB; ++r) {
-    const int t = threadIdx.x + 256 * r;
+    const int t = threadIdx.x + FftShape<M>::TPB * r;
     const int pos = t & (q - 1);
     const bool ok = t < M / 4;
     w1[r] = ok ? cconj(tw[pos * stride]) : make_double2(1.0, 0.0);
@@ -305,7 +308,7 @@ __device__ inline void dit4(double2* buf, int len, int Mmax, const double2* __re
   }
 #pragma unroll
   for (int r = 0; r < NB; ++r) {
-    const int t = threadIdx.x + 256 * r;
+    const int t = threadIdx.x + FftShape<M>::TPB * r;
     if (t < M / 4) {
       const int pos = t & (q - 1);
       const int i0 = (t - pos) * 4 + pos;
@@ -345,7 +348,7 @@ __device__ inline int brev(int x, int lg) { return (int)(__brev((unsigned)x) >> 
 // Forward: maps (sim-major, npix per map) -> spec (same layout, PERM per ring).
 // Inverse: spec -> maps.  grid.x = job within the class, grid.y = map index.
 template <int MAXM>
-__global__ __launch_bounds__(256) void k_ring_fft(const FftJob* __restrict__ jobs, int job0,
+__global__ __launch_bounds__(FftShape<MAXM>::TPB) void k_ring_fft(const FftJob* __restrict__ jobs, int job0,
                                                   const RingGeo* __restrict__ geo, const BlueInfo* __restrict__ blue,
                                                   const double2* __restrict__ chirp, const double2* __restrict__ filt,
                                                   const double2* __restrict__ tw, int Mmax, long long npix, int nsim,
@@ -476,7 +479,8 @@ hipError_t launch_ring_fft(Handle* h, const double* const* in_comp, double* spec
     dim3 grid(e - b, nmaps);
     const int M = h->fft_class_M[c];
 #define FSHT_LAUNCH_FFT(MM)                                                                                      \
-  hipLaunchKernelGGL(k_ring_fft<MM>, grid, dim3(256), 0, h->stream, h->d_fft_jobs, b, h->d_geo, h->d_blue,      \
+  hipLaunchKernelGGL(k_ring_fft<MM>, grid, dim3(FftShape<MM>::TPB), 0, h->stream, h->d_fft_jobs, b, h->d_geo,    \
+                     h->d_blue,                                                                                   \
                      h->d_chirp, h->d_filt, h->d_twid, h->Mmax, h->npix, h->nsim, in0, in1, out0, out1, inverse,   \
                      map0)
     switch (M) {
