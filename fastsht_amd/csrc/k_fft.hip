@@ -187,7 +187,7 @@ template <int M>
 struct FftShape {
   static constexpr int LG = (M >= 2) ? __builtin_ctz(M) : 0;
   static constexpr bool ODD = (LG & 1) != 0;
-  static constexpr int TPB = (M >= 4096) ? 512 : 256;  // threads per workgroup
+  static constexpr int TPB = (M >= 8192) ? 1024 : ((M >= 2048) ? 512 : 256);  // threads per workgroup
   static constexpr int NB2 = (M / 2 + TPB - 1) / TPB;    // radix-2 butterflies per thread
   static constexpr int NB4 = (M / 4 + TPB - 1) / TPB;    // radix-4 butterflies per thread
 };
