@@ -150,7 +150,7 @@ def host_round_trip(sht, maps_dev, args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--nside", type=int, default=1024)
     ap.add_argument("--lmax", type=int, default=None)
@@ -207,16 +207,22 @@ def main():
         step()
     torch.cuda.synchronize()
     _lib.check(lib.fsht_profile(sht._h, 1), sht._h)
-    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    # one event per step boundary on the stream the library works on: the
+    # whole-run time and the per-step (one request of nsim maps) latencies
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps + 1)]
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    start.record()
-    for _ in range(args.steps):
+    ev[0].record()
+    for i in range(args.steps):
         step()
-    stop.record()
+        ev[i + 1].record()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    ms_total = start.elapsed_time(stop)
+    ms_total = ev[0].elapsed_time(ev[-1])
+    lat = sorted(ev[i].elapsed_time(ev[i + 1]) for i in range(args.steps))
+
+    def pct(p):
+        return lat[min(len(lat) - 1, int(math.ceil(p / 100.0 * len(lat))) - 1)]
     prof = (ctypes.c_double * 14)()
     _lib.check(lib.fsht_profile_read(sht._h, prof, 14), sht._h)
     _lib.check(lib.fsht_profile(sht._h, 0), sht._h)
@@ -259,6 +265,8 @@ def main():
         "roofline": {"bound": "mfma", "kernel": dom, "achieved": achieved, "peak": PEAK_FP64_MFMA_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_MFMA_TFLOPS, "traffic": traffic,
                      "flops_per_launch": flops_launch, "avg_launch_ms": avg_ms},
+        "latency_ms_per_step": {"p50": pct(50), "p99": pct(99), "min": lat[0], "max": lat[-1],
+                                "what": f"one t2alm(niter={args.niter})+alm2t request of {args.nsim} maps"},
         "kernels_ms_per_step": {k: v["ms"] / args.steps for k, v in kern.items()},
         "wall_s": wall,
         "roundtrip_alm_rel_err_vs_input": rel,
