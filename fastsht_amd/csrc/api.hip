@@ -74,8 +74,8 @@ const char* fsht_last_error(const fsht_handle* h) {
 int fsht_create(int nside, int lmax, int nsim, int niter, int pol, fsht_handle** out) {
   if (!out) return fail(nullptr, FSHT_ERR_ARG, "out handle pointer is NULL");
   *out = nullptr;
-  if (!is_pow2(nside) || nside > 1024)
-    return fail(nullptr, FSHT_ERR_ARG, "nside must be a power of two in [1, 1024]");
+  if (!is_pow2(nside) || nside > 2048)
+    return fail(nullptr, FSHT_ERR_ARG, "nside must be a power of two in [1, 2048]");
   if (lmax < 0 || lmax > 4 * nside)
     return fail(nullptr, FSHT_ERR_ARG, "lmax must be in [0, 4*nside]");
   if (nsim < 1) return fail(nullptr, FSHT_ERR_ARG, "nsim must be >= 1");

@@ -20,16 +20,20 @@ struct RingGeo {
   double s;       // sin(theta)
 };
 
-// One ring-FFT job: a north ring and its mirror (or the equator ring alone).
+// One ring-FFT job: a north ring and its mirror as one complex transform of
+// length n (half = 0), or, for rings whose pair transform would not fit in
+// LDS, one of the two real rings alone as a complex transform of length n/2
+// (half = 1: north ring, 2: south ring).
 struct FftJob {
   int ring;      // north ring index
   int n;         // ring length
-  int M;         // transform length used in LDS (n if a power of two, else Bluestein size)
-  int blue;      // index into the Bluestein tables, -1 for direct power-of-two rings
+  int M;         // transform length used in LDS (the transform length if a power of two, else Bluestein size)
+  int blue;      // index into the Bluestein tables, -1 for direct power-of-two transforms
+  int half;      // 0: N/S pair; 1 or 2: north or south ring alone (length-n/2 transform)
 };
 
 struct BlueInfo {
-  long long chirp_off;  // offset of w_j = exp(-i pi j^2 / n), j < n
+  long long chirp_off;  // offset of w_j = exp(-i pi j^2 / L), j < L (L = Bluestein length)
   long long filt_off;   // offset of FFT_M(b)/M (M entries)
 };
 

@@ -5,9 +5,12 @@
 // (slot 0 = Re X_0, slot 1 = Re X_{n/2}, slots 2k,2k+1 = Re/Im X_k), inverse
 // unnormalised.  Here one workgroup transforms one north ring together with
 // its south mirror for one map (z = x_N + i x_S, one complex FFT, then the two
-// real spectra are separated).  Power-of-two rings use a radix-2 FFT in LDS;
+// real spectra are separated).  Power-of-two rings use a radix-4 FFT in LDS;
 // other polar rings (length 4q with q not a power of two, often with a large
 // prime factor) use Bluestein's chirp-z convolution with a power-of-two FFT.
+// At nside 2048 the polar rings longer than 4096 pixels would need a 16384-
+// point Bluestein filter (256 KB of LDS); they are transformed one real ring
+// per workgroup through a half-length complex transform instead.
 #include <cmath>
 #include <thread>
 
@@ -56,38 +59,54 @@ static void host_fft(std::vector<double2>& a, int sign) {
 }
 
 // Job list (grouped by LDS size class), twiddles and Bluestein tables.
+// kMaxLds is the longest transform held in LDS (8192 complex doubles, 128 KB).
+// A polar ring of length n = 4q (q not a power of two) is transformed with
+// its mirror as one length-n Bluestein transform when its filter length
+// next_pow2(2n-1) fits; longer rings (nside 2048, n in (4096, 8192)) are
+// transformed one real ring at a time as a complex length-n/2 Bluestein
+// transform of the even/odd pixel pairs.
+static constexpr int kMaxLds = 8192;
+
 hipError_t build_bluestein(Handle* h) {
   const int nside = h->nside;
   const int nn = h->nnorth;
   struct J {
-    int ring, n, M;
+    int ring, n, M, L, half;  // L: Bluestein length (0 for a direct power-of-two transform)
   };
   std::vector<J> jobs;
   for (int i = 0; i < nn; ++i) {
     const int n = (i < nside) ? 4 * (i + 1) : 4 * nside;
-    const int M = is_pow2(n) ? n : next_pow2(2 * n - 1);
-    jobs.push_back({i, n, M});
+    if (is_pow2(n)) {
+      jobs.push_back({i, n, n, 0, 0});
+    } else if (next_pow2(2 * n - 1) <= kMaxLds) {
+      jobs.push_back({i, n, next_pow2(2 * n - 1), n, 0});
+    } else {
+      const int L = n / 2;  // polar rings are never the equator, so both halves exist
+      jobs.push_back({i, n, next_pow2(2 * L - 1), L, 1});
+      jobs.push_back({i, n, next_pow2(2 * L - 1), L, 2});
+    }
   }
-  // Bluestein tables per distinct non-power-of-two length
-  std::vector<int> blue_of_ring(nn, -1);
+  // Bluestein tables per distinct transform length
+  std::vector<int> blue_of_job(jobs.size(), -1);
   h->blue_n.clear();
   h->blue_M.clear();
   h->blue_host.clear();
   long long coff = 0, foff = 0;
-  for (auto& j : jobs) {
-    if (is_pow2(j.n)) continue;
+  for (size_t ji = 0; ji < jobs.size(); ++ji) {
+    const J& j = jobs[ji];
+    if (j.L == 0) continue;
     bool found = false;
-    for (size_t b = 0; b < h->blue_n.size(); ++b)
-      if (h->blue_n[b] == j.n) {
-        blue_of_ring[j.ring] = (int)b;
+    for (size_t b = 0; b < h->blue_n.size() && !found; ++b)
+      if (h->blue_n[b] == j.L) {
+        blue_of_job[ji] = (int)b;
         found = true;
       }
     if (found) continue;
-    blue_of_ring[j.ring] = (int)h->blue_n.size();
-    h->blue_n.push_back(j.n);
+    blue_of_job[ji] = (int)h->blue_n.size();
+    h->blue_n.push_back(j.L);
     h->blue_M.push_back(j.M);
     h->blue_host.push_back(BlueInfo{coff, foff});
-    coff += j.n;
+    coff += j.L;
     foff += j.M;
   }
   const int nb = (int)h->blue_n.size();
@@ -128,15 +147,17 @@ hipError_t build_bluestein(Handle* h) {
     const int cM = classes[ci];
     h->fft_class_begin.push_back((int)h->fft_jobs.size());
     h->fft_class_M.push_back(cM);
-    for (auto& j : jobs)
-      if (j.M <= cM && j.M > lastM) h->fft_jobs.push_back(FftJob{j.ring, j.n, j.M, blue_of_ring[j.ring]});
+    for (size_t ji = 0; ji < jobs.size(); ++ji) {
+      const J& j = jobs[ji];
+      if (j.M <= cM && j.M > lastM) h->fft_jobs.push_back(FftJob{j.ring, j.n, j.M, blue_of_job[ji], j.half});
+    }
     lastM = cM;
   }
   h->fft_class_begin.push_back((int)h->fft_jobs.size());
   int Mmax = 2;
   for (auto& j : jobs) Mmax = std::max(Mmax, j.M);
-  if (Mmax > 8192) {
-    h->err = "ring length needs an LDS transform larger than 8192 points (nside > 1024 not supported)";
+  if (Mmax > kMaxLds) {
+    h->err = "ring length needs an LDS transform longer than 8192 points (nside > 2048 not supported)";
     return hipErrorInvalidValue;
   }
   h->Mmax = Mmax;
@@ -345,6 +366,76 @@ __device__ void lds_dit_inv(double2* buf, int Mmax, const double2* __restrict__ 
 
 __device__ inline int brev(int x, int lg) { return (int)(__brev((unsigned)x) >> (32 - lg)); }
 
+// One real ring of length n = 2L (L not a power of two) transformed alone:
+// y_j = x_{2j} + i x_{2j+1}, j < L, goes through a length-L Bluestein
+// transform (chirp w, filter fl), and the real spectrum is split out of it.
+//   forward: A_k = (Y_k + conj Y_{L-k})/2, B_k = (Y_k - conj Y_{L-k})/(2i),
+//            X_k = A_k + e^{-2 pi i k/n} B_k for k = 0..L (Y indices mod L);
+//   inverse: Y'_k = (X_k + conj X_{L-k}) + i e^{+2 pi i k/n} (X_k - conj X_{L-k})
+//            for k < L; y = unnormalised inverse DFT_L(Y'), x_{2j} = Re y_j,
+//            x_{2j+1} = Im y_j.
+// in / out point at the ring's first pixel (PERM spectrum or pixels).
+template <int M>
+__device__ void ring_fft_half(double2* buf, int n, const double2* __restrict__ w, const double2* __restrict__ fl,
+                              const double2* __restrict__ tw, int Mmax, const double* __restrict__ in,
+                              double* __restrict__ out, int inverse) {
+  constexpr int lg = FftShape<M>::LG;
+  const int L = n >> 1;
+  const double rn = 2.0 / (double)n;
+  if (!inverse) {
+    for (int j = threadIdx.x; j < M; j += blockDim.x)
+      buf[j] = (j < L) ? cmul(make_double2(in[2 * j], in[2 * j + 1]), w[j]) : make_double2(0.0, 0.0);
+    __syncthreads();
+    lds_dif<M>(buf, Mmax, tw);
+    for (int r = threadIdx.x; r < M; r += blockDim.x) buf[r] = cmul(buf[r], fl[brev(r, lg)]);
+    __syncthreads();
+    lds_dit_inv<M>(buf, Mmax, tw);
+    for (int k = threadIdx.x; k <= L; k += blockDim.x) {
+      const int k1 = (k == L) ? 0 : k, k2 = (k == 0) ? 0 : L - k;
+      const double2 Yk = cmul(buf[k1], w[k1]);
+      const double2 Yc = cconj(cmul(buf[k2], w[k2]));
+      const double2 A = make_double2(0.5 * (Yk.x + Yc.x), 0.5 * (Yk.y + Yc.y));
+      const double2 B = make_double2(0.5 * (Yk.y - Yc.y), -0.5 * (Yk.x - Yc.x));
+      double s, c;
+      sincospi((double)k * rn, &s, &c);
+      const double2 X = cadd(A, cmul(B, make_double2(c, -s)));
+      if (k == 0) {
+        out[0] = X.x;
+      } else if (k == L) {
+        out[1] = X.x;
+      } else {
+        out[2 * k] = X.x;
+        out[2 * k + 1] = X.y;
+      }
+    }
+  } else {
+    for (int k = threadIdx.x; k < M; k += blockDim.x) {
+      double2 Z = make_double2(0.0, 0.0);
+      if (k < L) {
+        const int kc = L - k;  // 1..L
+        const double2 Xk = (k == 0) ? make_double2(in[0], 0.0) : make_double2(in[2 * k], in[2 * k + 1]);
+        const double2 Xc = (kc == L) ? make_double2(in[1], 0.0) : make_double2(in[2 * kc], -in[2 * kc + 1]);
+        double s, c;
+        sincospi((double)k * rn, &s, &c);
+        const double2 D = cmul(csub(Xk, Xc), make_double2(c, s));
+        const double2 Yp = cadd(cadd(Xk, Xc), mul_pi(D));
+        Z = cmul(cconj(Yp), w[k]);
+      }
+      buf[k] = Z;
+    }
+    __syncthreads();
+    lds_dif<M>(buf, Mmax, tw);
+    for (int r = threadIdx.x; r < M; r += blockDim.x) buf[r] = cmul(buf[r], fl[brev(r, lg)]);
+    __syncthreads();
+    lds_dit_inv<M>(buf, Mmax, tw);
+    for (int j = threadIdx.x; j < L; j += blockDim.x) {
+      const double2 y = cconj(cmul(buf[j], w[j]));
+      out[2 * j] = y.x;
+      out[2 * j + 1] = y.y;
+    }
+  }
+}
+
 // Forward: maps (sim-major, npix per map) -> spec (same layout, PERM per ring).
 // Inverse: spec -> maps.  grid.x = job within the class, grid.y = map index.
 template <int MAXM>
@@ -368,6 +459,13 @@ __global__ __launch_bounds__(FftShape<MAXM>::TPB) void k_ring_fft(const FftJob* 
   const bool isblue = jb.blue >= 0;
   const double2* w = isblue ? chirp + blue[jb.blue].chirp_off : nullptr;
   const double2* fl = isblue ? filt + blue[jb.blue].filt_off : nullptr;
+  if constexpr (M == kMaxLds) {
+    if (jb.half) {
+      const long long i1 = (jb.half == 1) ? g.i1N : g.i1S;
+      ring_fft_half<M>(buf, n, w, fl, tw, Mmax, in + i1, out + i1, inverse);
+      return;
+    }
+  }
 
   if (!inverse) {
     // ---- load z = x_N + i x_S

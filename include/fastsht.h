@@ -48,7 +48,7 @@ const char* fsht_last_error(const fsht_handle* h);
 /* Create a transform configuration on the current HIP device: ring geometry,
  * ring-skip table, Legendre start seeds and workspace stay resident.
  * Replaces SHT.__init__ (SHT.py:17-62) -> sht_data_alloc + sht_set_data
- * (src/sht_data_init.f90:2-311).  nside: power of two <= 1024;
+ * (src/sht_data_init.f90:2-311).  nside: power of two <= 2048;
  * 0 <= lmax <= 4*nside; pol != 0 enables the spin-2 calls. */
 int fsht_create(int nside, int lmax, int nsim, int niter, int pol, fsht_handle** out);
 
