@@ -43,6 +43,7 @@ struct LegItem {
   int par;     // parity / symmetry class (0: l+m even, 1: odd)
   int blk;     // analysis: l-block; synthesis: ring block
   int colblk;  // analysis: block of kAnaCols columns; synthesis: first 16-column tile
+  int r0;      // analysis: first ring (from i0(m), a multiple of 4) with a kept P_lm in the l-block
 };
 
 // Tunables (also used by the host to size work lists).
@@ -54,7 +55,11 @@ constexpr int kAnaCols = 64;      // analysis columns per workgroup (4 waves x 1
 constexpr int kSynRings = 256;    // synthesis rings per workgroup (4 waves x 64)
 constexpr int kSynCols = 64;      // synthesis columns per workgroup (spin 0; spin 2 uses 32)
 constexpr int kSegL = 256;        // synthesis: recurrence coefficients staged per wave (l values)
-constexpr int kFlushExp = -899;   // seeds: a value p*2^e is kept iff frexp-exponent(p)+e >= this
+// P_lm (and the spin-2 functions built from it) below 2^-100 is dropped:
+// the recurrence of a ring starts at its first l with |P_lm| >= 2^-100.  A
+// dropped term changes a sum of O(1) terms by < 2^-100 * 10^8 (10^8 bounds
+// the largest spin-2 factor, 2/sin^2 at the polar ring), far below FP64 rounding.
+constexpr int kKeepExp = -99;     // seeds: a value p*2^e is kept iff frexp-exponent(p)+e >= this
 
 struct Handle;
 

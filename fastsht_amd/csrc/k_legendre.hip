@@ -184,7 +184,11 @@ __global__ __launch_bounds__(256, SPIN ? FSHT_ANA_MINBLK2 : FSHT_ANA_MINBLK0) vo
   const int myr = SPIN ? (tid % RS) : tid;
 
   const long long fb = goff[m] / (SPIN ? 1 : 2);
-  for (int rb = 0; rb < nr; rb += RS) {
+  // rings before it.r0 have no kept P_lm in this l-block
+  for (int rb = it.r0; rb < nr; rb += RS) {
+    // rings of the sub-block that exist; wave w generates (and MFMA group w
+    // consumes) rings 64*(w % (RS/64)) .. +63, skipped past the last ring
+    const int kvalid = min(RS, nr - rb);
     // ---- this thread's recurrence state (its ring of the sub-block)
     Rec rec;
     {
@@ -234,17 +238,24 @@ __global__ __launch_bounds__(256, SPIN ? FSHT_ANA_MINBLK2 : FSHT_ANA_MINBLK0) vo
             Pt[tid][s >> 1] = par ? v : keep;
           }
         };
-        const bool inj = (rec.st >= lc - 1) && (rec.st <= lc + 31);
-        if (__any(inj))
-          gen32<SPIN, true>(rec, cf + 32 * t, scf + (SPIN ? 32 * t : 0), lc, m, myf, emit);
-        else
-          gen32<SPIN, false>(rec, cf + 32 * t, scf + (SPIN ? 32 * t : 0), lc, m, myf, emit);
+        if (64 * (wave % (RS / 64)) < kvalid) {
+          const bool inj = (rec.st >= lc - 1) && (rec.st <= lc + 31);
+          if (__any(inj))
+            gen32<SPIN, true>(rec, cf + 32 * t, scf + (SPIN ? 32 * t : 0), lc, m, myf, emit);
+          else
+            gen32<SPIN, false>(rec, cf + 32 * t, scf + (SPIN ? 32 * t : 0), lc, m, myf, emit);
+        }
         __syncthreads();
         if (active) {
 #pragma unroll
-          for (int ks = 0; ks < 64; ++ks) {
-            const double a = Pt[4 * ks + q][li];
-            D[t] = mfma64(a, bfr[ks], D[t]);
+          for (int g = 0; g < 4; ++g) {
+            if (64 * (g % (RS / 64)) < kvalid) {
+#pragma unroll
+              for (int ks = 16 * g; ks < 16 * g + 16; ++ks) {
+                const double a = Pt[4 * ks + q][li];
+                D[t] = mfma64(a, bfr[ks], D[t]);
+              }
+            }
           }
         }
         __syncthreads();
@@ -417,9 +428,20 @@ __device__ inline void synth_wave(SynShared<SPIN>& sh, const LegItem it, const R
 #pragma unroll
     for (int u = 0; u < NT; ++u) D[a][u] = d4{0.0, 0.0, 0.0, 0.0};
 
+  // first chunk: the one holding the wave's earliest kept l (P is 0 before it)
+  int stmin = rec.st;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) stmin = min(stmin, __shfl_xor(stmin, o));
+  stmin = __builtin_amdgcn_readfirstlane(stmin);  // wave-uniform: keeps the loop scalar
+  const int lc0 = (stmin > lmax) ? lmax + 1 : m + ((stmin - m) / 32) * 32;
+
   double bcur[NKS][NT], bnx[NKS][NT];
-  load_b(m, bcur);
-  for (int lc = m; lc <= lmax; lc += 32) {
+  if ((lc0 - m) % SEG != 0 && lc0 <= lmax) {  // start inside a segment
+    load_seg(lc0 - (lc0 - m) % SEG);
+    wave_lds_sync();
+  }
+  load_b(lc0, bcur);
+  for (int lc = lc0; lc <= lmax; lc += 32) {
     const int jseg = (lc - m) % SEG;
     if (jseg == 0) {
       wave_lds_sync();

@@ -178,7 +178,7 @@ void ring_tab(int nside, int nm, const std::vector<RingGeo>& geo, std::vector<in
 // ---------------------------------------------------------------------------
 // Device: Legendre start seeds.  For order m, kept north ring i and analysis
 // l-block b (l in [m + b*LB, m + (b+1)*LB)), store the first l of the block at
-// which P_lm is non-negligible (>= 2^-900), st, and P_st, P_st+1.  A plain
+// which P_lm is kept (>= 2^-100), st, and P_st, P_st+1.  A plain
 // FP64 recurrence restarted from these two values reproduces the exponent-
 // rescaled recurrence bit for bit.  Block 0 doubles as the synthesis start.
 // ---------------------------------------------------------------------------
@@ -260,7 +260,7 @@ __global__ void k_seeds(int lmax, int nhr, const RingGeo* __restrict__ geo, cons
     if (first < 0 && p != 0.0) {
       int fp;
       frexp(p, &fp);
-      if ((long long)fp + e >= kFlushExp) first = l;
+      if ((long long)fp + e >= kKeepExp) first = l;
     }
     if (first < 0) continue;
     const double val = ldexp(p, (int)e);
@@ -305,6 +305,79 @@ hipError_t launch_seeds(Handle* h) {
   if (e != hipSuccess) return e;
   e = hipStreamSynchronize(h->stream);
   hipFree(d_dfac);
+  return e;
+}
+
+// Work bounds from the seeds, for the work lists.  Per (m, analysis l-block):
+// the first ring (from i0(m)) with a kept P_lm in the block, nr(m) if none.
+// Per (m, group of 64 rings = one synthesis wave): the group's first kept l.
+__global__ __launch_bounds__(256) void k_starts(int lmax, int nhr, const int* __restrict__ i0,
+                                               const long long* __restrict__ seedoff,
+                                               const int* __restrict__ seed_st, const long long* __restrict__ synoff,
+                                               const int* __restrict__ syn_st, const long long* __restrict__ blkoff,
+                                               int* __restrict__ ana_r0, const long long* __restrict__ grpoff,
+                                               int* __restrict__ syn_l0) {
+  __shared__ int red[256];
+  const int m = blockIdx.x;
+  const int nr = nhr + 1 - i0[m];
+  const int nb = (lmax + 1 - m + kAnaLB - 1) / kAnaLB;
+  for (int b = 0; b < nb; ++b) {
+    const int* st = seed_st + seedoff[m] + (long long)b * nr;
+    int r0 = nr;
+    for (int r = threadIdx.x; r < nr; r += 256)
+      if (st[r] != 0x3fffffff) r0 = min(r0, r);
+    red[threadIdx.x] = r0;
+    __syncthreads();
+    for (int k = 128; k > 0; k >>= 1) {
+      if ((int)threadIdx.x < k) red[threadIdx.x] = min(red[threadIdx.x], red[threadIdx.x + k]);
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) ana_r0[blkoff[m] + b] = red[0];
+    __syncthreads();
+  }
+  const int ng = (nr + 63) / 64;
+  for (int g = threadIdx.x; g < ng; g += 256) {
+    int l0 = 0x3fffffff;
+    const int e = min(nr, 64 * g + 64);
+    for (int r = 64 * g; r < e; ++r) l0 = min(l0, syn_st[synoff[m] + r]);
+    syn_l0[grpoff[m] + g] = l0;
+  }
+}
+
+static hipError_t compute_starts(Handle* h, std::vector<long long>& blkoff, std::vector<int>& ana_r0,
+                                 std::vector<long long>& grpoff, std::vector<int>& syn_l0) {
+  const int lmax = h->lmax;
+  blkoff.assign(lmax + 2, 0);
+  grpoff.assign(lmax + 2, 0);
+  for (int m = 0; m <= lmax; ++m) {
+    const int nr = h->nhr + 1 - h->i0[m];
+    blkoff[m + 1] = blkoff[m] + h->nblk[m];
+    grpoff[m + 1] = grpoff[m] + (nr + 63) / 64;
+  }
+  ana_r0.assign(blkoff[lmax + 1], 0);
+  syn_l0.assign(grpoff[lmax + 1], 0);
+  long long* d_off = nullptr;
+  int* d_out = nullptr;
+  const size_t noff = blkoff.size() + grpoff.size(), nout = ana_r0.size() + syn_l0.size();
+  hipError_t e = hipMalloc(&d_off, noff * sizeof(long long));
+  if (e == hipSuccess) e = hipMalloc(&d_out, std::max<size_t>(1, nout) * sizeof(int));
+  if (e == hipSuccess)
+    e = hipMemcpy(d_off, blkoff.data(), blkoff.size() * sizeof(long long), hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = hipMemcpy(d_off + blkoff.size(), grpoff.data(), grpoff.size() * sizeof(long long), hipMemcpyHostToDevice);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(k_starts, dim3(lmax + 1), dim3(256), 0, h->stream, lmax, h->nhr, h->d_i0, h->d_seedoff,
+                       h->d_seed_st, h->d_synoff, h->d_syn_st, d_off, d_out, d_off + blkoff.size(),
+                       d_out + ana_r0.size());
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+  if (e == hipSuccess && !ana_r0.empty())
+    e = hipMemcpy(ana_r0.data(), d_out, ana_r0.size() * sizeof(int), hipMemcpyDeviceToHost);
+  if (e == hipSuccess && !syn_l0.empty())
+    e = hipMemcpy(syn_l0.data(), d_out + ana_r0.size(), syn_l0.size() * sizeof(int), hipMemcpyDeviceToHost);
+  if (d_off) (void)hipFree(d_off);
+  if (d_out) (void)hipFree(d_out);
   return e;
 }
 
@@ -429,7 +502,7 @@ static std::vector<LegItem> xcd_order(const std::vector<LegItem>& v, const std::
   }
   size_t per = 0;
   for (const auto& l : lists) per = std::max(per, l.size());
-  std::vector<LegItem> out(per * kXcd, LegItem{-1, 0, 0, 0});
+  std::vector<LegItem> out(per * kXcd, LegItem{-1, 0, 0, 0, 0});
   for (int x = 0; x < kXcd; ++x)
     for (size_t j = 0; j < lists[x].size(); ++j) out[j * kXcd + x] = lists[x][j];
   return out;
@@ -447,6 +520,10 @@ static hipError_t upload_items(const std::vector<LegItem>& v, const std::vector<
 
 static hipError_t build_items(Handle* h) {
   const int lmax = h->lmax;
+  std::vector<long long> blkoff, grpoff;
+  std::vector<int> ana_r0, syn_l0;
+  hipError_t es = compute_starts(h, blkoff, ana_r0, grpoff, syn_l0);
+  if (es != hipSuccess) return es;
   for (int sp = 0; sp < 2; ++sp) {
     const int ncomp = sp ? 2 : 1;
     const int ncols = 2 * ncomp * h->nsimp;
@@ -459,20 +536,32 @@ static hipError_t build_items(Handle* h) {
         const int ncb = ceil_div(ncols, kAnaCols);
         for (int b = 0; b < h->nblk[m]; ++b) {
           const int nl = std::min(kAnaLB, lmax + 1 - m - b * kAnaLB);
+          int r0 = ana_r0[blkoff[m] + b];
+          if (r0 < nr) r0 &= ~3;  // whole MFMA K-steps of 4 rings
+          const double rings = 64.0 * ceil_div(nr - r0, 64);
           for (int cb = 0; cb < ncb; ++cb) {
-            ana.push_back(LegItem{m, par, b, cb});
-            ana_cost.push_back((double)nr * (nl + 32));
+            ana.push_back(LegItem{m, par, b, cb, r0});
+            ana_cost.push_back(rings * (nl + 32) + 64.0 * 32);
           }
         }
-        // synthesis: one ring block against all l of m (first 16-column tile t0)
+        // synthesis: one ring block against the kept l of m (first 16-column
+        // tile t0); a wave runs the 32-l chunks from its rings' first kept l
         const int nrb = ceil_div(nr, kSynRings);
         const int ntile = ncols / 16, tpi = sp ? 2 : 4;
-        const double chunks = (double)ceil_div(lmax + 1 - m, 32) + 1.0;
-        for (int rb = 0; rb < nrb; ++rb)
+        for (int rb = 0; rb < nrb; ++rb) {
+          double chunks = 0.0;
+          for (int w = 0; w < kSynRings / 64; ++w) {
+            const int g = rb * (kSynRings / 64) + w;
+            if (64 * g >= nr) break;
+            const int l0 = syn_l0[grpoff[m] + g];
+            chunks += 1.0;
+            if (l0 <= lmax) chunks += (double)ceil_div(lmax + 1 - (m + (l0 - m) / 32 * 32), 32);
+          }
           for (int t0 = 0; t0 < ntile; t0 += tpi) {
-            syn.push_back(LegItem{m, par, rb, t0});
+            syn.push_back(LegItem{m, par, rb, t0, 0});
             syn_cost.push_back(chunks * std::min(tpi, ntile - t0));
           }
+        }
       }
     }
     hipError_t e = upload_items(ana, ana_cost, &h->d_ana_items[sp], &h->n_ana_items[sp]);
