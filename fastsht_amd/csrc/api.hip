@@ -175,7 +175,7 @@ int fsht_info(const fsht_handle* H, long long* out, int n) {
 
 int fsht_ring_tab(const fsht_handle* H, int* out) {
   if (!H || !out) return fail(nullptr, FSHT_ERR_ARG, "NULL argument");
-  std::memcpy(out, H->h.i0.data(), H->h.i0.size() * sizeof(int));
+  std::memcpy(out, H->h.i0_ref.data(), H->h.i0_ref.size() * sizeof(int));
   return FSHT_OK;
 }
 

@@ -100,7 +100,9 @@ struct Handle {
   std::string err;
 
   // host tables
-  std::vector<int> i0;          // ring_tab per m (0..lmax)
+  std::vector<int> i0_ref;      // ring_tab per m (0..lmax): the reference's ring-skip table
+  std::vector<int> i0;          // first ring used per m: ring_tab raised past rings whose P_lm
+                                // is dropped for every l <= lmax (non-decreasing in m)
   std::vector<long long> aboff; // offset of order m in the (a,b) table
   std::vector<long long> seedoff;
   std::vector<long long> synoff;  // offset of order m in the per-(m, ring) synthesis seeds

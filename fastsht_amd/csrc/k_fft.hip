@@ -437,7 +437,7 @@ __device__ void ring_fft_half(double2* buf, int n, const double2* __restrict__ w
 }
 
 // Forward: maps (sim-major, npix per map) -> spec (same layout, PERM per ring).
-// Inverse: spec -> maps.  grid.x = job within the class, grid.y = map index.
+// Inverse: spec -> maps.  grid.x = map index, grid.y = job within the class.
 template <int MAXM>
 __global__ __launch_bounds__(FftShape<MAXM>::TPB) void k_ring_fft(const FftJob* __restrict__ jobs, int job0,
                                                   const RingGeo* __restrict__ geo, const BlueInfo* __restrict__ blue,
@@ -447,8 +447,8 @@ __global__ __launch_bounds__(FftShape<MAXM>::TPB) void k_ring_fft(const FftJob* 
                                                   double* __restrict__ out0, double* __restrict__ out1, int inverse,
                                                   int map0) {
   __shared__ double2 buf[MAXM];
-  const FftJob jb = jobs[job0 + blockIdx.x];
-  const int mapi = map0 + blockIdx.y;
+  const FftJob jb = jobs[job0 + blockIdx.y];
+  const int mapi = map0 + blockIdx.x;
   const int comp = mapi / nsim, k = mapi - comp * nsim;
   const double* in = (comp ? in1 : in0) + (long long)k * npix;
   double* out = (comp ? out1 : out0) + (long long)k * npix;
@@ -574,7 +574,9 @@ hipError_t launch_ring_fft(Handle* h, const double* const* in_comp, double* spec
   for (size_t c = 0; c + 1 < h->fft_class_begin.size(); ++c) {
     const int b = h->fft_class_begin[c], e = h->fft_class_begin[c + 1];
     if (e <= b) continue;
-    dim3 grid(e - b, nmaps);
+    // maps fastest: consecutive workgroups transform the same ring, so its
+    // Bluestein chirp and filter stay in L2 across the maps
+    dim3 grid(nmaps, e - b);
     const int M = h->fft_class_M[c];
 #define FSHT_LAUNCH_FFT(MM)                                                                                      \
   hipLaunchKernelGGL(k_ring_fft<MM>, grid, dim3(FftShape<MM>::TPB), 0, h->stream, h->d_fft_jobs, b, h->d_geo,    \

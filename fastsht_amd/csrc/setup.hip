@@ -592,21 +592,43 @@ void synth_offsets(Handle* h, int ncomp) {
   h->G_size = go;
 }
 
-hipError_t setup_tables(Handle* h) {
-  HostTables t;
-  ring_geometry(h->nside, t.geo, t.phi0);
-  ring_tab(h->nside, h->lmax + 1, t.geo, h->i0);
-  fill_ab_spin(h, t);
-  fill_slots(h, t);
+// Per m: the first ring (absolute index) with a kept P_lm at some l <= lmax,
+// from the synthesis starts; the equator ring when none is (never the case:
+// P_lm is O(1) there).
+__global__ __launch_bounds__(256) void k_first_ring(int nhr, const int* __restrict__ i0,
+                                                    const long long* __restrict__ synoff,
+                                                    const int* __restrict__ syn_st, int* __restrict__ first) {
+  __shared__ int red[256];
+  const int m = blockIdx.x;
+  const int nr = nhr + 1 - i0[m];
+  int r0 = nr - 1;
+  for (int r = threadIdx.x; r < nr; r += 256)
+    if (syn_st[synoff[m] + r] != 0x3fffffff) r0 = min(r0, r);
+  red[threadIdx.x] = r0;
+  __syncthreads();
+  for (int k = 128; k > 0; k >>= 1) {
+    if ((int)threadIdx.x < k) red[threadIdx.x] = min(red[threadIdx.x], red[threadIdx.x + k]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) first[m] = i0[m] + red[0];
+}
+
+// (Re)build the per-(m, ring) seed tables for the current h->i0.
+static hipError_t make_seeds(Handle* h) {
   fill_work(h);
+  hipFree(h->d_i0);
+  hipFree(h->d_seedoff);
+  hipFree(h->d_seed_st);
+  hipFree(h->d_seed_v);
+  hipFree(h->d_synoff);
+  hipFree(h->d_syn_st);
+  hipFree(h->d_syn_v);
+  h->d_i0 = nullptr;
+  h->d_seedoff = h->d_synoff = nullptr;
+  h->d_seed_st = h->d_syn_st = nullptr;
+  h->d_seed_v = h->d_syn_v = nullptr;
   hipError_t e;
-  if ((e = upload(&h->d_geo, t.geo)) != hipSuccess) return e;
   if ((e = upload(&h->d_i0, h->i0)) != hipSuccess) return e;
-  if ((e = upload(&h->d_ab, t.ab)) != hipSuccess) return e;
-  if ((e = upload(&h->d_spin, t.spin)) != hipSuccess) return e;
-  if ((e = upload(&h->d_aboff, h->aboff)) != hipSuccess) return e;
-  if ((e = upload(&h->d_tw, t.tw)) != hipSuccess) return e;
-  if ((e = upload(&h->d_slot, t.slot)) != hipSuccess) return e;
   if ((e = upload(&h->d_seedoff, h->seedoff)) != hipSuccess) return e;
   const long long nseed = h->seedoff[h->lmax + 1];
   if ((e = hipMalloc(&h->d_seed_st, std::max(1LL, nseed) * sizeof(int))) != hipSuccess) return e;
@@ -615,7 +637,51 @@ hipError_t setup_tables(Handle* h) {
   const long long nsyn = h->synoff[h->lmax + 1];
   if ((e = hipMalloc(&h->d_syn_st, std::max(1LL, nsyn) * sizeof(int))) != hipSuccess) return e;
   if ((e = hipMalloc(&h->d_syn_v, std::max(1LL, nsyn) * sizeof(double2))) != hipSuccess) return e;
-  if ((e = launch_seeds(h)) != hipSuccess) return e;
+  return launch_seeds(h);
+}
+
+hipError_t setup_tables(Handle* h) {
+  HostTables t;
+  ring_geometry(h->nside, t.geo, t.phi0);
+  ring_tab(h->nside, h->lmax + 1, t.geo, h->i0_ref);
+  h->i0 = h->i0_ref;
+  fill_ab_spin(h, t);
+  fill_slots(h, t);
+  hipError_t e;
+  if ((e = upload(&h->d_geo, t.geo)) != hipSuccess) return e;
+  if ((e = upload(&h->d_ab, t.ab)) != hipSuccess) return e;
+  if ((e = upload(&h->d_spin, t.spin)) != hipSuccess) return e;
+  if ((e = upload(&h->d_aboff, h->aboff)) != hipSuccess) return e;
+  if ((e = upload(&h->d_tw, t.tw)) != hipSuccess) return e;
+  if ((e = upload(&h->d_slot, t.slot)) != hipSuccess) return e;
+  if ((e = make_seeds(h)) != hipSuccess) return e;
+  // Rings past ring_tab whose P_lm is dropped at every l <= lmax carry no
+  // work for m: raise the first ring to the suffix minimum over m' >= m of
+  // the first ring with a kept value (non-decreasing in m, never past a ring
+  // with a kept value), and rebuild the seeds for the smaller ring sets.
+  {
+    const int nm = h->lmax + 1;
+    std::vector<int> first(nm);
+    int* d_first = nullptr;
+    if ((e = hipMalloc(&d_first, nm * sizeof(int))) != hipSuccess) return e;
+    hipLaunchKernelGGL(k_first_ring, dim3(nm), dim3(256), 0, h->stream, h->nhr, h->d_i0, h->d_synoff, h->d_syn_st,
+                       d_first);
+    e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    if (e == hipSuccess) e = hipMemcpy(first.data(), d_first, nm * sizeof(int), hipMemcpyDeviceToHost);
+    (void)hipFree(d_first);
+    if (e != hipSuccess) return e;
+    std::vector<int> eff(nm);
+    int run = h->nhr;
+    for (int m = nm - 1; m >= 0; --m) {
+      run = std::min(run, first[m]);
+      eff[m] = std::max(run, h->i0_ref[m]);
+    }
+    if (eff != h->i0) {
+      h->i0 = eff;
+      if ((e = make_seeds(h)) != hipSuccess) return e;
+    }
+  }
   synth_offsets(h, 2);  // sized for the largest (spin-2) layout; spin-0 uses a prefix
   if ((e = upload(&h->d_goff, h->goff)) != hipSuccess) return e;
   if ((e = build_items(h)) != hipSuccess) return e;
