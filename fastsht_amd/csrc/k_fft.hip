@@ -66,6 +66,10 @@ static void host_fft(std::vector<double2>& a, int sign) {
 // transformed one real ring at a time as a complex length-n/2 Bluestein
 // transform of the even/odd pixel pairs.
 static constexpr int kMaxLds = 8192;
+#ifndef FSHT_FFT_HALF_ALL
+#define FSHT_FFT_HALF_ALL 1
+#endif
+static constexpr bool kHalfAll = FSHT_FFT_HALF_ALL != 0;
 
 hipError_t build_bluestein(Handle* h) {
   const int nside = h->nside;
@@ -78,7 +82,7 @@ hipError_t build_bluestein(Handle* h) {
     const int n = (i < nside) ? 4 * (i + 1) : 4 * nside;
     if (is_pow2(n)) {
       jobs.push_back({i, n, n, 0, 0});
-    } else if (next_pow2(2 * n - 1) <= kMaxLds) {
+    } else if (!kHalfAll && next_pow2(2 * n - 1) <= kMaxLds) {
       jobs.push_back({i, n, next_pow2(2 * n - 1), n, 0});
     } else {
       const int L = n / 2;  // polar rings are never the equator, so both halves exist
@@ -459,7 +463,7 @@ __global__ __launch_bounds__(FftShape<MAXM>::TPB) void k_ring_fft(const FftJob* 
   const bool isblue = jb.blue >= 0;
   const double2* w = isblue ? chirp + blue[jb.blue].chirp_off : nullptr;
   const double2* fl = isblue ? filt + blue[jb.blue].filt_off : nullptr;
-  if constexpr (M == kMaxLds) {
+  if constexpr (M >= 8) {
     if (jb.half) {
       const long long i1 = (jb.half == 1) ? g.i1N : g.i1S;
       ring_fft_half<M>(buf, n, w, fl, tw, Mmax, in + i1, out + i1, inverse);
