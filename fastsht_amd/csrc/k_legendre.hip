@@ -255,6 +255,15 @@ __global__ __launch_bounds__(256, SPIN ? FSHT_ANA_MINBLK2 : FSHT_ANA_MINBLK0) vo
                 const double a = Pt[4 * ks + q][li];
                 D[t] = mfma64(a, bfr[ks], D[t]);
               }
+              // software pipeline of the A reads: the LDS pair read for MFMAs
+              // k+4, k+5 is issued before MFMAs k, k+1 (two pairs in flight)
+              __builtin_amdgcn_sched_group_barrier(0x100, 2, 0);  // 2 LDS reads
+#pragma unroll
+              for (int j = 0; j < 6; ++j) {
+                __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+                __builtin_amdgcn_sched_group_barrier(0x008, 2, 0);  // 2 MFMAs
+              }
+              __builtin_amdgcn_sched_group_barrier(0x008, 4, 0);
             }
           }
         }
