@@ -326,8 +326,11 @@ __device__ inline void wave_lds_sync() {
 #ifndef FSHT_SYN_SEG2
 #define FSHT_SYN_SEG2 64
 #endif
+// spin 2 asks for no minimum: the compiler then keeps the accumulators in
+// AGPRs and still fits two workgroups per CU (faster than the all-VGPR
+// allocation a minimum of 2 produces)
 #ifndef FSHT_SYN_MINBLK2
-#define FSHT_SYN_MINBLK2 2
+#define FSHT_SYN_MINBLK2 1
 #endif
 template <int SPIN>
 struct SynShared {
