@@ -10,8 +10,10 @@ from oracle import oracle as O
 pytestmark = pytest.mark.gpu
 TOL = 1e-12
 
+# (128, 40) and (64, 30): lmax far below 2*nside, so most polar rings carry
+# no kept P_lm for the larger orders and drop out of the per-order ring sets
 CASES = [(1, 2, 3), (2, 5, 2), (4, 8, 3), (4, 16, 5), (8, 20, 17), (8, 31, 4), (16, 47, 33),
-         (16, 64, 3), (32, 95, 20), (32, 40, 16)]
+         (16, 64, 3), (32, 95, 20), (32, 40, 16), (64, 30, 2), (128, 40, 3)]
 
 
 @pytest.fixture(scope="module")
@@ -63,6 +65,21 @@ def test_spin2_vs_oracle(torch_cuda, ns, lmax, nsim):
     E2, B2 = S2.qu2eb(Q, U)
     Eo2, Bo2 = R.qu2eb(Q, U, niter=2)
     assert maxrel(E2, Eo2) < TOL and maxrel(B2, Bo2) < TOL
+
+
+def test_spin0_lmax0_and_nside256(torch_cuda):
+    """lmax = 0 (monopole only) and nside 256 / lmax 511, niter 0 and 3."""
+    from fastsht_amd import SHT
+    for ns, lmax, nsim in [(4, 0, 2), (256, 511, 2)]:
+        print(f"case spin0 nside={ns} lmax={lmax} nsim={nsim}", flush=True)
+        rng = np.random.default_rng(ns + lmax)
+        maps = _maps(rng, ns, nsim)
+        R = O.OracleSHT(ns, lmax, nsim)
+        S = SHT(ns, lmax, nsim, niter=3)
+        assert np.array_equal(S.ring_tab(), R.rtab.astype(np.int32))
+        assert maxrel(S.t2alm(maps), R.t2alm(maps, niter=3)) < TOL
+        alm = O.random_alm(lmax, nsim, seed=4)
+        assert maxrel(S.alm2t(alm), R.alm2t(alm)) < TOL
 
 
 def test_c0_config_vs_oracle(torch_cuda):
@@ -271,6 +288,7 @@ def test_nside2048_orders_closed_form(torch_cuda):
         T += (1.0 if m == 0 else 2.0) * np.real(np.repeat(g_ring, nfft) * eph(m))
         per_m[m] = (a, P, sgn)
     sht = SHT(nside, lmax, nsim)
+    assert np.array_equal(sht.ring_tab(), O.ring_tab(nside, nm=L, lcap=6143))
     maps = sht.alm2t(alm)
     assert maxrel(maps[:, 0], T) < TOL
     out = sht.t2alm(np.asfortranarray(T[:, None]))
