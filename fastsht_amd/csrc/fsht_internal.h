@@ -124,9 +124,9 @@ struct Handle {
   int* d_syn_st = nullptr;      // first l whose P_lm is kept, per (m, ring): the synthesis start
   double2* d_syn_v = nullptr;   // (P_st, P_st+1) at that start
   long long* d_goff = nullptr;
-  LegItem* d_ana_items[2] = {nullptr, nullptr};  // per spin (0, 2)
+  LegItem* d_ana_items[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};  // [spin 0/2][full / 2-tile column blocks]
   LegItem* d_syn_items[2] = {nullptr, nullptr};
-  int n_ana_items[2] = {0, 0}, n_syn_items[2] = {0, 0};
+  int n_ana_items[2][2] = {{0, 0}, {0, 0}}, n_syn_items[2] = {0, 0};
 
   // FFT tables
   std::vector<FftJob> fft_jobs;          // host copy, grouped by class

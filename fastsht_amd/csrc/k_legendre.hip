@@ -132,7 +132,12 @@ __device__ inline void gen32(Rec& r, const double2* cf, const double2* scf, int 
 #ifndef FSHT_ANA_MINBLK2
 #define FSHT_ANA_MINBLK2 2
 #endif
-template <int SPIN>
+// KS = 1: the 4 waves own 4 column tiles and every ring of a sub-block.
+// KS = 2 (column blocks of 2 tiles: the last block of a spin-0 batch whose
+// 16-column tile count is not a multiple of 4): waves w and w+2 share column
+// tile w & 1, each over one half of the sub-block's K-steps, and the two
+// partial sums are added through LDS at the end.
+template <int SPIN, int KS>
 __global__ __launch_bounds__(256, SPIN ? FSHT_ANA_MINBLK2 : FSHT_ANA_MINBLK0) void k_analysis(
     const LegItem* __restrict__ items, const RingGeo* __restrict__ geo, const int* __restrict__ i0tab,
     const double2* __restrict__ ab, const double2* __restrict__ spc, const long long* __restrict__ aboff,
@@ -156,8 +161,10 @@ __global__ __launch_bounds__(256, SPIN ? FSHT_ANA_MINBLK2 : FSHT_ANA_MINBLK0) vo
   const int L0 = m + it.blk * kAnaLB;
   const int L1 = min(L0 + kAnaLB, lmax + 1);
   const int nt = (L1 - L0 - par + 31) / 32;  // l-tiles in this block
-  // this wave's output column tile
-  const int c0 = (it.colblk * 4 + wave) * 16;
+  // this wave's output column tile and K half
+  constexpr int NKB = 64 / KS;                 // K-steps (of 4 rings) per wave and sub-block
+  const int kh = wave / (4 / KS);              // 0 for KS = 1
+  const int c0 = (it.colblk * 4 + wave % (4 / KS)) * 16;
   const bool colok = c0 < ncols;
   const int otype = colok ? c0 / nsimp : 0;
   const int sim0 = colok ? c0 - otype * nsimp : 0;
@@ -209,9 +216,10 @@ __global__ __launch_bounds__(256, SPIN ? FSHT_ANA_MINBLK2 : FSHT_ANA_MINBLK0) vo
     }
     // ---- B fragments: the wave's 16 columns of the folded, rotated ring
     // spectra for the sub-block's rings (one coalesced row read per ring)
-    double bfr[64];
+    double bfr[NKB];
 #pragma unroll
-    for (int ks = 0; ks < 64; ++ks) {
+    for (int kj = 0; kj < NKB; ++kj) {
+      const int ks = kh * NKB + kj;
       const int krow = 4 * ks + q;
       const int f = SPIN ? (ks >= 32 ? 1 : 0) : 0;
       const int r = SPIN ? (krow % RS) : krow;
@@ -222,7 +230,7 @@ __global__ __launch_bounds__(256, SPIN ? FSHT_ANA_MINBLK2 : FSHT_ANA_MINBLK0) vo
       const int fold = par ^ s.flip;
       const int col = (s.comp * 2 + s.reim) * nsimp + sim0 + li;
       const double v = F[fb + ((long long)fold * nr + (ic - i0)) * ncols + col];
-      bfr[ks] = (active && valid) ? s.sign * v : 0.0;
+      bfr[kj] = (active && valid) ? s.sign * v : 0.0;
     }
 
 #pragma unroll
@@ -248,12 +256,13 @@ __global__ __launch_bounds__(256, SPIN ? FSHT_ANA_MINBLK2 : FSHT_ANA_MINBLK0) vo
         __syncthreads();
         if (active) {
 #pragma unroll
-          for (int g = 0; g < 4; ++g) {
+          for (int gj = 0; gj < 4 / KS; ++gj) {
+            const int g = kh * (4 / KS) + gj;  // group of 16 K-steps = 64 rings
             if (64 * (g % (RS / 64)) < kvalid) {
 #pragma unroll
-              for (int ks = 16 * g; ks < 16 * g + 16; ++ks) {
-                const double a = Pt[4 * ks + q][li];
-                D[t] = mfma64(a, bfr[ks], D[t]);
+              for (int kj = 16 * gj; kj < 16 * gj + 16; ++kj) {
+                const double a = Pt[4 * (16 * g + kj - 16 * gj) + q][li];
+                D[t] = mfma64(a, bfr[kj], D[t]);
               }
               // software pipeline of the A reads: the LDS pair read for MFMAs
               // k+4, k+5 is issued before MFMAs k, k+1 (two pairs in flight)
@@ -272,6 +281,23 @@ __global__ __launch_bounds__(256, SPIN ? FSHT_ANA_MINBLK2 : FSHT_ANA_MINBLK0) vo
     }
   }
 
+  if (KS == 2) {
+    // waves 2, 3 hand their partial sums to waves 0, 1 (Pt is free: the last
+    // chunk ended with a barrier)
+    double* red = &Pt[0][0];
+    if (kh == 1) {
+#pragma unroll
+      for (int t = 0; t < 8; ++t)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) red[(((wave & 1) * 8 + t) * 4 + v) * 64 + lane] = D[t][v];
+    }
+    __syncthreads();
+    if (kh == 1) return;
+#pragma unroll
+    for (int t = 0; t < 8; ++t)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) D[t][v] += red[(((wave & 1) * 8 + t) * 4 + v) * 64 + lane];
+  }
   if (!active) return;
   double* alm = ocomp ? alm1 : alm0;
   const int sim = sim0 + li;
@@ -582,16 +608,25 @@ __global__ __launch_bounds__(256, SPIN ? FSHT_SYN_MINBLK2 : FSHT_SYN_MINBLK0) vo
 hipError_t launch_analysis(Handle* h, const double* F, double* alm0, double* alm1, double beta, int want0,
                            int want1) {
   const int sp = h->spin ? 1 : 0;
-  dim3 grid(h->n_ana_items[sp]);
-  if (sp)
-    hipLaunchKernelGGL(k_analysis<2>, grid, dim3(256), 0, h->stream, h->d_ana_items[1], h->d_geo, h->d_i0, h->d_ab,
-                       h->d_spin, h->d_aboff, h->d_seedoff, h->d_seed_st, h->d_seed_v, F, h->d_goff, alm0, alm1,
-                       h->nsim, h->nsimp, h->lmax, h->nhr, h->pa, beta, want0, want1);
-  else
-    hipLaunchKernelGGL(k_analysis<0>, grid, dim3(256), 0, h->stream, h->d_ana_items[0], h->d_geo, h->d_i0, h->d_ab,
-                       h->d_spin, h->d_aboff, h->d_seedoff, h->d_seed_st, h->d_seed_v, F, h->d_goff, alm0, alm1,
-                       h->nsim, h->nsimp, h->lmax, h->nhr, h->pa, beta, 1, 0);
-  return hipGetLastError();
+  if (!sp) { want0 = 1; want1 = 0; }
+  for (int ks = 0; ks < 2; ++ks) {  // full column blocks, then 2-tile blocks
+    const int n = h->n_ana_items[sp][ks];
+    if (n == 0) continue;
+    const LegItem* items = h->d_ana_items[sp][ks];
+#define FSHT_LAUNCH_ANA(SPV, KSV)                                                                               \
+  hipLaunchKernelGGL((k_analysis<SPV, KSV>), dim3(n), dim3(256), 0, h->stream, items, h->d_geo, h->d_i0, h->d_ab, \
+                     h->d_spin, h->d_aboff, h->d_seedoff, h->d_seed_st, h->d_seed_v, F, h->d_goff, alm0, alm1,  \
+                     h->nsim, h->nsimp, h->lmax, h->nhr, h->pa, beta, want0, want1)
+    if (sp) {
+      if (ks) FSHT_LAUNCH_ANA(2, 2); else FSHT_LAUNCH_ANA(2, 1);
+    } else {
+      if (ks) FSHT_LAUNCH_ANA(0, 2); else FSHT_LAUNCH_ANA(0, 1);
+    }
+#undef FSHT_LAUNCH_ANA
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
 }
 
 hipError_t launch_synthesis(Handle* h, const double* alm0, const double* alm1, double* G) {

@@ -527,8 +527,8 @@ static hipError_t build_items(Handle* h) {
   for (int sp = 0; sp < 2; ++sp) {
     const int ncomp = sp ? 2 : 1;
     const int ncols = 2 * ncomp * h->nsimp;
-    std::vector<LegItem> ana, syn;
-    std::vector<double> ana_cost, syn_cost;  // estimated run time of each item
+    std::vector<LegItem> ana[2], syn;          // analysis: full column blocks, 2-tile blocks
+    std::vector<double> ana_cost[2], syn_cost;  // estimated run time of each item
     for (int m = 0; m <= lmax; ++m) {
       const int nr = h->nhr + 1 - h->i0[m];
       for (int par = 0; par < 2; ++par) {
@@ -540,8 +540,9 @@ static hipError_t build_items(Handle* h) {
           if (r0 < nr) r0 &= ~3;  // whole MFMA K-steps of 4 rings
           const double rings = 64.0 * ceil_div(nr - r0, 64);
           for (int cb = 0; cb < ncb; ++cb) {
-            ana.push_back(LegItem{m, par, b, cb, r0});
-            ana_cost.push_back(rings * (nl + 32) + 64.0 * 32);
+            const int k2 = (ncols / 16 - 4 * cb) <= 2 ? 1 : 0;  // 2-tile block: K split over wave pairs
+            ana[k2].push_back(LegItem{m, par, b, cb, r0});
+            ana_cost[k2].push_back(rings * (nl + 32) + 64.0 * 32);
           }
         }
         // synthesis: one ring block against the kept l of m (first 16-column
@@ -564,7 +565,9 @@ static hipError_t build_items(Handle* h) {
         }
       }
     }
-    hipError_t e = upload_items(ana, ana_cost, &h->d_ana_items[sp], &h->n_ana_items[sp]);
+    hipError_t e = hipSuccess;
+    for (int k2 = 0; k2 < 2 && e == hipSuccess; ++k2)
+      e = upload_items(ana[k2], ana_cost[k2], &h->d_ana_items[sp][k2], &h->n_ana_items[sp][k2]);
     if (e != hipSuccess) return e;
     e = upload_items(syn, syn_cost, &h->d_syn_items[sp], &h->n_syn_items[sp]);
     if (e != hipSuccess) return e;
