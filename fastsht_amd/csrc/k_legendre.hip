@@ -164,7 +164,7 @@ __global__ __launch_bounds__(256, SPIN ? FSHT_ANA_MINBLK2 : FSHT_ANA_MINBLK0) vo
   // this wave's output column tile and K half
   constexpr int NKB = 64 / KS;                 // K-steps (of 4 rings) per wave and sub-block
   const int kh = wave / (4 / KS);              // 0 for KS = 1
-  const int c0 = (it.colblk * 4 + wave % (4 / KS)) * 16;
+  const int c0 = (it.colblk * (4 / KS) + wave % (4 / KS)) * 16;  // colblk counts blocks of 4/KS tiles
   const bool colok = c0 < ncols;
   const int otype = colok ? c0 / nsimp : 0;
   const int sim0 = colok ? c0 - otype * nsimp : 0;

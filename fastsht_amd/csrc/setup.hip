@@ -540,9 +540,16 @@ static hipError_t build_items(Handle* h) {
           if (r0 < nr) r0 &= ~3;  // whole MFMA K-steps of 4 rings
           const double rings = 64.0 * ceil_div(nr - r0, 64);
           for (int cb = 0; cb < ncb; ++cb) {
-            const int k2 = (ncols / 16 - 4 * cb) <= 2 ? 1 : 0;  // 2-tile block: K split over wave pairs
-            ana[k2].push_back(LegItem{m, par, b, cb, r0});
-            ana_cost[k2].push_back(rings * (nl + 32) + 64.0 * 32);
+            const int ntl = std::min(4, ncols / 16 - 4 * cb);  // 16-column tiles in the block
+            if (ntl > 2) {
+              ana[0].push_back(LegItem{m, par, b, cb, r0});
+              ana_cost[0].push_back(rings * (nl + 32) + 64.0 * 32);
+            } else {  // blocks of 2 tiles: K split over wave pairs (colblk counts 2-tile blocks)
+              for (int c2 = 0; 2 * c2 < ntl; ++c2) {
+                ana[1].push_back(LegItem{m, par, b, 2 * cb + c2, r0});
+                ana_cost[1].push_back(rings * (nl + 32) + 64.0 * 32);
+              }
+            }
           }
         }
         // synthesis: one ring block against the kept l of m (first 16-column
