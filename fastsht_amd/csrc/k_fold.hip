@@ -17,6 +17,22 @@
 
 namespace fsht {
 
+// Thread blocks of the gather / fold kernels: x = 16 spectrum columns (one
+// 128-byte line of a pixel-major row; nsimp is a multiple of 16, so column
+// blocks are never partly empty), y = rings (gathers) or slots (fold).
+#ifndef FSHT_FOLD_BX
+#define FSHT_FOLD_BX 16
+#endif
+struct FoldBlk {
+  int x, y;
+};
+constexpr FoldBlk kFoldBlk{FSHT_FOLD_BX, 256 / FSHT_FOLD_BX};
+#ifndef FSHT_GATHER_RINGS
+#define FSHT_GATHER_RINGS 32
+#endif
+constexpr int kGatherRings = FSHT_GATHER_RINGS;  // rings per gather workgroup
+
+
 template <int NCOMP>
 __global__ __launch_bounds__(256) void k_fold(const RingGeo* __restrict__ geo, const int* __restrict__ i0tab,
                                               const double2* __restrict__ tw, const double* __restrict__ G,
@@ -24,8 +40,8 @@ __global__ __launch_bounds__(256) void k_fold(const RingGeo* __restrict__ geo, c
                                               double* __restrict__ out, int residual, int lmax, int nhr,
                                               int nnorth, int nsimp) {
   const int i = blockIdx.x;  // north ring
-  const int kk = blockIdx.y * 4 + threadIdx.y;
-  const int col = blockIdx.z * 64 + threadIdx.x;  // spectrum column: comp*nsimp + sim
+  const int kk = blockIdx.y * kFoldBlk.y + threadIdx.y;
+  const int col = blockIdx.z * kFoldBlk.x + threadIdx.x;  // spectrum column: comp*nsimp + sim
   const int ldT = NCOMP * nsimp;
   if (col >= ldT) return;
   const RingGeo g = geo[i];
@@ -128,7 +144,7 @@ __global__ __launch_bounds__(256) void k_gather(const RingGeo* __restrict__ geo,
   const int m = blockIdx.x;
   const int i0 = i0tab[m];
   const int ldT = NCOMP * nsimp;
-  const int col = blockIdx.z * 64 + threadIdx.x;  // spectrum column comp*nsimp + sim
+  const int col = blockIdx.z * kFoldBlk.x + threadIdx.x;  // spectrum column comp*nsimp + sim
   if (col >= ldT) return;
   const int comp = col / nsimp, sim = col - comp * nsimp;
   const int nr = nhr + 1 - i0;
@@ -137,7 +153,8 @@ __global__ __launch_bounds__(256) void k_gather(const RingGeo* __restrict__ geo,
   const long long odd = (long long)nr * ncols;
   const int cre = (comp * 2) * nsimp + sim, cim = cre + nsimp;
   const long long mrow = (long long)m * nnorth;
-  for (int i = i0 + blockIdx.y * 16 + threadIdx.y; i <= nhr && i < i0 + (blockIdx.y + 1) * 16; i += 4) {
+  for (int i = i0 + blockIdx.y * kGatherRings + threadIdx.y; i <= nhr && i < i0 + (blockIdx.y + 1) * kGatherRings;
+       i += kFoldBlk.y) {
     const int sinfo = slot[mrow + i];
     const int rel = sinfo >> 2, cls = sinfo & 3;
     const double sg = cls == 1 ? 1.0 : (cls == 2 ? -1.0 : 0.0);
@@ -183,7 +200,7 @@ __global__ __launch_bounds__(256) void k_gather_prefix(const RingGeo* __restrict
   const int m = blockIdx.x;
   const int i0 = i0tab[m];
   const int ldT = NCOMP * nsimp;
-  const int col = blockIdx.z * 64 + threadIdx.x;
+  const int col = blockIdx.z * kFoldBlk.x + threadIdx.x;
   if (col >= ldT) return;
   const int comp = col / nsimp, sim = col - comp * nsimp;
   const int nr = nhr + 1 - i0;
@@ -192,7 +209,8 @@ __global__ __launch_bounds__(256) void k_gather_prefix(const RingGeo* __restrict
   const long long odd = (long long)nr * ncols;
   const int cre = (comp * 2) * nsimp + sim, cim = cre + nsimp;
   const long long mrow = (long long)m * nnorth;
-  for (int i = i0 + blockIdx.y * 16 + threadIdx.y; i <= nhr && i < i0 + (blockIdx.y + 1) * 16; i += 4) {
+  for (int i = i0 + blockIdx.y * kGatherRings + threadIdx.y; i <= nhr && i < i0 + (blockIdx.y + 1) * kGatherRings;
+       i += kFoldBlk.y) {
     const RingGeo g = geo[i];
     const int n = g.nfft;
     const int pos = m % n;
@@ -276,8 +294,9 @@ __global__ __launch_bounds__(256) void k_gather_prefix(const RingGeo* __restrict
 
 hipError_t launch_gather_prefix(Handle* h, const double* G, const double* specT, double* F, int full) {
   const int ncomp = h->spin ? 2 : 1;
-  dim3 block(64, 4);
-  dim3 grid(h->lmax + 1, (h->nnorth + 15) / 16, (ncomp * h->nsimp + 63) / 64);
+  dim3 block(kFoldBlk.x, kFoldBlk.y);
+  dim3 grid(h->lmax + 1, (h->nnorth + kGatherRings - 1) / kGatherRings,
+            (ncomp * h->nsimp + kFoldBlk.x - 1) / kFoldBlk.x);
   if (ncomp == 2)
     hipLaunchKernelGGL(k_gather_prefix<2>, grid, block, 0, h->stream, h->d_geo, h->d_i0, h->d_tw, h->d_slot, G,
                        specT, F, h->d_goff, h->nhr, h->nnorth, h->nsimp, h->lmax, full);
@@ -289,8 +308,9 @@ hipError_t launch_gather_prefix(Handle* h, const double* G, const double* specT,
 
 hipError_t launch_gather(Handle* h, const double* specT, double* F) {
   const int ncomp = h->spin ? 2 : 1;
-  dim3 block(64, 4);
-  dim3 grid(h->lmax + 1, (h->nnorth + 15) / 16, (ncomp * h->nsimp + 63) / 64);
+  dim3 block(kFoldBlk.x, kFoldBlk.y);
+  dim3 grid(h->lmax + 1, (h->nnorth + kGatherRings - 1) / kGatherRings,
+            (ncomp * h->nsimp + kFoldBlk.x - 1) / kFoldBlk.x);
   if (ncomp == 2)
     hipLaunchKernelGGL(k_gather<2>, grid, block, 0, h->stream, h->d_geo, h->d_i0, h->d_tw, h->d_slot, specT, F,
                        h->d_goff, h->nhr, h->nnorth, h->nsimp);
@@ -303,8 +323,8 @@ hipError_t launch_gather(Handle* h, const double* specT, double* F) {
 hipError_t launch_fold(Handle* h, const double* G, const double* specT_t, double* specT_out, int residual) {
   const int ncomp = h->spin ? 2 : 1;
   const int maxk = 2 * h->nside;  // n/2 for the longest ring
-  dim3 block(64, 4);
-  dim3 grid(h->nnorth, (maxk + 1 + 3) / 4, (ncomp * h->nsimp + 63) / 64);
+  dim3 block(kFoldBlk.x, kFoldBlk.y);
+  dim3 grid(h->nnorth, (maxk + 1 + kFoldBlk.y - 1) / kFoldBlk.y, (ncomp * h->nsimp + kFoldBlk.x - 1) / kFoldBlk.x);
   if (ncomp == 2)
     hipLaunchKernelGGL(k_fold<2>, grid, block, 0, h->stream, h->d_geo, h->d_i0, h->d_tw, G, h->d_goff, specT_t,
                        specT_out, residual, h->lmax, h->nhr, h->nnorth, h->nsimp);
