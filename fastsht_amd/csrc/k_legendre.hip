@@ -473,7 +473,10 @@ __device__ inline void synth_wave(SynShared<SPIN>& sh, const LegItem it, const R
   stmin = __builtin_amdgcn_readfirstlane(stmin);  // wave-uniform: keeps the loop scalar
   const int lc0 = (stmin > lmax) ? lmax + 1 : m + ((stmin - m) / 32) * 32;
 
-  double bcur[NKS][NT], bnx[NKS][NT];
+  // alm operand of the current chunk; the next chunk's is loaded into the
+  // same registers right after this chunk's MFMAs are issued, so the loads
+  // overlap their execution and the next recurrence (no second buffer, no copy)
+  double bcur[NKS][NT];
   if ((lc0 - m) % SEG != 0 && lc0 <= lmax) {  // start inside a segment
     load_seg(lc0 - (lc0 - m) % SEG);
     wave_lds_sync();
@@ -487,7 +490,6 @@ __device__ inline void synth_wave(SynShared<SPIN>& sh, const LegItem it, const R
       wave_lds_sync();
     }
     const bool more = lc + 32 <= lmax;
-    if (more) load_b(lc + 32, bnx);
 
     // ---- 32 recurrence steps; step s has l - m parity s & 1 (lc - m is even)
     const double2* c = cf + jseg;
@@ -537,13 +539,8 @@ __device__ inline void synth_wave(SynShared<SPIN>& sh, const LegItem it, const R
 #pragma unroll
         for (int ks = 0; ks < NKS; ++ks) D[a][u] = mfma64(av[ks], bcur[ks][u], D[a][u]);
     }
+    if (more) load_b(lc + 32, bcur);
     wave_lds_sync();
-    if (more) {
-#pragma unroll
-      for (int ks = 0; ks < NKS; ++ks)
-#pragma unroll
-        for (int u = 0; u < NT; ++u) bcur[ks][u] = bnx[ks][u];
-    }
   }
 
 #pragma unroll
