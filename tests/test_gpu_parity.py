@@ -303,6 +303,40 @@ def test_nside2048_orders_closed_form(torch_cuda):
         assert maxrel(got, ad) < TOL, m
 
 
+@pytest.mark.parametrize("ns,lmax", [(1024, 64), (2048, 96)])
+def test_high_nside_spin2_vs_oracle(torch_cuda, ns, lmax):
+    """Spin 2 at nside 1024 and 2048 with a small lmax: per-order ring sets far
+    shorter than ring_tab's, and (2048) the long polar rings' half-length
+    transforms on Q/U.
+
+    The spin-2 functions F1/F2 (src/get_plm_recursive.f90:115-124) subtract
+    terms of size ~ l/sin^2(theta) * P to leave ~ l^2 * P, so near the poles
+    they carry a relative rounding error of ~ 2/(l sin^2 theta) * 1e-16 in any
+    FP64 evaluation: ~1e-11 on the first rings at nside 2048.  Two correct
+    implementations differ by that much there, so maps are held to 1e-12
+    away from the 64 most polar rings of each cap, and to 5e-11 overall."""
+    from fastsht_amd import SHT
+    nsim = 1
+    print(f"case spin2 nside={ns} lmax={lmax} nsim={nsim}", flush=True)
+    R = O.OracleSHT(ns, lmax, nsim)
+    Ea, Ba = O.random_alm(lmax, nsim, seed=5), O.random_alm(lmax, nsim, seed=6)
+    S = SHT(ns, lmax, nsim, pol=True)
+    Qg, Ug = S.eb2qu(Ea, Ba)
+    Qo, Uo = R.eb2qu(Ea, Ba)
+    K = 64
+    cap = 2 * K * (K + 1)  # pixels of the first K rings
+    npix = 12 * ns * ns
+    body = slice(cap, npix - cap)
+    for g, o in ((Qg, Qo), (Ug, Uo)):
+        err = np.abs(g - o)
+        scale = np.abs(o).max()
+        assert err[body].max() / scale < TOL, err[body].max() / scale
+        assert err.max() / scale < 5e-11, err.max() / scale
+    E, B = S.qu2eb(Qo, Uo)
+    Eo, Bo = R.qu2eb(Qo, Uo)
+    assert maxrel(E, Eo) < TOL and maxrel(B, Bo) < TOL
+
+
 @pytest.mark.parametrize("ns,lmax,nsim,niter", [(8, 16, 3, 0), (16, 32, 2, 2), (32, 80, 3, 1)])
 def test_fix_eb_vs_oracle(torch_cuda, ns, lmax, nsim, niter):
     """EB-leakage cleaning through the C ABI (host arrays) against the oracle:
