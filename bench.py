@@ -8,6 +8,7 @@ roofline fraction (HIP events on the library stream) and a bounded CPU
 baseline from the CPU restatement of the algorithm.
 
   python bench.py [--steps K] [--warmup W] [--nside 1024 --lmax 2047 --nsim 200 --niter 3]
+  python bench.py --pol [--steps K]      # spin-2 qu2eb+eb2qu (separate line, not the value)
 """
 from __future__ import annotations
 
@@ -114,6 +115,90 @@ def _cpu_model():
     return "unknown"
 
 
+def make_eb(sht, seed):
+    """Seeded E/B Gaussian alm (C_l^EE = 1/(l(l+1)), C_l^BB = 0.05 C_l^EE, l >= 2)
+    in the packed storage, on the GPU."""
+    import torch
+
+    L = sht.lmax + 1
+    g = torch.Generator(device="cuda")
+    g.manual_seed(seed)
+    l = torch.arange(L, dtype=torch.float64, device="cuda")
+    cl = torch.zeros(L, dtype=torch.float64, device="cuda")
+    cl[2:] = 1.0 / (l[2:] * (l[2:] + 1.0))
+    c = torch.arange(L, device="cuda")[:, None]
+    r = torch.arange(L, device="cuda")[None, :]
+    re_part = r >= c
+    ell = torch.where(re_part, r, r + L - c).clamp(max=L - 1)
+    half = torch.where(re_part & (c == 0), torch.ones_like(cl[0]), torch.full_like(cl[0], 0.5))
+    out = []
+    for amp in (1.0, 0.05):
+        sd = torch.sqrt(cl[ell] * half * amp)
+        out.append((torch.randn((L, L, sht.nsim), dtype=torch.float64, device="cuda", generator=g)
+                    * sd[:, :, None]).permute(2, 1, 0))
+    return out
+
+
+def pol_round_trip(args, lmax):
+    """Spin-2 workload (nside 1024, lmax 2047, nsim Q/U pairs): one step =
+    qu2eb(niter) + eb2qu on Q/U built as eb2qu of seeded E/B alm.  Prints one
+    JSON line (Q/U pairs per second, per-niter E/B error against the input alm);
+    the T round trip stays the benchmark value."""
+    import torch
+
+    from fastsht_amd import SHT, _lib
+
+    sht = SHT(args.nside, lmax, args.nsim, niter=args.niter, pol=True)
+    lib = _lib.load()
+    E, B = make_eb(sht, args.seed)
+    Q, U = sht.eb2qu(E, B)
+    Eo = torch.empty_like(E.permute(2, 1, 0).contiguous()).permute(2, 1, 0)
+    Bo = torch.empty_like(Eo)
+    p = lambda t: ctypes.c_void_p(t.data_ptr())
+    scale = max(float(E.abs().max()), float(B.abs().max()))
+    errs = {}
+    for it in range(args.niter + 1):  # per-niter error, outside the timed region
+        _lib.check(lib.fsht_qu2eb(sht._h, p(Q), p(U), p(Eo), p(Bo), it, _lib.FSHT_DEVICE), sht._h)
+        torch.cuda.synchronize()
+        errs[str(it)] = max(float((Eo - E).abs().max()), float((Bo - B).abs().max())) / scale
+    del E, B
+    torch.cuda.empty_cache()
+
+    def step():  # Q/U are overwritten by their own round trip (kept in HBM)
+        _lib.check(lib.fsht_qu2eb(sht._h, p(Q), p(U), p(Eo), p(Bo), args.niter, _lib.FSHT_DEVICE), sht._h)
+        _lib.check(lib.fsht_eb2qu(sht._h, p(Eo), p(Bo), p(Q), p(U), _lib.FSHT_DEVICE), sht._h)
+
+    sht._use_torch_stream()  # the events below are recorded on torch's current stream
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    _lib.check(lib.fsht_profile(sht._h, 1), sht._h)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(args.steps):
+        step()
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1)
+    prof = (ctypes.c_double * 14)()
+    _lib.check(lib.fsht_profile_read(sht._h, prof, 14), sht._h)
+    kern = {KERNEL_CLASSES[c]: {"launches": int(prof[2 * c]), "ms": prof[2 * c + 1]} for c in range(7)}
+    fpair = 4 * legendre_flops_per_map(args.nside, lmax, sht.ring_tab())
+    tf = {k: fpair * args.nsim / (kern[k]["ms"] / max(1, kern[k]["launches"]) / 1e3) / 1e12
+          for k in ("legendre_analysis", "legendre_synthesis")}
+    print(json.dumps({
+        "metric": "Q/U pairs/s qu2eb+eb2qu (spin 2; not the benchmark value)",
+        "value": args.nsim * args.steps / (ms / 1e3), "unit": "QU pairs/s", "ms_per_step": ms / args.steps,
+        "steps": args.steps, "warmup": args.warmup, "dtype": "f64",
+        "config": {"workload": f"qu2eb(niter={args.niter})+eb2qu nside={args.nside} lmax={lmax} nsim={args.nsim} "
+                               "Q/U from seeded E/B (C_l^BB = 0.05 C_l^EE)",
+                   "nside": args.nside, "lmax": lmax, "nsim": args.nsim, "niter": args.niter},
+        "legendre_tflops": tf,
+        "eb_alm_rel_err_vs_input_by_niter": errs,
+        "kernels_ms_per_step": {k: v["ms"] / args.steps for k, v in kern.items()},
+    }))
+
+
 def host_round_trip(sht, maps_dev, args):
     """t2alm(niter) + alm2t on host arrays: pageable NumPy maps in (staged
     through pinned buffers, FFTs overlapping the transfer), pinned alms and
@@ -162,6 +247,9 @@ def main():
     ap.add_argument("--host", action="store_true",
                     help="time the round trip from host arrays instead (pageable maps in, pinned alms/maps out): "
                          "the PCIe-inclusive rate, reported separately from the HBM-resident value")
+    ap.add_argument("--pol", action="store_true",
+                    help="time the spin-2 round trip qu2eb(niter)+eb2qu instead and report per-niter E/B errors "
+                         "(reported separately from the temperature value)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device copy of MIB MiB (HBM-counter calibration under rocprofv3)")
     args = ap.parse_args()
@@ -190,6 +278,9 @@ def main():
     from fastsht_amd import SHT, _lib
 
     torch.cuda.set_device(0)
+    if args.pol:
+        pol_round_trip(args, lmax)
+        return
     sht = SHT(args.nside, lmax, args.nsim, niter=args.niter)
     alm, maps = make_inputs(sht, args.seed)
     alm_out = torch.empty_like(alm.permute(2, 1, 0).contiguous()).permute(2, 1, 0)

@@ -120,6 +120,7 @@ struct Handle {
   long long* d_seedoff = nullptr;
   int* d_seed_st = nullptr;
   double2* d_seed_v = nullptr;
+  double* d_seed_pm1 = nullptr;  // P_{st-1} per (m, ring, l-block): spin-2 functions at a block restart
   long long* d_synoff = nullptr;
   int* d_syn_st = nullptr;      // first l whose P_lm is kept, per (m, ring): the synthesis start
   double2* d_syn_v = nullptr;   // (P_st, P_st+1) at that start

@@ -142,7 +142,8 @@ __global__ __launch_bounds__(256, SPIN ? FSHT_ANA_MINBLK2 : FSHT_ANA_MINBLK0) vo
     const LegItem* __restrict__ items, const RingGeo* __restrict__ geo, const int* __restrict__ i0tab,
     const double2* __restrict__ ab, const double2* __restrict__ spc, const long long* __restrict__ aboff,
     const long long* __restrict__ seedoff, const int* __restrict__ seed_st, const double2* __restrict__ seed_v,
-    const double* __restrict__ F, const long long* __restrict__ goff, double* __restrict__ alm0,
+    const double* __restrict__ seed_pm1, const double* __restrict__ F, const long long* __restrict__ goff,
+    double* __restrict__ alm0,
     double* __restrict__ alm1, int nsim, int nsimp, int lmax, int nhr, double pa, double beta, int want0,
     int want1) {
   constexpr int RS = SPIN ? kAnaRings2 : kAnaRings0;  // rings per sub-block
@@ -211,7 +212,9 @@ __global__ __launch_bounds__(256, SPIN ? FSHT_ANA_MINBLK2 : FSHT_ANA_MINBLK0) vo
       rec.bw = g.z / s2;
       rec.t2 = 2.0 / s2;
       rec.cw = (double)m / s2;
-      rec.p1 = 0.0;
+      // a restart at the block start continues the recurrence: the spin-2
+      // functions at l = st also read P_{st-1}
+      rec.p1 = (SPIN && rec.st == L0) ? seed_pm1[so] : 0.0;
       rec.p2 = 0.0;
     }
     // ---- B fragments: the wave's 16 columns of the folded, rotated ring
@@ -612,7 +615,8 @@ hipError_t launch_analysis(Handle* h, const double* F, double* alm0, double* alm
     const LegItem* items = h->d_ana_items[sp][ks];
 #define FSHT_LAUNCH_ANA(SPV, KSV)                                                                               \
   hipLaunchKernelGGL((k_analysis<SPV, KSV>), dim3(n), dim3(256), 0, h->stream, items, h->d_geo, h->d_i0, h->d_ab, \
-                     h->d_spin, h->d_aboff, h->d_seedoff, h->d_seed_st, h->d_seed_v, F, h->d_goff, alm0, alm1,  \
+                     h->d_spin, h->d_aboff, h->d_seedoff, h->d_seed_st, h->d_seed_v, h->d_seed_pm1, F, h->d_goff,  \
+                     alm0, alm1,                                                                                  \
                      h->nsim, h->nsimp, h->lmax, h->nhr, h->pa, beta, want0, want1)
     if (sp) {
       if (ks) FSHT_LAUNCH_ANA(2, 2); else FSHT_LAUNCH_ANA(2, 1);

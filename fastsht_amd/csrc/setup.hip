@@ -208,7 +208,7 @@ __device__ static void pow_scaled_dev(double x, int n, double* mant, long long* 
 __global__ void k_seeds(int lmax, int nhr, const RingGeo* __restrict__ geo, const int* __restrict__ i0,
                         const double* __restrict__ dfac, const double2* __restrict__ ab,
                         const long long* __restrict__ aboff, const long long* __restrict__ seedoff,
-                        int* __restrict__ seed_st, double2* __restrict__ seed_v,
+                        int* __restrict__ seed_st, double2* __restrict__ seed_v, double* __restrict__ seed_pm1,
                         const long long* __restrict__ synoff, int* __restrict__ syn_st,
                         double2* __restrict__ syn_v) {
   const int m = blockIdx.y;
@@ -220,6 +220,7 @@ __global__ void k_seeds(int lmax, int nhr, const RingGeo* __restrict__ geo, cons
   for (int b = 0; b < nb; ++b) {
     seed_st[base + (long long)b * nr] = 0x3fffffff;
     seed_v[base + (long long)b * nr] = make_double2(0.0, 0.0);
+    seed_pm1[base + (long long)b * nr] = 0.0;
   }
   // synthesis start: the first kept l of the ring over the whole l range
   const long long ys = synoff[m] + (i - i0[m]);
@@ -284,6 +285,9 @@ __global__ void k_seeds(int lmax, int nhr, const RingGeo* __restrict__ geo, cons
       v1 = val;
       seed_st[base + (long long)b * nr] = st;
       seed_v[base + (long long)b * nr] = make_double2(v1, 0.0);
+      // P_{st-1} for a restart at the block start (the spin-2 functions at st
+      // need it); at the ring's first kept l it is dropped like everything before
+      seed_pm1[base + (long long)b * nr] = (st == first || l == m) ? 0.0 : ldexp(p2, (int)e);
     }
   }
 }
@@ -299,7 +303,7 @@ hipError_t launch_seeds(Handle* h) {
   e = hipMemcpyAsync(d_dfac, dfac.data(), dfac.size() * sizeof(double), hipMemcpyHostToDevice, h->stream);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_seeds, grid, dim3(256), 0, h->stream, h->lmax, h->nhr, h->d_geo, h->d_i0, d_dfac,
-                     h->d_ab, h->d_aboff, h->d_seedoff, h->d_seed_st, h->d_seed_v, h->d_synoff, h->d_syn_st,
+                     h->d_ab, h->d_aboff, h->d_seedoff, h->d_seed_st, h->d_seed_v, h->d_seed_pm1, h->d_synoff, h->d_syn_st,
                      h->d_syn_v);
   e = hipGetLastError();
   if (e != hipSuccess) return e;
@@ -630,6 +634,7 @@ static hipError_t make_seeds(Handle* h) {
   hipFree(h->d_seedoff);
   hipFree(h->d_seed_st);
   hipFree(h->d_seed_v);
+  hipFree(h->d_seed_pm1);
   hipFree(h->d_synoff);
   hipFree(h->d_syn_st);
   hipFree(h->d_syn_v);
@@ -637,12 +642,14 @@ static hipError_t make_seeds(Handle* h) {
   h->d_seedoff = h->d_synoff = nullptr;
   h->d_seed_st = h->d_syn_st = nullptr;
   h->d_seed_v = h->d_syn_v = nullptr;
+  h->d_seed_pm1 = nullptr;
   hipError_t e;
   if ((e = upload(&h->d_i0, h->i0)) != hipSuccess) return e;
   if ((e = upload(&h->d_seedoff, h->seedoff)) != hipSuccess) return e;
   const long long nseed = h->seedoff[h->lmax + 1];
   if ((e = hipMalloc(&h->d_seed_st, std::max(1LL, nseed) * sizeof(int))) != hipSuccess) return e;
   if ((e = hipMalloc(&h->d_seed_v, std::max(1LL, nseed) * sizeof(double2))) != hipSuccess) return e;
+  if ((e = hipMalloc(&h->d_seed_pm1, std::max(1LL, nseed) * sizeof(double))) != hipSuccess) return e;
   if ((e = upload(&h->d_synoff, h->synoff)) != hipSuccess) return e;
   const long long nsyn = h->synoff[h->lmax + 1];
   if ((e = hipMalloc(&h->d_syn_st, std::max(1LL, nsyn) * sizeof(int))) != hipSuccess) return e;

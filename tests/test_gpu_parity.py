@@ -11,9 +11,13 @@ pytestmark = pytest.mark.gpu
 TOL = 1e-12
 
 # (128, 40) and (64, 30): lmax far below 2*nside, so most polar rings carry
-# no kept P_lm for the larger orders and drop out of the per-order ring sets
+# no kept P_lm for the larger orders and drop out of the per-order ring sets.
+# (128, 257) and (128, 300): more than one 256-l analysis block per order, so
+# the analysis restarts its recurrence from the block seeds (the spin-2
+# functions at the restart also need P_{l-1})
 CASES = [(1, 2, 3), (2, 5, 2), (4, 8, 3), (4, 16, 5), (8, 20, 17), (8, 31, 4), (16, 47, 33),
-         (16, 64, 3), (32, 95, 20), (32, 40, 16), (64, 30, 2), (128, 40, 3)]
+         (16, 64, 3), (32, 95, 20), (32, 40, 16), (64, 30, 2), (128, 40, 3), (128, 257, 2),
+         (128, 300, 2)]
 
 
 @pytest.fixture(scope="module")
