@@ -172,6 +172,16 @@ __global__ __launch_bounds__(256, SPIN ? FSHT_ANA_MINBLK2 : FSHT_ANA_MINBLK0) vo
   const int ocomp = otype >> 1, oreim = otype & 1;
   bool active = colok && !(m == 0 && oreim == 1);
   if (SPIN) active = active && (ocomp == 0 ? want0 : want1);
+  // operand column, fold flip and sign of each Legendre function (spin 2: F1, F2)
+  int scol[SPIN ? 2 : 1], sflip[SPIN ? 2 : 1];
+  double ssign[SPIN ? 2 : 1];
+#pragma unroll
+  for (int f = 0; f < (SPIN ? 2 : 1); ++f) {
+    const Src s = ana_src(SPIN, otype, f);
+    scol[f] = (s.comp * 2 + s.reim) * nsimp + sim0 + li;
+    sflip[f] = s.flip;
+    ssign[f] = active ? s.sign : 0.0;
+  }
 
   // recurrence coefficients of the block (zero beyond lmax: P stays 0 there)
   {
@@ -229,11 +239,9 @@ __global__ __launch_bounds__(256, SPIN ? FSHT_ANA_MINBLK2 : FSHT_ANA_MINBLK0) vo
       const int i = i0 + rb + r;
       const bool valid = i <= nhr;
       const int ic = valid ? i : nhr;
-      const Src s = ana_src(SPIN, otype, f);
-      const int fold = par ^ s.flip;
-      const int col = (s.comp * 2 + s.reim) * nsimp + sim0 + li;
-      const double v = F[fb + ((long long)fold * nr + (ic - i0)) * ncols + col];
-      bfr[kj] = (active && valid) ? s.sign * v : 0.0;
+      const int fold = par ^ sflip[f];
+      const double v = F[fb + ((long long)fold * nr + (ic - i0)) * ncols + scol[f]];
+      bfr[kj] = valid ? ssign[f] * v : 0.0;
     }
 
 #pragma unroll
