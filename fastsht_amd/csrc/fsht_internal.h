@@ -34,7 +34,7 @@ struct FftJob {
 
 struct BlueInfo {
   long long chirp_off;  // offset of w_j = exp(-i pi j^2 / L), j < L (L = Bluestein length)
-  long long filt_off;   // offset of FFT_M(b)/M (M entries)
+  long long filt_off;   // offset of FFT_M(b)/M (M entries, bit-reversed order)
 };
 
 // Work item of the Legendre kernels.

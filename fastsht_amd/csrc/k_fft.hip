@@ -134,7 +134,15 @@ hipError_t build_bluestein(Handle* h) {
             if (j) bx[M - j] = make_double2(c, s);
           }
           host_fft(bx, -1);
-          for (int k = 0; k < M; ++k) filt[f0 + k] = make_double2(bx[k].x / M, bx[k].y / M);
+          // stored in bit-reversed order: the kernels multiply the bit-reversed
+          // output of the forward DIF pass position by position (coalesced reads)
+          const int lgM = ilog2(M);
+          for (int k = 0; k < M; ++k) {
+            unsigned rk = 0;
+            for (int b = 0; b < lgM; ++b)
+              if (k & (1 << b)) rk |= 1u << (lgM - 1 - b);
+            filt[f0 + k] = make_double2(bx[rk].x / M, bx[rk].y / M);
+          }
         }
       });
     }
@@ -391,7 +399,7 @@ __device__ void ring_fft_half(double2* buf, int n, const double2* __restrict__ w
       buf[j] = (j < L) ? cmul(make_double2(in[2 * j], in[2 * j + 1]), w[j]) : make_double2(0.0, 0.0);
     __syncthreads();
     lds_dif<M>(buf, Mmax, tw);
-    for (int r = threadIdx.x; r < M; r += blockDim.x) buf[r] = cmul(buf[r], fl[brev(r, lg)]);
+    for (int r = threadIdx.x; r < M; r += blockDim.x) buf[r] = cmul(buf[r], fl[r]);
     __syncthreads();
     lds_dit_inv<M>(buf, Mmax, tw);
     for (int k = threadIdx.x; k <= L; k += blockDim.x) {
@@ -429,7 +437,7 @@ __device__ void ring_fft_half(double2* buf, int n, const double2* __restrict__ w
     }
     __syncthreads();
     lds_dif<M>(buf, Mmax, tw);
-    for (int r = threadIdx.x; r < M; r += blockDim.x) buf[r] = cmul(buf[r], fl[brev(r, lg)]);
+    for (int r = threadIdx.x; r < M; r += blockDim.x) buf[r] = cmul(buf[r], fl[r]);
     __syncthreads();
     lds_dit_inv<M>(buf, Mmax, tw);
     for (int j = threadIdx.x; j < L; j += blockDim.x) {
@@ -485,7 +493,7 @@ __global__ __launch_bounds__(FftShape<MAXM>::TPB) void k_ring_fft(const FftJob* 
     __syncthreads();
     lds_dif<M>(buf, Mmax, tw);
     if (isblue) {
-      for (int r = threadIdx.x; r < M; r += blockDim.x) buf[r] = cmul(buf[r], fl[brev(r, lg)]);
+      for (int r = threadIdx.x; r < M; r += blockDim.x) buf[r] = cmul(buf[r], fl[r]);
       __syncthreads();
       lds_dit_inv<M>(buf, Mmax, tw);
     }
@@ -550,7 +558,7 @@ __global__ __launch_bounds__(FftShape<MAXM>::TPB) void k_ring_fft(const FftJob* 
     __syncthreads();
     if (isblue) {
       lds_dif<M>(buf, Mmax, tw);
-      for (int r = threadIdx.x; r < M; r += blockDim.x) buf[r] = cmul(buf[r], fl[brev(r, lg)]);
+      for (int r = threadIdx.x; r < M; r += blockDim.x) buf[r] = cmul(buf[r], fl[r]);
       __syncthreads();
     }
     lds_dit_inv<M>(buf, Mmax, tw);
