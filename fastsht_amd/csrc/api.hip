@@ -49,7 +49,7 @@ static int fail(Handle* h, int code, const std::string& msg) {
 
 static void free_all(Handle* h) {
   void* ptrs[] = {h->d_geo, h->d_i0, h->d_ab, h->d_spin, h->d_aboff, h->d_tw, h->d_slot, h->d_seedoff,
-                  h->d_seed_st, h->d_seed_v, h->d_seed_pm1, h->d_synoff, h->d_syn_st, h->d_syn_v, h->d_goff, h->d_ana_items[0][0], h->d_ana_items[0][1],
+                  h->d_seed_st, h->d_seed_v, h->d_seed_p, h->d_synoff, h->d_syn_st, h->d_syn_v, h->d_goff, h->d_ana_items[0][0], h->d_ana_items[0][1],
                   h->d_ana_items[1][0], h->d_ana_items[1][1], h->d_syn_items[0], h->d_syn_items[1], h->d_fft_jobs, h->d_blue, h->d_chirp, h->d_filt,
                   h->d_twid, h->ws_spec, h->ws_specT, h->ws_G, h->ws_F, h->ws_alm[0], h->ws_alm[1]};
   for (void* p : ptrs)

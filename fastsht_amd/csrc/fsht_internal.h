@@ -118,12 +118,14 @@ struct Handle {
   double2* d_tw = nullptr;      // (cos m phi0, sin m phi0) per (m, north ring)
   int* d_slot = nullptr;        // relative PERM slot*4 + class per (m, north ring)
   long long* d_seedoff = nullptr;
+  // per (m, ring, l-block): first kept l in the block, st, and the recurrence
+  // state at the 32-step chunk start ls <= st (l - m = 0 mod 32)
   int* d_seed_st = nullptr;
-  double2* d_seed_v = nullptr;
-  double* d_seed_pm1 = nullptr;  // P_{st-1} per (m, ring, l-block): spin-2 functions at a block restart
+  double2* d_seed_v = nullptr;  // (P_ls-2, P_ls-1)
+  double* d_seed_p = nullptr;   // P_ls
   long long* d_synoff = nullptr;
   int* d_syn_st = nullptr;      // first l whose P_lm is kept, per (m, ring): the synthesis start
-  double2* d_syn_v = nullptr;   // (P_st, P_st+1) at that start
+  double2* d_syn_v = nullptr;   // (P_ls-2, P_ls-1) at the chunk start ls <= that l
   long long* d_goff = nullptr;
   LegItem* d_ana_items[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};  // [spin 0/2][full / 2-tile column blocks]
   LegItem* d_syn_items[2] = {nullptr, nullptr};

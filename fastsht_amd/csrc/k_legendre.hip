@@ -19,8 +19,8 @@
 // FP64 MFMA and FP64 VALU share one datapath on gfx950, so the recurrence is
 // kept lean: coefficients are staged in LDS, every global gather is
 // branch-free (clamped addresses, selected-away values) so loads are issued in
-// batches, and the seed injection (2 compares + 2 selects per step) only runs
-// in the 32-step chunks where some lane of the wave actually starts.
+// batches, and a ring enters the recurrence at a chunk boundary from a stored
+// state (two selects per chunk, none per step).
 #include <type_traits>
 
 #include "fsht_internal.h"
@@ -86,27 +86,31 @@ __device__ inline double spin_f(int f, double P, double Pm1, double2 sp, int l, 
   return (Pm1 * lam - P * (2.0 * x * (dl - 1.0))) * cw * nl;
 }
 
-// Per-thread recurrence state for one ring.
+// Per-thread recurrence state for one ring.  The recurrence is 0 until the
+// chunk that starts at l = ls, where (p1, p2) are loaded from sv.
 struct Rec {
   double p1, p2;  // P_{l-1}, P_{l-2}
   double x, bw, t2, cw;
-  int st;         // first non-negligible l (seeds injected at st, st+1)
-  double2 sv;     // P_st, P_st+1
+  int ls;         // window start at which the ring enters (0x3fffffff: never)
+  double2 sv;     // (P_{ls-1}, P_{ls-2})
 };
+
+__device__ inline void enter_chunk(Rec& r, int lc) {
+  if (__any(r.ls == lc)) {
+    const bool h = r.ls == lc;
+    r.p1 = h ? r.sv.x : r.p1;
+    r.p2 = h ? r.sv.y : r.p2;
+  }
+}
 
 // Run 32 recurrence steps l = lc .. lc+31 with coefficients cf[0..31]
 // (zero beyond lmax) and hand every value (P, or F_f for spin 2) to emit(s, v).
-template <int SPIN, bool INJ, class Emit>
+template <int SPIN, class Emit>
 __device__ inline void gen32(Rec& r, const double2* cf, const double2* scf, int lc, int m, int f, Emit&& emit) {
 #pragma unroll
   for (int s = 0; s < 32; ++s) {
     const double2 c = cf[s];
-    double p = fma(c.x * r.x, r.p1, -(c.y * r.p2));
-    if (INJ) {
-      const int l = lc + s;
-      p = (l == r.st) ? r.sv.x : p;
-      p = (l == r.st + 1) ? r.sv.y : p;
-    }
+    const double p = fma(c.x * r.x, r.p1, -(c.y * r.p2));
     double v = p;
     if (SPIN) v = spin_f(f, p, r.p1, scf[s], lc + s, m, r.x, r.bw, r.t2, r.cw);
     emit(s, v);
@@ -122,7 +126,9 @@ __device__ inline void gen32(Rec& r, const double2* cf, const double2* scf, int 
 // columns and all 8 l-tiles of the block.  Rings are consumed in sub-blocks:
 // each wave loads its X columns for the sub-block into registers, then the
 // workgroup sweeps the l-block generating P (one ring per thread, plain FP64
-// recurrence restarted from the resident seeds) into LDS.
+// recurrence entered from the resident start states) into LDS.  The 32-step
+// windows of class par start at l = L0 + 32t + par, so the class's values
+// are the even steps of every window.
 // ---------------------------------------------------------------------------
 
 // Workgroups per CU the register allocation must allow (spin 0 / spin 2).
@@ -142,15 +148,15 @@ __global__ __launch_bounds__(256, SPIN ? FSHT_ANA_MINBLK2 : FSHT_ANA_MINBLK0) vo
     const LegItem* __restrict__ items, const RingGeo* __restrict__ geo, const int* __restrict__ i0tab,
     const double2* __restrict__ ab, const double2* __restrict__ spc, const long long* __restrict__ aboff,
     const long long* __restrict__ seedoff, const int* __restrict__ seed_st, const double2* __restrict__ seed_v,
-    const double* __restrict__ seed_pm1, const double* __restrict__ F, const long long* __restrict__ goff,
+    const double* __restrict__ seed_p, const double* __restrict__ F, const long long* __restrict__ goff,
     double* __restrict__ alm0,
     double* __restrict__ alm1, int nsim, int nsimp, int lmax, int nhr, double pa, double beta, int want0,
     int want1) {
   constexpr int RS = SPIN ? kAnaRings2 : kAnaRings0;  // rings per sub-block
   constexpr int NCOMP = SPIN ? 2 : 1;
   __shared__ double Pt[256][17];
-  __shared__ double2 cf[kAnaLB];
-  __shared__ double2 scf[SPIN ? kAnaLB : 1];
+  __shared__ double2 cf[kAnaLB + 32];  // l = L0 .. L0+287: windows of class 1 end at L0+256
+  __shared__ double2 scf[SPIN ? kAnaLB + 32 : 1];
   const LegItem it = items[blockIdx.x];
   if (it.m < 0) return;
   const int m = it.m, par = it.par;
@@ -164,7 +170,7 @@ __global__ __launch_bounds__(256, SPIN ? FSHT_ANA_MINBLK2 : FSHT_ANA_MINBLK0) vo
   const int nt = (L1 - L0 - par + 31) / 32;  // l-tiles in this block
   // this wave's output column tile and K half
   constexpr int NKB = 64 / KS;                 // K-steps (of 4 rings) per wave and sub-block
-  const int kh = wave / (4 / KS);              // 0 for KS = 1
+  const int kh = (KS == 1) ? 0 : wave / 2;
   const int c0 = (it.colblk * (4 / KS) + wave % (4 / KS)) * 16;  // colblk counts blocks of 4/KS tiles
   const bool colok = c0 < ncols;
   const int otype = colok ? c0 / nsimp : 0;
@@ -172,22 +178,29 @@ __global__ __launch_bounds__(256, SPIN ? FSHT_ANA_MINBLK2 : FSHT_ANA_MINBLK0) vo
   const int ocomp = otype >> 1, oreim = otype & 1;
   bool active = colok && !(m == 0 && oreim == 1);
   if (SPIN) active = active && (ocomp == 0 ? want0 : want1);
-  // operand column, fold flip and sign of each Legendre function (spin 2: F1, F2)
-  int scol[SPIN ? 2 : 1], sflip[SPIN ? 2 : 1];
-  double ssign[SPIN ? 2 : 1];
-#pragma unroll
-  for (int f = 0; f < (SPIN ? 2 : 1); ++f) {
-    const Src s = ana_src(SPIN, otype, f);
-    scol[f] = (s.comp * 2 + s.reim) * nsimp + sim0 + li;
-    sflip[f] = s.flip;
-    ssign[f] = active ? s.sign : 0.0;
+  // operand column, fold flip and sign of each Legendre function (spin 2: F1,
+  // F2), held in scalars: the function index of a K-step depends on the wave
+  // when KS = 2, and a runtime index into a local array would live in scratch
+  int scol0, sflip0, scol1 = 0, sflip1 = 0;
+  double ssign0, ssign1 = 0.0;
+  {
+    const Src s = ana_src(SPIN, otype, 0);
+    scol0 = (s.comp * 2 + s.reim) * nsimp + sim0 + li;
+    sflip0 = s.flip;
+    ssign0 = active ? s.sign : 0.0;
+  }
+  if (SPIN) {
+    const Src s = ana_src(SPIN, otype, 1);
+    scol1 = (s.comp * 2 + s.reim) * nsimp + sim0 + li;
+    sflip1 = s.flip;
+    ssign1 = active ? s.sign : 0.0;
   }
 
   // recurrence coefficients of the block (zero beyond lmax: P stays 0 there)
   {
     const double2* abm = ab + aboff[m];
     const double2* spm = spc + aboff[m];
-    for (int k = tid; k < kAnaLB; k += 256) {
+    for (int k = tid; k < kAnaLB + 32; k += 256) {
       const int l = L0 + k;
       cf[k] = (l <= lmax) ? abm[l - m] : make_double2(0.0, 0.0);
       if (SPIN) scf[k] = (l <= lmax) ? spm[l - m] : make_double2(0.0, 0.0);
@@ -215,16 +228,18 @@ __global__ __launch_bounds__(256, SPIN ? FSHT_ANA_MINBLK2 : FSHT_ANA_MINBLK0) vo
       const int ic = valid ? i : nhr;
       const RingGeo g = geo[ic];
       const long long so = seedoff[m] + (long long)it.blk * nr + (ic - i0);
-      rec.st = valid ? seed_st[so] : 0x3fffffff;
-      rec.sv = seed_v[so];
+      const int st = valid ? seed_st[so] : 0x3fffffff;
+      const double2 q = seed_v[so];  // P at ls-2, ls-1
+      const double q2 = seed_p[so];  // P at ls
+      // class 1 enters one step later, at ls + 1, from (P_ls, P_ls-1)
+      rec.ls = (st > lmax) ? 0x3fffffff : L0 + ((st - L0) & ~31) + par;
+      rec.sv = par ? make_double2(q2, q.y) : make_double2(q.y, q.x);
       rec.x = g.z;
       const double s2 = g.s * g.s;
       rec.bw = g.z / s2;
       rec.t2 = 2.0 / s2;
       rec.cw = (double)m / s2;
-      // a restart at the block start continues the recurrence: the spin-2
-      // functions at l = st also read P_{st-1}
-      rec.p1 = (SPIN && rec.st == L0) ? seed_pm1[so] : 0.0;
+      rec.p1 = 0.0;
       rec.p2 = 0.0;
     }
     // ---- B fragments: the wave's 16 columns of the folded, rotated ring
@@ -239,30 +254,24 @@ __global__ __launch_bounds__(256, SPIN ? FSHT_ANA_MINBLK2 : FSHT_ANA_MINBLK0) vo
       const int i = i0 + rb + r;
       const bool valid = i <= nhr;
       const int ic = valid ? i : nhr;
-      const int fold = par ^ sflip[f];
-      const double v = F[fb + ((long long)fold * nr + (ic - i0)) * ncols + scol[f]];
-      bfr[kj] = valid ? ssign[f] * v : 0.0;
+      const int fold = par ^ (f ? sflip1 : sflip0);
+      const double v = F[fb + ((long long)fold * nr + (ic - i0)) * ncols + (f ? scol1 : scol0)];
+      bfr[kj] = valid ? (f ? ssign1 : ssign0) * v : 0.0;
     }
 
-#pragma unroll
+    // spin 2 keeps the chunk loop rolled: unrolled, the compiler hoists the
+    // per-l spin-2 constants of all 8 chunks out of the ring loop and spills
+#pragma unroll (SPIN ? 1 : 8)
     for (int t = 0; t < 8; ++t) {
       if (t < nt) {
-        const int lc = L0 + 32 * t;
-        // generate 32 l-steps; keep the 16 values of the parity class
-        double keep = 0.0;
+        const int lc = L0 + 32 * t + par;
+        // generate 32 l-steps; keep the 16 values of the parity class (even steps)
         auto emit = [&](int s, double v) {
-          if ((s & 1) == 0) {
-            keep = v;
-          } else {
-            Pt[tid][s >> 1] = par ? v : keep;
-          }
+          if ((s & 1) == 0) Pt[tid][s >> 1] = v;
         };
         if (64 * (wave % (RS / 64)) < kvalid) {
-          const bool inj = (rec.st >= lc - 1) && (rec.st <= lc + 31);
-          if (__any(inj))
-            gen32<SPIN, true>(rec, cf + 32 * t, scf + (SPIN ? 32 * t : 0), lc, m, myf, emit);
-          else
-            gen32<SPIN, false>(rec, cf + 32 * t, scf + (SPIN ? 32 * t : 0), lc, m, myf, emit);
+          enter_chunk(rec, lc);
+          gen32<SPIN>(rec, cf + 32 * t + par, scf + (SPIN ? 32 * t + par : 0), lc, m, myf, emit);
         }
         __syncthreads();
         if (active) {
@@ -341,8 +350,8 @@ __global__ __launch_bounds__(256, SPIN ? FSHT_ANA_MINBLK2 : FSHT_ANA_MINBLK0) vo
 // the two waves a SIMD holds drift apart and one's recurrence runs while the
 // other's MFMAs do.  Per 32-step l chunk a wave
 //   1. issues the global loads of the next chunk's alm operand (registers),
-//   2. runs the plain FP64 recurrence for its rings (started at the ring's
-//      first kept l from the resident seeds), keeping the values of its class
+//   2. runs the plain FP64 recurrence for its rings (each entered at the chunk
+//      holding its first kept l from a resident state), keeping the values of its class
 //      (spin 2: F1 of its class and F2 of the other one) in LDS,
 //   3. contracts them with 4 x NT x NKS v_mfma_f64_16x16x4 (A = P from LDS,
 //      B = alm from registers), each accumulator's K chain back to back.
@@ -406,8 +415,10 @@ __device__ inline void synth_wave(SynShared<SPIN>& sh, const LegItem it, const R
     const int myring = rbase + lane;
     const bool ring_ok = myring <= nhr;
     const int ic = ring_ok ? myring : nhr;
-    rec.st = ring_ok ? syn_st[ys0 + (ic - i0)] : 0x3fffffff;
-    rec.sv = syn_v[ys0 + (ic - i0)];
+    const int st = ring_ok ? syn_st[ys0 + (ic - i0)] : 0x3fffffff;
+    const double2 q = syn_v[ys0 + (ic - i0)];  // P at ls-2, ls-1
+    rec.ls = (st > lmax) ? 0x3fffffff : m + ((st - m) & ~31);
+    rec.sv = make_double2(q.y, q.x);
     const RingGeo g = geo[ic];
     rec.x = g.z;
     const double s2 = g.s * g.s;
@@ -477,12 +488,12 @@ __device__ inline void synth_wave(SynShared<SPIN>& sh, const LegItem it, const R
 #pragma unroll
     for (int u = 0; u < NT; ++u) D[a][u] = d4{0.0, 0.0, 0.0, 0.0};
 
-  // first chunk: the one holding the wave's earliest kept l (P is 0 before it)
-  int stmin = rec.st;
+  // first chunk: the earliest one at which a ring of the wave enters
+  int lsmin = rec.ls;
 #pragma unroll
-  for (int o = 32; o > 0; o >>= 1) stmin = min(stmin, __shfl_xor(stmin, o));
-  stmin = __builtin_amdgcn_readfirstlane(stmin);  // wave-uniform: keeps the loop scalar
-  const int lc0 = (stmin > lmax) ? lmax + 1 : m + ((stmin - m) / 32) * 32;
+  for (int o = 32; o > 0; o >>= 1) lsmin = min(lsmin, __shfl_xor(lsmin, o));
+  lsmin = __builtin_amdgcn_readfirstlane(lsmin);  // wave-uniform: keeps the loop scalar
+  const int lc0 = (lsmin > lmax) ? lmax + 1 : lsmin;
 
   // alm operand of the current chunk; the next chunk's is loaded into the
   // same registers right after this chunk's MFMAs are issued, so the loads
@@ -505,17 +516,12 @@ __device__ inline void synth_wave(SynShared<SPIN>& sh, const LegItem it, const R
     // ---- 32 recurrence steps; step s has l - m parity s & 1 (lc - m is even)
     const double2* c = cf + jseg;
     const double2* sc = sf + (SPIN ? jseg : 0);
-    auto steps = [&](auto injc) {
-      constexpr bool INJ = decltype(injc)::value;
+    enter_chunk(rec, lc);
+    {
 #pragma unroll
       for (int s = 0; s < 32; ++s) {
         const double2 cc = c[s];
-        double p = fma(cc.x * rec.x, rec.p1, -(cc.y * rec.p2));
-        if (INJ) {
-          const int l = lc + s;
-          p = (l == rec.st) ? rec.sv.x : p;
-          p = (l == rec.st + 1) ? rec.sv.y : p;
-        }
+        const double p = fma(cc.x * rec.x, rec.p1, -(cc.y * rec.p2));
         if (SPIN) {
           // F1 of this class at steps of its parity, F2 of the other class at the others
           if ((s & 1) == CLS)
@@ -528,12 +534,7 @@ __device__ inline void synth_wave(SynShared<SPIN>& sh, const LegItem it, const R
         rec.p2 = rec.p1;
         rec.p1 = p;
       }
-    };
-    const bool inj = (rec.st >= lc - 1) && (rec.st <= lc + 31);
-    if (__any(inj))
-      steps(std::true_type{});
-    else
-      steps(std::false_type{});
+    }
     wave_lds_sync();
 
     // ---- contraction: ring tile a (16 rings) x column tile u, K = NV values
@@ -623,7 +624,7 @@ hipError_t launch_analysis(Handle* h, const double* F, double* alm0, double* alm
     const LegItem* items = h->d_ana_items[sp][ks];
 #define FSHT_LAUNCH_ANA(SPV, KSV)                                                                               \
   hipLaunchKernelGGL((k_analysis<SPV, KSV>), dim3(n), dim3(256), 0, h->stream, items, h->d_geo, h->d_i0, h->d_ab, \
-                     h->d_spin, h->d_aboff, h->d_seedoff, h->d_seed_st, h->d_seed_v, h->d_seed_pm1, F, h->d_goff,  \
+                     h->d_spin, h->d_aboff, h->d_seedoff, h->d_seed_st, h->d_seed_v, h->d_seed_p, F, h->d_goff,    \
                      alm0, alm1,                                                                                  \
                      h->nsim, h->nsimp, h->lmax, h->nhr, h->pa, beta, want0, want1)
     if (sp) {

@@ -176,11 +176,17 @@ void ring_tab(int nside, int nm, const std::vector<RingGeo>& geo, std::vector<in
 }
 
 // ---------------------------------------------------------------------------
-// Device: Legendre start seeds.  For order m, kept north ring i and analysis
-// l-block b (l in [m + b*LB, m + (b+1)*LB)), store the first l of the block at
-// which P_lm is kept (>= 2^-100), st, and P_st, P_st+1.  A plain
-// FP64 recurrence restarted from these two values reproduces the exponent-
-// rescaled recurrence bit for bit.  Block 0 doubles as the synthesis start.
+// Device: Legendre start states.  The Legendre kernels run the plain FP64
+// recurrence in chunks of 32 l-steps on the grid l - m = 0 (mod 32).  For
+// order m, kept north ring i and analysis l-block b (l in [m + b*LB,
+// m + (b+1)*LB)), store st, the first l of the block at which P_lm is kept
+// (>= 2^-100), and P at ls-2, ls-1, ls with ls the chunk start at or below
+// st.  A recurrence entered at ls (or at ls+1 from P_ls, P_ls-1) reproduces
+// the exponent-rescaled recurrence bit for bit: rescaling is by powers of two
+// and these values are >= 2^-300 (at most 33 steps of growth below 2^-100).
+// At ls = m the state is (P_mm, 0, P_mm): the coefficients at l = m are
+// (a, b) = (0, -1), so the recurrence yields P_mm there.  The synthesis state
+// is the one of the chunk holding the ring's first kept l over the whole range.
 // ---------------------------------------------------------------------------
 __device__ static void pow_scaled_dev(double x, int n, double* mant, long long* ex) {
   int ex0;
@@ -208,7 +214,7 @@ __device__ static void pow_scaled_dev(double x, int n, double* mant, long long* 
 __global__ void k_seeds(int lmax, int nhr, const RingGeo* __restrict__ geo, const int* __restrict__ i0,
                         const double* __restrict__ dfac, const double2* __restrict__ ab,
                         const long long* __restrict__ aboff, const long long* __restrict__ seedoff,
-                        int* __restrict__ seed_st, double2* __restrict__ seed_v, double* __restrict__ seed_pm1,
+                        int* __restrict__ seed_st, double2* __restrict__ seed_v, double* __restrict__ seed_p,
                         const long long* __restrict__ synoff, int* __restrict__ syn_st,
                         double2* __restrict__ syn_v) {
   const int m = blockIdx.y;
@@ -220,13 +226,11 @@ __global__ void k_seeds(int lmax, int nhr, const RingGeo* __restrict__ geo, cons
   for (int b = 0; b < nb; ++b) {
     seed_st[base + (long long)b * nr] = 0x3fffffff;
     seed_v[base + (long long)b * nr] = make_double2(0.0, 0.0);
-    seed_pm1[base + (long long)b * nr] = 0.0;
+    seed_p[base + (long long)b * nr] = 0.0;
   }
-  // synthesis start: the first kept l of the ring over the whole l range
   const long long ys = synoff[m] + (i - i0[m]);
   syn_st[ys] = 0x3fffffff;
   syn_v[ys] = make_double2(0.0, 0.0);
-  double y1 = 0.0;
   const double z = geo[i].z, s = geo[i].s;
   const double2* abm = ab + aboff[m];
   double ms;
@@ -239,8 +243,8 @@ __global__ void k_seeds(int lmax, int nhr, const RingGeo* __restrict__ geo, cons
   e += fe;
   double p2 = 0.0;
   int first = -1;
-  int cur_st = -1, cur_b = -1;
-  double v1 = 0.0;
+  double q0 = 0.0, q1 = 0.0, q2 = 0.0;  // P at ls-2, ls-1, ls of the latest chunk start ls
+  double n0 = 0.0, n1 = 0.0;            // P at the two steps before the next chunk start
   for (int l = m; l <= lmax; ++l) {
     double p;
     if (l == m) {
@@ -258,36 +262,36 @@ __global__ void k_seeds(int lmax, int nhr, const RingGeo* __restrict__ geo, cons
       e += sh;
       p = p1;
     }
+    const int jc = (l - m) & 31;
+    if (l == m) {
+      q2 = ldexp(p, (int)max(e, -2000LL));
+      q0 = q2;
+      q1 = 0.0;
+    } else if (jc == 30) {
+      n0 = ldexp(p, (int)max(e, -2000LL));
+    } else if (jc == 31) {
+      n1 = ldexp(p, (int)max(e, -2000LL));
+    } else if (jc == 0) {
+      q0 = n0;
+      q1 = n1;
+      q2 = ldexp(p, (int)max(e, -2000LL));
+    }
     if (first < 0 && p != 0.0) {
       int fp;
       frexp(p, &fp);
       if ((long long)fp + e >= kKeepExp) first = l;
     }
     if (first < 0) continue;
-    const double val = ldexp(p, (int)e);
     if (l == first) {
-      y1 = val;
       syn_st[ys] = first;
-      syn_v[ys] = make_double2(val, 0.0);
-    } else if (l == first + 1) {
-      syn_v[ys] = make_double2(y1, val);
-    }
-    // second seed value of the previous start (may fall in the next block)
-    if (cur_b >= 0 && l == cur_st + 1) {
-      seed_v[base + (long long)cur_b * nr] = make_double2(v1, val);
+      syn_v[ys] = make_double2(q0, q1);
     }
     const int b = (l - m) / kAnaLB;
     const int L0 = m + b * kAnaLB;
-    const int st = first > L0 ? first : L0;
-    if (l == st) {  // start of this block's non-zero run
-      cur_st = st;
-      cur_b = b;
-      v1 = val;
-      seed_st[base + (long long)b * nr] = st;
-      seed_v[base + (long long)b * nr] = make_double2(v1, 0.0);
-      // P_{st-1} for a restart at the block start (the spin-2 functions at st
-      // need it); at the ring's first kept l it is dropped like everything before
-      seed_pm1[base + (long long)b * nr] = (st == first || l == m) ? 0.0 : ldexp(p2, (int)e);
+    if (l == (first > L0 ? first : L0)) {  // start of this block's kept run
+      seed_st[base + (long long)b * nr] = l;
+      seed_v[base + (long long)b * nr] = make_double2(q0, q1);
+      seed_p[base + (long long)b * nr] = q2;
     }
   }
 }
@@ -303,7 +307,7 @@ hipError_t launch_seeds(Handle* h) {
   e = hipMemcpyAsync(d_dfac, dfac.data(), dfac.size() * sizeof(double), hipMemcpyHostToDevice, h->stream);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_seeds, grid, dim3(256), 0, h->stream, h->lmax, h->nhr, h->d_geo, h->d_i0, d_dfac,
-                     h->d_ab, h->d_aboff, h->d_seedoff, h->d_seed_st, h->d_seed_v, h->d_seed_pm1, h->d_synoff, h->d_syn_st,
+                     h->d_ab, h->d_aboff, h->d_seedoff, h->d_seed_st, h->d_seed_v, h->d_seed_p, h->d_synoff, h->d_syn_st,
                      h->d_syn_v);
   e = hipGetLastError();
   if (e != hipSuccess) return e;
@@ -634,7 +638,7 @@ static hipError_t make_seeds(Handle* h) {
   hipFree(h->d_seedoff);
   hipFree(h->d_seed_st);
   hipFree(h->d_seed_v);
-  hipFree(h->d_seed_pm1);
+  hipFree(h->d_seed_p);
   hipFree(h->d_synoff);
   hipFree(h->d_syn_st);
   hipFree(h->d_syn_v);
@@ -642,14 +646,14 @@ static hipError_t make_seeds(Handle* h) {
   h->d_seedoff = h->d_synoff = nullptr;
   h->d_seed_st = h->d_syn_st = nullptr;
   h->d_seed_v = h->d_syn_v = nullptr;
-  h->d_seed_pm1 = nullptr;
+  h->d_seed_p = nullptr;
   hipError_t e;
   if ((e = upload(&h->d_i0, h->i0)) != hipSuccess) return e;
   if ((e = upload(&h->d_seedoff, h->seedoff)) != hipSuccess) return e;
   const long long nseed = h->seedoff[h->lmax + 1];
   if ((e = hipMalloc(&h->d_seed_st, std::max(1LL, nseed) * sizeof(int))) != hipSuccess) return e;
   if ((e = hipMalloc(&h->d_seed_v, std::max(1LL, nseed) * sizeof(double2))) != hipSuccess) return e;
-  if ((e = hipMalloc(&h->d_seed_pm1, std::max(1LL, nseed) * sizeof(double))) != hipSuccess) return e;
+  if ((e = hipMalloc(&h->d_seed_p, std::max(1LL, nseed) * sizeof(double))) != hipSuccess) return e;
   if ((e = upload(&h->d_synoff, h->synoff)) != hipSuccess) return e;
   const long long nsyn = h->synoff[h->lmax + 1];
   if ((e = hipMalloc(&h->d_syn_st, std::max(1LL, nsyn) * sizeof(int))) != hipSuccess) return e;
