@@ -82,6 +82,10 @@ int fsht_create(int nside, int lmax, int nsim, int niter, int pol, fsht_handle**
   // the Legendre kernels address one alm column (nsim x (lmax+1) doubles) with 32-bit byte offsets
   if ((long long)nsim * (lmax + 1) >= (1LL << 28))
     return fail(nullptr, FSHT_ERR_ARG, "nsim*(lmax+1) must be below 2^28");
+  // ... and one order's folded spectrum (2*nside rings x 4 column blocks of nsim
+  // rounded to 16, spin 2) likewise
+  if (16LL * nside * 4 * (((long long)nsim + 15) / 16 * 16) >= (1LL << 32))
+    return fail(nullptr, FSHT_ERR_ARG, "nside*nsim too large (one order's spectrum must stay below 4 GiB)");
   if (niter < 0) return fail(nullptr, FSHT_ERR_ARG, "niter must be >= 0");
   fsht_handle* H = new fsht_handle();
   Handle* h = &H->h;

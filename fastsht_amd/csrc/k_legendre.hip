@@ -160,7 +160,8 @@ __global__ __launch_bounds__(256, SPIN ? FSHT_ANA_MINBLK2 : FSHT_ANA_MINBLK0) vo
   const LegItem it = items[blockIdx.x];
   if (it.m < 0) return;
   const int m = it.m, par = it.par;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // uniform: operand bases in scalar registers
   const int li = lane & 15, q = lane >> 4;
   const int ncols = 2 * NCOMP * nsimp;
   const int i0 = i0tab[m];
@@ -178,21 +179,23 @@ __global__ __launch_bounds__(256, SPIN ? FSHT_ANA_MINBLK2 : FSHT_ANA_MINBLK0) vo
   const int ocomp = otype >> 1, oreim = otype & 1;
   bool active = colok && !(m == 0 && oreim == 1);
   if (SPIN) active = active && (ocomp == 0 ? want0 : want1);
-  // operand column, fold flip and sign of each Legendre function (spin 2: F1,
-  // F2), held in scalars: the function index of a K-step depends on the wave
-  // when KS = 2, and a runtime index into a local array would live in scratch
-  int scol0, sflip0, scol1 = 0, sflip1 = 0;
+  // operand of each Legendre function (spin 2: F1, F2): element (ring row r,
+  // column sim0 + li) of the folded spectrum sits at base[r * ncols + li], with
+  // a wave-uniform base (source column, fold class).  Held in scalars: the
+  // function index of a K-step depends on the wave when KS = 2, and a runtime
+  // index into a local array would live in scratch.
+  const long long fb = goff[m] / (SPIN ? 1 : 2);
+  const double* base0;
+  const double* base1 = nullptr;
   double ssign0, ssign1 = 0.0;
   {
     const Src s = ana_src(SPIN, otype, 0);
-    scol0 = (s.comp * 2 + s.reim) * nsimp + sim0 + li;
-    sflip0 = s.flip;
+    base0 = F + fb + (long long)((par ^ s.flip) * nr) * ncols + (s.comp * 2 + s.reim) * nsimp + sim0;
     ssign0 = active ? s.sign : 0.0;
   }
   if (SPIN) {
     const Src s = ana_src(SPIN, otype, 1);
-    scol1 = (s.comp * 2 + s.reim) * nsimp + sim0 + li;
-    sflip1 = s.flip;
+    base1 = F + fb + (long long)((par ^ s.flip) * nr) * ncols + (s.comp * 2 + s.reim) * nsimp + sim0;
     ssign1 = active ? s.sign : 0.0;
   }
 
@@ -206,6 +209,8 @@ __global__ __launch_bounds__(256, SPIN ? FSHT_ANA_MINBLK2 : FSHT_ANA_MINBLK0) vo
       if (SPIN) scf[k] = (l <= lmax) ? spm[l - m] : make_double2(0.0, 0.0);
     }
   }
+  // every wave's recurrence reads coefficients staged by other waves
+  __syncthreads();
 
   d4 D[8];
 #pragma unroll
@@ -214,7 +219,28 @@ __global__ __launch_bounds__(256, SPIN ? FSHT_ANA_MINBLK2 : FSHT_ANA_MINBLK0) vo
   const int myf = SPIN ? (tid / RS) : 0;
   const int myr = SPIN ? (tid % RS) : tid;
 
-  const long long fb = goff[m] / (SPIN ? 1 : 2);
+  // start data of this thread's ring in a sub-block; the next sub-block's are
+  // fetched while the current one runs
+  struct RecIn {
+    int st;
+    double2 q;   // P at ls-2, ls-1
+    double q2;   // P at ls
+    double z, s;
+  };
+  auto fetch = [&](int rb, RecIn& in) {
+    const int i = i0 + rb + myr;
+    const int ic = min(i, nhr);
+    const long long so = seedoff[m] + (long long)it.blk * nr + (ic - i0);
+    const int st = seed_st[so];
+    in.st = (i <= nhr) ? st : 0x3fffffff;
+    in.q = seed_v[so];
+    in.q2 = seed_p[so];
+    in.z = geo[ic].z;
+    in.s = geo[ic].s;
+  };
+  RecIn nxt;
+  fetch(it.r0, nxt);
+
   // rings before it.r0 have no kept P_lm in this l-block
   for (int rb = it.r0; rb < nr; rb += RS) {
     // rings of the sub-block that exist; wave w generates (and MFMA group w
@@ -223,27 +249,22 @@ __global__ __launch_bounds__(256, SPIN ? FSHT_ANA_MINBLK2 : FSHT_ANA_MINBLK0) vo
     // ---- this thread's recurrence state (its ring of the sub-block)
     Rec rec;
     {
-      const int i = i0 + rb + myr;
-      const bool valid = i <= nhr;
-      const int ic = valid ? i : nhr;
-      const RingGeo g = geo[ic];
-      const long long so = seedoff[m] + (long long)it.blk * nr + (ic - i0);
-      const int st = valid ? seed_st[so] : 0x3fffffff;
-      const double2 q = seed_v[so];  // P at ls-2, ls-1
-      const double q2 = seed_p[so];  // P at ls
+      const int st = nxt.st;
       // class 1 enters one step later, at ls + 1, from (P_ls, P_ls-1)
       rec.ls = (st > lmax) ? 0x3fffffff : L0 + ((st - L0) & ~31) + par;
-      rec.sv = par ? make_double2(q2, q.y) : make_double2(q.y, q.x);
-      rec.x = g.z;
-      const double s2 = g.s * g.s;
-      rec.bw = g.z / s2;
+      rec.sv = par ? make_double2(nxt.q2, nxt.q.y) : make_double2(nxt.q.y, nxt.q.x);
+      rec.x = nxt.z;
+      const double s2 = nxt.s * nxt.s;
+      rec.bw = nxt.z / s2;
       rec.t2 = 2.0 / s2;
       rec.cw = (double)m / s2;
       rec.p1 = 0.0;
       rec.p2 = 0.0;
     }
     // ---- B fragments: the wave's 16 columns of the folded, rotated ring
-    // spectra for the sub-block's rings (one coalesced row read per ring)
+    // spectra for the sub-block's rings (one coalesced row read per ring).
+    // Rows past the last ring are clamped and read real values: their P rows
+    // are 0 (rings that do not exist never enter the recurrence).
     double bfr[NKB];
 #pragma unroll
     for (int kj = 0; kj < NKB; ++kj) {
@@ -251,17 +272,18 @@ __global__ __launch_bounds__(256, SPIN ? FSHT_ANA_MINBLK2 : FSHT_ANA_MINBLK0) vo
       const int krow = 4 * ks + q;
       const int f = SPIN ? (ks >= 32 ? 1 : 0) : 0;
       const int r = SPIN ? (krow % RS) : krow;
-      const int i = i0 + rb + r;
-      const bool valid = i <= nhr;
-      const int ic = valid ? i : nhr;
-      const int fold = par ^ (f ? sflip1 : sflip0);
-      const double v = F[fb + ((long long)fold * nr + (ic - i0)) * ncols + (f ? scol1 : scol0)];
-      bfr[kj] = valid ? (f ? ssign1 : ssign0) * v : 0.0;
+      const unsigned row = (unsigned)min(rb + r, nr - 1);
+      // 32-bit unsigned byte offset from a uniform base: scalar-base loads
+      const unsigned off = 8u * (row * (unsigned)ncols + (unsigned)li);
+      const double v = *reinterpret_cast<const double*>(reinterpret_cast<const char*>(f ? base1 : base0) + off);
+      bfr[kj] = SPIN ? (f ? ssign1 : ssign0) * v : v;
     }
+    if (rb + RS < nr) fetch(rb + RS, nxt);
 
     // spin 2 keeps the chunk loop rolled: unrolled, the compiler hoists the
     // per-l spin-2 constants of all 8 chunks out of the ring loop and spills
-#pragma unroll (SPIN ? 1 : 8)
+    constexpr int kUnrollT = SPIN ? 1 : 8;
+#pragma unroll kUnrollT
     for (int t = 0; t < 8; ++t) {
       if (t < nt) {
         const int lc = L0 + 32 * t + par;

@@ -436,6 +436,36 @@ def test_t2alm_iter_vs_oracle(torch_cuda, ns, lmax, nsim, niter):
     assert maxrel(sht.t2alm(maps), R.t2alm(maps, niter=niter)) < TOL
 
 
+def test_repeated_calls_bit_identical(torch_cuda):
+    """Every kernel reads only data written before it in the same workgroup
+    or launch: repeated transforms of the same device input agree bit for bit,
+    for spin 0 and spin 2, with high orders whose l-blocks hold a few l values
+    (lmax > 2*nside) and 2-tile column blocks (nsim = 48)."""
+    torch = torch_cuda
+    from fastsht_amd import SHT, empty_maps
+    ns, lmax, nsim = 256, 600, 48
+    print(f"case repeated calls nside={ns} lmax={lmax} nsim={nsim}", flush=True)
+    npix = 12 * ns * ns
+    rng = np.random.default_rng(23)
+    maps = empty_maps(npix, nsim, device=True)
+    maps.copy_(torch.from_numpy(np.asfortranarray(rng.standard_normal((npix, nsim)))))
+    sht = SHT(ns, lmax, nsim, niter=1)
+    ref = sht.t2alm(maps).cpu().numpy()
+    mref = sht.alm2t(sht.t2alm(maps)).cpu().numpy()
+    for _ in range(3):
+        assert np.array_equal(sht.t2alm(maps).cpu().numpy(), ref)
+    assert np.array_equal(sht.alm2t(sht.t2alm(maps)).cpu().numpy(), mref)
+    sp = SHT(ns, lmax, 16, niter=1, pol=True)
+    Q = empty_maps(npix, 16, device=True)
+    U = empty_maps(npix, 16, device=True)
+    Q.copy_(torch.from_numpy(np.asfortranarray(rng.standard_normal((npix, 16)))))
+    U.copy_(torch.from_numpy(np.asfortranarray(rng.standard_normal((npix, 16)))))
+    E0, B0 = (x.cpu().numpy() for x in sp.qu2eb(Q, U))
+    for _ in range(2):
+        E, B = sp.qu2eb(Q, U)
+        assert np.array_equal(E.cpu().numpy(), E0) and np.array_equal(B.cpu().numpy(), B0)
+
+
 def test_host_staging_matches_device(torch_cuda):
     """Host arrays move through pinned staging in chunks of whole maps (here
     3 map chunks and 2 alm chunks), each chunk's ring FFTs starting on its
