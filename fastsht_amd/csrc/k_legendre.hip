@@ -280,8 +280,11 @@ __global__ __launch_bounds__(256, SPIN ? FSHT_ANA_MINBLK2 : FSHT_ANA_MINBLK0) vo
     }
     if (rb + RS < nr) fetch(rb + RS, nxt);
 
-    // spin 2 keeps the chunk loop rolled: unrolled, the compiler hoists the
-    // per-l spin-2 constants of all 8 chunks out of the ring loop and spills
+    // spin 2 keeps the chunk loop rolled (unrolled, the compiler hoists the
+    // per-l spin-2 constants of all 8 chunks out of the ring loop and spills);
+    // its chunk t accumulates into D[0], and the 8 tiles rotate one slot per
+    // chunk, so they stay in registers (a runtime D[t] would live in scratch)
+    // and are back in order after each sub-block
     constexpr int kUnrollT = SPIN ? 1 : 8;
 #pragma unroll kUnrollT
     for (int t = 0; t < 8; ++t) {
@@ -304,7 +307,7 @@ __global__ __launch_bounds__(256, SPIN ? FSHT_ANA_MINBLK2 : FSHT_ANA_MINBLK0) vo
 #pragma unroll
               for (int kj = 16 * gj; kj < 16 * gj + 16; ++kj) {
                 const double a = Pt[4 * (16 * g + kj - 16 * gj) + q][li];
-                D[t] = mfma64(a, bfr[kj], D[t]);
+                D[SPIN ? 0 : t] = mfma64(a, bfr[kj], D[SPIN ? 0 : t]);
               }
               // software pipeline of the A reads: the LDS pair read for MFMAs
               // k+4, k+5 is issued before MFMAs k, k+1 (two pairs in flight)
@@ -319,6 +322,12 @@ __global__ __launch_bounds__(256, SPIN ? FSHT_ANA_MINBLK2 : FSHT_ANA_MINBLK0) vo
           }
         }
         __syncthreads();
+      }
+      if (SPIN) {  // rotate: the next chunk's accumulator moves to slot 0
+        const d4 d0 = D[0];
+#pragma unroll
+        for (int i = 0; i < 7; ++i) D[i] = D[i + 1];
+        D[7] = d0;
       }
     }
   }
