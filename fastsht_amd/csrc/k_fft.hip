@@ -210,8 +210,7 @@ __device__ inline double2 csub(double2 a, double2 b) { return make_double2(a.x -
 // fused, half the LDS round trips and barriers), with one radix-2 stage when
 // log2(M) is odd.  FftShape<M>::TPB threads (512 for the long transforms,
 // whose LDS footprint leaves one workgroup per CU); tw[k] = exp(-2 pi i k /
-// Mmax) for k < Mmax
-// (the full circle, so w^{3k} is a table entry).
+// Mmax) for k < Mmax.
 
 __device__ inline double2 mul_mi(double2 a) { return make_double2(a.y, -a.x); }  // a * (-i)
 __device__ inline double2 mul_pi(double2 a) { return make_double2(-a.y, a.x); }  // a * (+i)
@@ -261,9 +260,11 @@ __device__ inline void dif4(double2* buf, int len, int Mmax, const double2* __re
     const int t = threadIdx.x + FftShape<M>::TPB * r;
     const int pos = t & (q - 1);
     const bool ok = t < M / 4;
+    // one coalesced table load; w^2 and w^3 in registers (the table reads at
+    // 2*pos and 3*pos would touch two and three times the cache lines)
     w1[r] = ok ? tw[pos * stride] : make_double2(1.0, 0.0);
-    w2[r] = ok ? tw[2 * pos * stride] : make_double2(1.0, 0.0);
-    w3[r] = ok ? tw[3 * pos * stride] : make_double2(1.0, 0.0);
+    w2[r] = cmul(w1[r], w1[r]);
+    w3[r] = cmul(w2[r], w1[r]);
   }
 #pragma unroll
   for (int r = 0; r < NB; ++r) {
@@ -336,8 +337,8 @@ __device__ inline void dit4(double2* buf, int len, int Mmax, const double2* __re
     const int pos = t & (q - 1);
     const bool ok = t < M / 4;
     w1[r] = ok ? cconj(tw[pos * stride]) : make_double2(1.0, 0.0);
-    w2[r] = ok ? cconj(tw[2 * pos * stride]) : make_double2(1.0, 0.0);
-    w3[r] = ok ? cconj(tw[3 * pos * stride]) : make_double2(1.0, 0.0);
+    w2[r] = cmul(w1[r], w1[r]);
+    w3[r] = cmul(w2[r], w1[r]);
   }
 #pragma unroll
   for (int r = 0; r < NB; ++r) {
