@@ -392,7 +392,6 @@ template <int M>
 __device__ void ring_fft_half(double2* buf, int n, const double2* __restrict__ w, const double2* __restrict__ fl,
                               const double2* __restrict__ tw, int Mmax, const double* __restrict__ in,
                               double* __restrict__ out, int inverse) {
-  constexpr int lg = FftShape<M>::LG;
   const int L = n >> 1;
   const double rn = 2.0 / (double)n;
   if (!inverse) {

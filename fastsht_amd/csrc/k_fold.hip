@@ -153,7 +153,7 @@ __global__ __launch_bounds__(256) void k_gather(const RingGeo* __restrict__ geo,
   const long long odd = (long long)nr * ncols;
   const int cre = (comp * 2) * nsimp + sim, cim = cre + nsimp;
   const long long mrow = (long long)m * nnorth;
-  for (int i = i0 + blockIdx.y * kGatherRings + threadIdx.y; i <= nhr && i < i0 + (blockIdx.y + 1) * kGatherRings;
+  for (int i = i0 + blockIdx.y * kGatherRings + threadIdx.y; i <= nhr && i < i0 + ((int)blockIdx.y + 1) * kGatherRings;
        i += kFoldBlk.y) {
     const int sinfo = slot[mrow + i];
     const int rel = sinfo >> 2, cls = sinfo & 3;
@@ -209,7 +209,7 @@ __global__ __launch_bounds__(256) void k_gather_prefix(const RingGeo* __restrict
   const long long odd = (long long)nr * ncols;
   const int cre = (comp * 2) * nsimp + sim, cim = cre + nsimp;
   const long long mrow = (long long)m * nnorth;
-  for (int i = i0 + blockIdx.y * kGatherRings + threadIdx.y; i <= nhr && i < i0 + (blockIdx.y + 1) * kGatherRings;
+  for (int i = i0 + blockIdx.y * kGatherRings + threadIdx.y; i <= nhr && i < i0 + ((int)blockIdx.y + 1) * kGatherRings;
        i += kFoldBlk.y) {
     const RingGeo g = geo[i];
     const int n = g.nfft;

@@ -312,7 +312,7 @@ hipError_t launch_seeds(Handle* h) {
   e = hipGetLastError();
   if (e != hipSuccess) return e;
   e = hipStreamSynchronize(h->stream);
-  hipFree(d_dfac);
+  (void)hipFree(d_dfac);
   return e;
 }
 
@@ -634,14 +634,14 @@ __global__ __launch_bounds__(256) void k_first_ring(int nhr, const int* __restri
 // (Re)build the per-(m, ring) seed tables for the current h->i0.
 static hipError_t make_seeds(Handle* h) {
   fill_work(h);
-  hipFree(h->d_i0);
-  hipFree(h->d_seedoff);
-  hipFree(h->d_seed_st);
-  hipFree(h->d_seed_v);
-  hipFree(h->d_seed_p);
-  hipFree(h->d_synoff);
-  hipFree(h->d_syn_st);
-  hipFree(h->d_syn_v);
+  (void)hipFree(h->d_i0);
+  (void)hipFree(h->d_seedoff);
+  (void)hipFree(h->d_seed_st);
+  (void)hipFree(h->d_seed_v);
+  (void)hipFree(h->d_seed_p);
+  (void)hipFree(h->d_synoff);
+  (void)hipFree(h->d_syn_st);
+  (void)hipFree(h->d_syn_v);
   h->d_i0 = nullptr;
   h->d_seedoff = h->d_synoff = nullptr;
   h->d_seed_st = h->d_syn_st = nullptr;
