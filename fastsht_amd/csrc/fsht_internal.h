@@ -53,7 +53,7 @@ constexpr int kAnaRings2 = 128;   // analysis ring sub-block (spin 2, x2 functio
 constexpr int kAnaLB = 256;       // analysis l-block (l steps)
 constexpr int kAnaCols = 64;      // analysis columns per workgroup (4 waves x 16)
 constexpr int kSynRings = 256;    // synthesis rings per workgroup (4 waves x 64)
-constexpr int kSynCols = 64;      // synthesis columns per workgroup (spin 0; spin 2 uses 32)
+constexpr int kSynCols = 64;      // synthesis columns per wave (4 tiles of 16; spin 0 batches end with a 2-tile block)
 constexpr int kSegL = 256;        // synthesis: recurrence coefficients staged per wave (l values)
 // P_lm (and the spin-2 functions built from it) below 2^-100 is dropped:
 // the recurrence of a ring starts at its first l with |P_lm| >= 2^-100.  A

@@ -403,11 +403,12 @@ __device__ inline void wave_lds_sync() {
 #ifndef FSHT_SYN_SEG2
 #define FSHT_SYN_SEG2 64
 #endif
-// spin 2 asks for no minimum: the compiler then keeps the accumulators in
-// AGPRs and still fits two workgroups per CU (faster than the all-VGPR
-// allocation a minimum of 2 produces)
+// spin 2 (4 column tiles per wave, like spin 0) asks for two workgroups per
+// CU: the all-VGPR allocation spills a few registers but beats the
+// accumulators-in-AGPRs allocation, which fits only one workgroup per CU
+// (51 against 36 TFLOP/s; 2 column tiles per wave: 45)
 #ifndef FSHT_SYN_MINBLK2
-#define FSHT_SYN_MINBLK2 1
+#define FSHT_SYN_MINBLK2 2
 #endif
 template <int SPIN>
 struct SynShared {
@@ -631,8 +632,8 @@ __global__ __launch_bounds__(256, SPIN ? FSHT_SYN_MINBLK2 : FSHT_SYN_MINBLK0) vo
   using I0 = std::integral_constant<int, 0>;
   using I1 = std::integral_constant<int, 1>;
   if (SPIN) {
-    using N2 = std::integral_constant<int, 2>;
-    if (it.par) run(N2{}, I1{}); else run(N2{}, I0{});
+    using N4 = std::integral_constant<int, 4>;
+    if (it.par) run(N4{}, I1{}); else run(N4{}, I0{});
   } else {
     using N4 = std::integral_constant<int, 4>;
     using N2 = std::integral_constant<int, 2>;

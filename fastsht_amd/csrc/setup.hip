@@ -563,7 +563,7 @@ static hipError_t build_items(Handle* h) {
         // synthesis: one ring block against the kept l of m (first 16-column
         // tile t0); a wave runs the 32-l chunks from its rings' first kept l
         const int nrb = ceil_div(nr, kSynRings);
-        const int ntile = ncols / 16, tpi = sp ? 2 : 4;
+        const int ntile = ncols / 16, tpi = kSynCols / 16;
         for (int rb = 0; rb < nrb; ++rb) {
           double chunks = 0.0;
           for (int w = 0; w < kSynRings / 64; ++w) {
