@@ -197,6 +197,9 @@ hipError_t build_bluestein(Handle* h) {
 // ---------------------------------------------------------------------------
 // Device FFT helpers (one workgroup, data in LDS).
 // ---------------------------------------------------------------------------
+// LDS position of transform element i: one pad element per 16 spreads the
+// short-span butterflies over the banks
+__device__ inline int pz(int i) { return i + (i >> 4); }
 __device__ inline double2 cmul(double2 a, double2 b) {
   return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
@@ -241,9 +244,9 @@ __device__ inline void dif2(double2* buf, int len, int Mmax, const double2* __re
     if (t < M / 2) {
       const int pos = t & (half - 1);
       const int i = (t - pos) * 2 + pos;
-      const double2 u = buf[i], v = buf[i + half];
-      buf[i] = cadd(u, v);
-      buf[i + half] = cmul(csub(u, v), w[r]);
+      const double2 u = buf[pz(i)], v = buf[pz(i + half)];
+      buf[pz(i)] = cadd(u, v);
+      buf[pz(i + half)] = cmul(csub(u, v), w[r]);
     }
   }
   __syncthreads();
@@ -272,13 +275,13 @@ __device__ inline void dif4(double2* buf, int len, int Mmax, const double2* __re
     if (t < M / 4) {
       const int pos = t & (q - 1);
       const int i0 = (t - pos) * 4 + pos;
-      const double2 a0 = buf[i0], a1 = buf[i0 + q], a2 = buf[i0 + 2 * q], a3 = buf[i0 + 3 * q];
+      const double2 a0 = buf[pz(i0)], a1 = buf[pz(i0 + q)], a2 = buf[pz(i0 + 2 * q)], a3 = buf[pz(i0 + 3 * q)];
       const double2 s02 = cadd(a0, a2), d02 = csub(a0, a2);
       const double2 s13 = cadd(a1, a3), d13 = mul_mi(csub(a1, a3));
-      buf[i0] = cadd(s02, s13);
-      buf[i0 + q] = cmul(csub(s02, s13), w2[r]);
-      buf[i0 + 2 * q] = cmul(cadd(d02, d13), w1[r]);
-      buf[i0 + 3 * q] = cmul(csub(d02, d13), w3[r]);
+      buf[pz(i0)] = cadd(s02, s13);
+      buf[pz(i0 + q)] = cmul(csub(s02, s13), w2[r]);
+      buf[pz(i0 + 2 * q)] = cmul(cadd(d02, d13), w1[r]);
+      buf[pz(i0 + 3 * q)] = cmul(csub(d02, d13), w3[r]);
     }
   }
   __syncthreads();
@@ -316,10 +319,10 @@ __device__ inline void dit2(double2* buf, int len, int Mmax, const double2* __re
     if (t < M / 2) {
       const int pos = t & (half - 1);
       const int i = (t - pos) * 2 + pos;
-      const double2 u = buf[i];
-      const double2 v = cmul(buf[i + half], w[r]);
-      buf[i] = cadd(u, v);
-      buf[i + half] = csub(u, v);
+      const double2 u = buf[pz(i)];
+      const double2 v = cmul(buf[pz(i + half)], w[r]);
+      buf[pz(i)] = cadd(u, v);
+      buf[pz(i + half)] = csub(u, v);
     }
   }
   __syncthreads();
@@ -346,15 +349,15 @@ __device__ inline void dit4(double2* buf, int len, int Mmax, const double2* __re
     if (t < M / 4) {
       const int pos = t & (q - 1);
       const int i0 = (t - pos) * 4 + pos;
-      const double2 a0 = buf[i0], a1 = cmul(buf[i0 + q], w2[r]);
-      const double2 a2 = cmul(buf[i0 + 2 * q], w1[r]), a3 = cmul(buf[i0 + 3 * q], w3[r]);
+      const double2 a0 = buf[pz(i0)], a1 = cmul(buf[pz(i0 + q)], w2[r]);
+      const double2 a2 = cmul(buf[pz(i0 + 2 * q)], w1[r]), a3 = cmul(buf[pz(i0 + 3 * q)], w3[r]);
       // span len/2: (a0, a1) and (a2, a3); span len: pairs (0, 2) and (1, 3)
       const double2 b0 = cadd(a0, a1), b1 = csub(a0, a1);
       const double2 b2 = cadd(a2, a3), b3 = mul_pi(csub(a2, a3));
-      buf[i0] = cadd(b0, b2);
-      buf[i0 + q] = cadd(b1, b3);
-      buf[i0 + 2 * q] = csub(b0, b2);
-      buf[i0 + 3 * q] = csub(b1, b3);
+      buf[pz(i0)] = cadd(b0, b2);
+      buf[pz(i0 + q)] = cadd(b1, b3);
+      buf[pz(i0 + 2 * q)] = csub(b0, b2);
+      buf[pz(i0 + 3 * q)] = csub(b1, b3);
     }
   }
   __syncthreads();
@@ -396,16 +399,16 @@ __device__ void ring_fft_half(double2* buf, int n, const double2* __restrict__ w
   const double rn = 2.0 / (double)n;
   if (!inverse) {
     for (int j = threadIdx.x; j < M; j += blockDim.x)
-      buf[j] = (j < L) ? cmul(make_double2(in[2 * j], in[2 * j + 1]), w[j]) : make_double2(0.0, 0.0);
+      buf[pz(j)] = (j < L) ? cmul(make_double2(in[2 * j], in[2 * j + 1]), w[j]) : make_double2(0.0, 0.0);
     __syncthreads();
     lds_dif<M>(buf, Mmax, tw);
-    for (int r = threadIdx.x; r < M; r += blockDim.x) buf[r] = cmul(buf[r], fl[r]);
+    for (int r = threadIdx.x; r < M; r += blockDim.x) buf[pz(r)] = cmul(buf[pz(r)], fl[r]);
     __syncthreads();
     lds_dit_inv<M>(buf, Mmax, tw);
     for (int k = threadIdx.x; k <= L; k += blockDim.x) {
       const int k1 = (k == L) ? 0 : k, k2 = (k == 0) ? 0 : L - k;
-      const double2 Yk = cmul(buf[k1], w[k1]);
-      const double2 Yc = cconj(cmul(buf[k2], w[k2]));
+      const double2 Yk = cmul(buf[pz(k1)], w[k1]);
+      const double2 Yc = cconj(cmul(buf[pz(k2)], w[k2]));
       const double2 A = make_double2(0.5 * (Yk.x + Yc.x), 0.5 * (Yk.y + Yc.y));
       const double2 B = make_double2(0.5 * (Yk.y - Yc.y), -0.5 * (Yk.x - Yc.x));
       double s, c;
@@ -433,15 +436,15 @@ __device__ void ring_fft_half(double2* buf, int n, const double2* __restrict__ w
         const double2 Yp = cadd(cadd(Xk, Xc), mul_pi(D));
         Z = cmul(cconj(Yp), w[k]);
       }
-      buf[k] = Z;
+      buf[pz(k)] = Z;
     }
     __syncthreads();
     lds_dif<M>(buf, Mmax, tw);
-    for (int r = threadIdx.x; r < M; r += blockDim.x) buf[r] = cmul(buf[r], fl[r]);
+    for (int r = threadIdx.x; r < M; r += blockDim.x) buf[pz(r)] = cmul(buf[pz(r)], fl[r]);
     __syncthreads();
     lds_dit_inv<M>(buf, Mmax, tw);
     for (int j = threadIdx.x; j < L; j += blockDim.x) {
-      const double2 y = cconj(cmul(buf[j], w[j]));
+      const double2 y = cconj(cmul(buf[pz(j)], w[j]));
       out[2 * j] = y.x;
       out[2 * j + 1] = y.y;
     }
@@ -458,7 +461,7 @@ __global__ __launch_bounds__(FftShape<MAXM>::TPB) void k_ring_fft(const FftJob* 
                                                   const double* __restrict__ in0, const double* __restrict__ in1,
                                                   double* __restrict__ out0, double* __restrict__ out1, int inverse,
                                                   int map0) {
-  __shared__ double2 buf[MAXM];
+  __shared__ double2 buf[MAXM + MAXM / 16];
   const FftJob jb = jobs[job0 + blockIdx.y];
   const int mapi = map0 + blockIdx.x;
   const int comp = mapi / nsim, k = mapi - comp * nsim;
@@ -488,12 +491,12 @@ __global__ __launch_bounds__(FftShape<MAXM>::TPB) void k_ring_fft(const FftJob* 
         z.y = pair ? in[g.i1S + j] : 0.0;
         if (isblue) z = cmul(z, w[j]);
       }
-      buf[j] = z;
+      buf[pz(j)] = z;
     }
     __syncthreads();
     lds_dif<M>(buf, Mmax, tw);
     if (isblue) {
-      for (int r = threadIdx.x; r < M; r += blockDim.x) buf[r] = cmul(buf[r], fl[r]);
+      for (int r = threadIdx.x; r < M; r += blockDim.x) buf[pz(r)] = cmul(buf[pz(r)], fl[r]);
       __syncthreads();
       lds_dit_inv<M>(buf, Mmax, tw);
     }
@@ -502,11 +505,11 @@ __global__ __launch_bounds__(FftShape<MAXM>::TPB) void k_ring_fft(const FftJob* 
       const int nk = (n - kk) % n;
       double2 Zk, Zn;
       if (isblue) {
-        Zk = cmul(buf[kk], w[kk]);
-        Zn = cmul(buf[nk], w[nk]);
+        Zk = cmul(buf[pz(kk)], w[kk]);
+        Zn = cmul(buf[pz(nk)], w[nk]);
       } else {
-        Zk = buf[brev(kk, lg)];
-        Zn = buf[brev(nk, lg)];
+        Zk = buf[pz(brev(kk, lg))];
+        Zn = buf[pz(brev(nk, lg))];
       }
       const double2 XN = make_double2(0.5 * (Zk.x + Zn.x), 0.5 * (Zk.y - Zn.y));
       const double2 XS = make_double2(0.5 * (Zk.y + Zn.y), -0.5 * (Zk.x - Zn.x));
@@ -551,19 +554,19 @@ __global__ __launch_bounds__(FftShape<MAXM>::TPB) void k_ring_fft(const FftJob* 
         if (isblue) Z = cmul(cconj(Z), w[kk]);
       }
       if (isblue)
-        buf[kk] = Z;
+        buf[pz(kk)] = Z;
       else if (kk < M)
-        buf[brev(kk, lg)] = Z;
+        buf[pz(brev(kk, lg))] = Z;
     }
     __syncthreads();
     if (isblue) {
       lds_dif<M>(buf, Mmax, tw);
-      for (int r = threadIdx.x; r < M; r += blockDim.x) buf[r] = cmul(buf[r], fl[r]);
+      for (int r = threadIdx.x; r < M; r += blockDim.x) buf[pz(r)] = cmul(buf[pz(r)], fl[r]);
       __syncthreads();
     }
     lds_dit_inv<M>(buf, Mmax, tw);
     for (int j = threadIdx.x; j < n; j += blockDim.x) {
-      double2 z = buf[j];
+      double2 z = buf[pz(j)];
       if (isblue) z = cconj(cmul(z, w[j]));
       out[g.i1N + j] = z.x;
       if (pair) out[g.i1S + j] = z.y;
