@@ -33,7 +33,10 @@ def test_create_rejects_bad_arguments():
     from fastsht_amd import _lib
     lib = _lib.load()
     h = ctypes.c_void_p()
-    for args in [(3, 8, 1, 0, 0), (4096, 100, 1, 0, 0), (8, 33, 1, 0, 0), (8, 8, 0, 0, 0), (8, 8, 1, -1, 0)]:
+    # (2048, 0, 40000): one order's folded spectrum would pass 4 GiB (the
+    # Legendre kernels address it with 32-bit byte offsets)
+    for args in [(3, 8, 1, 0, 0), (4096, 100, 1, 0, 0), (8, 33, 1, 0, 0), (8, 8, 0, 0, 0), (8, 8, 1, -1, 0),
+                 (2048, 0, 40000, 0, 0)]:
         rc = lib.fsht_create(*args, ctypes.byref(h))
         assert rc == _lib.FSHT_ERR_ARG, args
         assert h.value is None
