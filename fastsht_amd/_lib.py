@@ -18,12 +18,14 @@ FSHT_ERR_NOMEM = -2
 FSHT_ERR_HIP = -3
 FSHT_ERR_STATE = -4
 FSHT_HOST = 0
+FSHT_PREC_FP64 = 0
+FSHT_PREC_MIXED = 1
 FSHT_DEVICE = 1
 
 # every symbol declared in include/fastsht.h
 EXPORTS = (
     "fsht_version", "fsht_last_error", "fsht_create", "fsht_destroy", "fsht_get_stream",
-    "fsht_set_stream", "fsht_synchronize", "fsht_info", "fsht_ring_tab", "fsht_t2alm", "fsht_t2alm_iter",
+    "fsht_set_stream", "fsht_synchronize", "fsht_info", "fsht_ring_tab", "fsht_set_precision", "fsht_t2alm", "fsht_t2alm_iter",
     "fsht_get_alm", "fsht_alm2t", "fsht_qu2eb", "fsht_qu2e", "fsht_qu2b", "fsht_eb2qu", "fsht_fix_eb",
     "fsht_profile", "fsht_profile_read", "fsht_norm_alm", "fsht_filt_alm", "fsht_convert_alm_healpy", "fsht_convert_alm_healpy_inv",
     "fsht_alm2cl", "fsht_cl2alm", "fsht_host_alloc", "fsht_host_free",
@@ -63,6 +65,7 @@ def load():
     lib.fsht_synchronize.argtypes = [H]
     lib.fsht_info.argtypes = [H, ctypes.POINTER(ctypes.c_longlong), i]
     lib.fsht_ring_tab.argtypes = [H, ip]
+    lib.fsht_set_precision.argtypes = [H, i]
     lib.fsht_t2alm.argtypes = [H, dp, dp, i, i]
     lib.fsht_t2alm_iter.argtypes = [H, dp, dp, i, i]
     lib.fsht_get_alm.argtypes = [H, dp, i]

@@ -13,7 +13,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libfastsht.so")
-SOURCES = ["setup.hip", "k_fft.hip", "k_legendre.hip", "k_fold.hip", "k_alm.hip", "k_maps.hip", "api.hip"]
+SOURCES = ["setup.hip", "k_fft.hip", "k_legendre.hip", "k_mixed.hip", "k_fold.hip", "k_alm.hip", "k_maps.hip",
+           "api.hip"]
 ARCH = os.environ.get("FSHT_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 CFLAGS = [

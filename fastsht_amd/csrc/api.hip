@@ -48,6 +48,7 @@ static int fail(Handle* h, int code, const std::string& msg) {
   } while (0)
 
 static void free_all(Handle* h) {
+  fsht::mixed_free(h);
   void* ptrs[] = {h->d_geo, h->d_i0, h->d_ab, h->d_spin, h->d_aboff, h->d_tw, h->d_slot, h->d_seedoff,
                   h->d_seed_st, h->d_seed_v, h->d_seed_p, h->d_synoff, h->d_syn_st, h->d_syn_v, h->d_goff, h->d_ana_items[0][0], h->d_ana_items[0][1],
                   h->d_ana_items[1][0], h->d_ana_items[1][1], h->d_syn_items[0], h->d_syn_items[1], h->d_fft_jobs, h->d_blue, h->d_chirp, h->d_filt,
@@ -64,7 +65,7 @@ static bool is_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
 
 extern "C" {
 
-const char* fsht_version(void) { return "fastsht_amd 0.1 (gfx950, FP64 MFMA)"; }
+const char* fsht_version(void) { return "fastsht_amd 0.2 (gfx950, FP64 MFMA; mixed: int8-slice MFMA)"; }
 
 const char* fsht_last_error(const fsht_handle* h) {
   if (h && !h->h.err.empty()) return h->h.err.c_str();
@@ -172,14 +173,48 @@ int fsht_set_stream(fsht_handle* H, void* stream) {
 int fsht_info(const fsht_handle* H, long long* out, int n) {
   if (!H || !out) return fail(nullptr, FSHT_ERR_ARG, "NULL argument");
   const Handle* h = &H->h;
-  long long v[8] = {h->nside, h->lmax, h->nsim, h->niter, h->pol, h->npix, h->nsimp, h->G_size};
-  for (int i = 0; i < n && i < 8; ++i) out[i] = v[i];
+  long long v[10] = {h->nside, h->lmax, h->nsim,      h->niter,     h->pol,
+                     h->npix,  h->nsimp, h->G_size, h->precision, h->mix_bytes};
+  for (int i = 0; i < n && i < 10; ++i) out[i] = v[i];
   return FSHT_OK;
 }
 
 int fsht_ring_tab(const fsht_handle* H, int* out) {
   if (!H || !out) return fail(nullptr, FSHT_ERR_ARG, "NULL argument");
   std::memcpy(out, H->h.i0_ref.data(), H->h.i0_ref.size() * sizeof(int));
+  return FSHT_OK;
+}
+
+int fsht_set_precision(fsht_handle* H, int precision) {
+  if (!H) return fail(nullptr, FSHT_ERR_ARG, "NULL handle");
+  Handle* h = &H->h;
+  if (precision != FSHT_PREC_FP64 && precision != FSHT_PREC_MIXED)
+    return fail(h, FSHT_ERR_ARG, "precision must be FSHT_PREC_FP64 or FSHT_PREC_MIXED");
+  if (precision == FSHT_PREC_MIXED && !h->mix_ready) {
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    const hipError_t e = fsht::mixed_setup(h);
+    if (e == hipErrorInvalidValue) return fail(h, FSHT_ERR_ARG, "mixed precision needs lmax - m <= 4607");
+    if (e == hipErrorOutOfMemory)
+      return fail(h, FSHT_ERR_NOMEM, "mixed precision: the int8 Legendre tables and sliced operands do not fit");
+    if (e != hipSuccess) return fail(h, FSHT_ERR_HIP, std::string("mixed precision set-up: ") + hipGetErrorString(e));
+  }
+  // slice counts: 4 for passes damped at least twice and 5 otherwise while
+  // lmax <= 2.5 nside (Jacobi contracts by ~0.1 per iteration there); stronger
+  // aliasing contracts less, so 5 everywhere up to lmax = 3.5 nside and 6
+  // beyond; 6 everywhere for nside <= 16
+  if (h->nside <= 16) {  // coarse grids contract least; their passes are cheap
+    h->mix_k_lo = h->mix_k_hi = 6;
+  } else if (2 * h->lmax <= 5 * h->nside) {
+    h->mix_k_lo = 4;
+    h->mix_k_hi = 5;
+  } else if (2 * h->lmax <= 7 * h->nside) {
+    h->mix_k_lo = h->mix_k_hi = 5;
+  } else {
+    h->mix_k_lo = h->mix_k_hi = 6;
+  }
+  h->mix_damp_min = 2;
+  h->mix_k_out = 6;
+  h->precision = precision;
   return FSHT_OK;
 }
 
@@ -409,7 +444,38 @@ static int check_common(fsht_handle* H, int loc) {
 // method 0 (Jacobi, t2alm_iter_old): every order's residual has all synthesised
 // orders subtracted.  method 1 (immediate update, t2alm_iter): order m's
 // residual has only the aliases m' <= m of its slot subtracted.
+// Mixed precision (temperature): the same passes with int8-slice Legendre
+// contractions.  Slices per pass follow how strongly the later Jacobi
+// iterations damp that pass's error: the last analysis and the last two
+// residual syntheses carry more.
+static int analysis_iter_mixed(Handle* h, double* a0, int niter, int method) {
+  // error of analysis k is damped by the niter - k later iterations, that of
+  // synthesis k (the residual of iteration k) by niter - k - 1
+  auto kslices = [&](int damping) { return damping >= h->mix_damp_min ? h->mix_k_lo : h->mix_k_hi; };
+  HIPCHK(h, fsht::launch_fexp_reset(h));
+  HIPCHK(h, timed(h, fsht::KC_GATHER, [&] { return fsht::launch_gather(h, h->ws_specT, h->ws_G, h->d_FQe); }));
+  int ka = kslices(niter);
+  HIPCHK(h, timed(h, fsht::KC_SLICE, [&] { return fsht::launch_quant_F(h, h->ws_G, ka); }));
+  HIPCHK(h, timed(h, fsht::KC_ANALYSIS, [&] { return fsht::launch_analysis_q(h, a0, 0.0, ka); }));
+  if (niter > 0 && !h->ws_F)
+    HIPCHK(h, hipMalloc((void**)&h->ws_F, (size_t)(h->G_size / (h->pol ? 1 : 2)) * sizeof(double)));
+  for (int it = 0; it < niter; ++it) {
+    const int ks = kslices(niter - it - 1);
+    HIPCHK(h, timed(h, fsht::KC_SLICE, [&] { return fsht::launch_quant_A(h, a0, ks); }));
+    HIPCHK(h, timed(h, fsht::KC_SYNTHESIS, [&] { return fsht::launch_synthesis_q(h, h->ws_G, ks); }));
+    HIPCHK(h, fsht::launch_fexp_reset(h));
+    HIPCHK(h, timed(h, fsht::KC_FOLD, [&] {
+      return fsht::launch_gather_prefix(h, h->ws_G, h->ws_specT, h->ws_F, method == 0 ? 1 : 0, h->d_FQe);
+    }));
+    ka = kslices(niter - it - 1);
+    HIPCHK(h, timed(h, fsht::KC_SLICE, [&] { return fsht::launch_quant_F(h, h->ws_F, ka); }));
+    HIPCHK(h, timed(h, fsht::KC_ANALYSIS, [&] { return fsht::launch_analysis_q(h, a0, 1.0, ka); }));
+  }
+  return FSHT_OK;
+}
+
 static int analysis_iter(Handle* h, double* a0, double* a1, int niter, int want0, int want1, int method = 0) {
+  if (h->precision == FSHT_PREC_MIXED && h->spin == 0) return analysis_iter_mixed(h, a0, niter, method);
   HIPCHK(h, timed(h, fsht::KC_GATHER, [&] { return fsht::launch_gather(h, h->ws_specT, h->ws_G); }));
   HIPCHK(h, timed(h, fsht::KC_ANALYSIS, [&] { return fsht::launch_analysis(h, h->ws_G, a0, a1, 0.0, want0, want1); }));
   if (niter > 0 && !h->ws_F)
@@ -485,7 +551,12 @@ int fsht_alm2t(fsht_handle* H, const double* alms, double* maps, int loc) {
     a = h->ws_alm[0];
     h->have_dev_alm = 0;
   }
-  HIPCHK(h, timed(h, fsht::KC_SYNTHESIS, [&] { return fsht::launch_synthesis(h, a, nullptr, h->ws_G); }));
+  if (h->precision == FSHT_PREC_MIXED) {
+    HIPCHK(h, timed(h, fsht::KC_SLICE, [&] { return fsht::launch_quant_A(h, a, h->mix_k_out); }));
+    HIPCHK(h, timed(h, fsht::KC_SYNTHESIS, [&] { return fsht::launch_synthesis_q(h, h->ws_G, h->mix_k_out); }));
+  } else {
+    HIPCHK(h, timed(h, fsht::KC_SYNTHESIS, [&] { return fsht::launch_synthesis(h, a, nullptr, h->ws_G); }));
+  }
   HIPCHK(h, timed(h, fsht::KC_FOLD, [&] { return fsht::launch_fold(h, h->ws_G, nullptr, h->ws_specT, 0); }));
   if ((rc = inverse_spectra(h, maps, nullptr, loc))) return rc;
   if (!loc) HIPCHK(h, hipStreamSynchronize(h->stream));

@@ -61,6 +61,40 @@ constexpr int kSegL = 256;        // synthesis: recurrence coefficients staged p
 // the largest spin-2 factor, 2/sin^2 at the polar ring), far below FP64 rounding.
 constexpr int kKeepExp = -99;     // seeds: a value p*2^e is kept iff frexp-exponent(p)+e >= this
 
+// ---------------------------------------------------------------------------
+// Mixed precision: the Legendre contractions as sums of int8 x int8 products
+// of fixed-point digits (v_mfma_i32_32x32x32_i8, exact int32 accumulation).
+// A value v of a row or column with exponent e is x = v * 2^-e in
+// [-127/128, 127/128], written as K balanced base-128 digits
+// x = sum_i d_i 2^-7(i+1) (|d_0| <= 127, |d_i| <= 64 after).  Products keep
+// the digit pairs i + j < K.  Every row of the left operand and every column
+// of the right one has one exponent over the whole contraction, so the int32
+// digit sums are converted to FP64 once, after the last step.  Operand tiles
+// are 1 KB blocks in the MFMA operand order: lane L = idx + 32 g holds the 16
+// bytes of contraction index 16 g .. 16 g + 15 (idx: l or ring row of the
+// tile, or column).  Per (m, parity class) the layouts are
+//   TA  (P for the analysis):  [l-tile][ring step][slice]  lanes: l, 16 rings each
+//   TS  (P for the synthesis): [ring tile][l step][slice]  lanes: ring, 16 l each
+//   FQ  (analysis operand):    [col tile][ring step][slice]
+//   AQ  (synthesis operand):   [col tile][l step][slice]
+// with tiles and steps of 32 (ring rows from i0(m); l as class values
+// j = (l - m - class) / 2).  Exponents: TA per l and TS per ring row (32 per
+// tile, stored in accumulator-row order), FQ and AQ per column.
+// ---------------------------------------------------------------------------
+constexpr int kMixK = 6;           // slices stored in the P tables
+constexpr int kMixZeroE = -30000;  // exponent of an all-zero row or column
+// Exponent e of a row or column with max |v| = mx: |v| * 2^-e <= 127/128.
+__device__ inline int mix_exp(double mx) { return mx > 0.0 ? ilogb(mx * (128.0 / 127.0)) + 1 : kMixZeroE; }
+struct MixOff {
+  long long ta, ts;    // byte offsets into the P tables
+  long long fq, aq;    // byte offsets into the sliced operands (room for kMixK slices)
+  long long tae, tse;  // offsets (ints) of the P-table exponents
+  long long fqe, aqe;  // offsets (ints) of the operand exponents
+  int nrs;             // ring steps (= ring tiles) of 32 rows: ceil(rows / 32)
+  int nlt;             // l steps (= l tiles) of 32 class values
+  int fa, fs;          // offsets into the first-nonzero-step tables
+};
+
 struct Handle;
 
 // Kernel launchers (k_*.hip). All enqueue on h->stream.
@@ -71,15 +105,29 @@ hipError_t launch_ring_fft(Handle* h, const double* const* maps_comp, double* sp
                            double* const* out_comp, int map0 = 0, int nmap = 0);
 hipError_t launch_transpose_to_T(Handle* h, const double* spec, double* specT);
 hipError_t launch_transpose_from_T(Handle* h, const double* specT, double* spec);
-hipError_t launch_gather(Handle* h, const double* specT, double* F);
+// fexp (mixed precision, else NULL): per (m, class, column) the max over rings
+// of the exponent of |F| is folded in with atomicMax (reset to kMixZeroE first)
+hipError_t launch_gather(Handle* h, const double* specT, double* F, int* fexp = nullptr);
 // residual analysis operand straight from the synthesis buffer G: aliases m' <= m
 // (full = 0, immediate update) or all aliases (full = 1, Jacobi)
-hipError_t launch_gather_prefix(Handle* h, const double* G, const double* specT, double* F, int full);
+hipError_t launch_gather_prefix(Handle* h, const double* G, const double* specT, double* F, int full,
+                                int* fexp = nullptr);
 hipError_t launch_analysis(Handle* h, const double* F, double* alm0, double* alm1, double beta,
                            int want0, int want1);
 hipError_t launch_synthesis(Handle* h, const double* alm0, const double* alm1, double* G);
 hipError_t launch_fold(Handle* h, const double* G, const double* specT_t, double* specT_out,
                        int residual);
+// XCD-ordered upload of a work list (setup.hip): items of one m stay on one XCD,
+// m-groups spread over the XCDs by the estimated cost of each item
+hipError_t upload_items(const std::vector<LegItem>& v, const std::vector<double>& cost, LegItem** d, int* n);
+// mixed precision (k_mixed.hip)
+hipError_t mixed_setup(Handle* h);        // P tables + work lists (allocates; set-up only)
+void mixed_free(Handle* h);
+hipError_t launch_fexp_reset(Handle* h);                          // FQe <- kMixZeroE
+hipError_t launch_quant_F(Handle* h, const double* F, int K);      // m-major FP64 operand -> FQ (FQe set)
+hipError_t launch_quant_A(Handle* h, const double* alm, int K);    // packed alm -> AQ
+hipError_t launch_analysis_q(Handle* h, double* alm, double beta, int K);
+hipError_t launch_synthesis_q(Handle* h, double* G, int K);
 // map-domain helpers (k_maps.hip)
 hipError_t launch_mask_maps(double* maps, const double* mask, long long npix, int nmaps, hipStream_t s);
 hipError_t launch_linfit_resi(const double* x, const double* y, const double* mask, double* resi, double* coef,
@@ -163,6 +211,27 @@ struct Handle {
   // current call's spin (0 or 2)
   int spin = 0;
 
+  // mixed precision (temperature transforms): int8-slice P tables, quantized
+  // operands, work lists and the slice counts per pass
+  int precision = 0;  // 0: FP64, 1: mixed
+  int mix_ready = 0;
+  std::vector<MixOff> mix_off;  // per (m, class)
+  MixOff* d_mix_off = nullptr;
+  signed char *d_TA = nullptr, *d_TS = nullptr, *d_FQ = nullptr, *d_AQ = nullptr;
+  int *d_TAe = nullptr, *d_TSe = nullptr, *d_FQe = nullptr, *d_AQe = nullptr;
+  int *d_firstA = nullptr, *d_firstS = nullptr;
+  // slice-GEMM work lists for workgroups of 2 and of 4 row tiles
+  LegItem* d_anaq_items[2] = {nullptr, nullptr};
+  LegItem* d_synq_items[2] = {nullptr, nullptr};
+  int n_anaq_items[2] = {0, 0}, n_synq_items[2] = {0, 0};
+  long long mix_bytes = 0;  // device memory held by the mixed-precision tables and operands
+  int mix_nrb_max = 0;  // 256-row blocks of the longest order (grid of the operand slicer)
+  // slices per pass: a pass whose error the remaining Jacobi iterations of
+  // t2alm damp at least mix_damp_min times uses mix_k_lo, the others mix_k_hi
+  // (set from lmax / nside: the stronger the aliasing, the weaker the damping);
+  // alm2t uses mix_k_out
+  int mix_k_lo = 4, mix_k_hi = 5, mix_damp_min = 2, mix_k_out = 6;
+
   // optional per-kernel-class timing with HIP events on `stream`
   int profiling = 0;
   struct Timed {
@@ -174,6 +243,16 @@ struct Handle {
 };
 
 // kernel classes reported by fsht_profile_read
-enum KernelClass { KC_FFT_FWD = 0, KC_FFT_INV, KC_TRANSPOSE, KC_ANALYSIS, KC_SYNTHESIS, KC_FOLD, KC_GATHER, KC_COUNT };
+enum KernelClass {
+  KC_FFT_FWD = 0,
+  KC_FFT_INV,
+  KC_TRANSPOSE,
+  KC_ANALYSIS,
+  KC_SYNTHESIS,
+  KC_FOLD,
+  KC_GATHER,
+  KC_SLICE,  // mixed precision: operand slicing (k_quant_F / k_quant_A)
+  KC_COUNT
+};
 
 }  // namespace fsht

@@ -126,6 +126,34 @@ __global__ __launch_bounds__(256) void k_fold(const RingGeo* __restrict__ geo, c
   }
 }
 
+// Mixed precision: the per-(m, class, column) exponent of max |F| over the
+// rings, for the int8 slicing of the analysis operand (k_mixed.hip).  A
+// thread tracks its columns' maxima (Re/Im of both parity classes), the
+// block reduces them over its rings and folds them into fexp (index
+// (2 m + class) * ncols + col) with atomicMax.
+struct ColMax {
+  double mx[4] = {0.0, 0.0, 0.0, 0.0};  // class 0 Re, class 0 Im, class 1 Re, class 1 Im
+  __device__ void add(double v0, double v1, double v2, double v3) {
+    mx[0] = fmax(mx[0], fabs(v0));
+    mx[1] = fmax(mx[1], fabs(v1));
+    mx[2] = fmax(mx[2], fabs(v2));
+    mx[3] = fmax(mx[3], fabs(v3));
+  }
+  __device__ void publish(int* fexp, int m, int ncols, int cre, int cim, bool colok) {
+    __shared__ int red[4][kFoldBlk.y][kFoldBlk.x];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) red[k][threadIdx.y][threadIdx.x] = mix_exp(mx[k]);
+    __syncthreads();
+    if (threadIdx.y < 4 && colok) {
+      const int k = threadIdx.y;
+      int e = red[k][0][threadIdx.x];
+#pragma unroll
+      for (int y = 1; y < kFoldBlk.y; ++y) e = max(e, red[k][y][threadIdx.x]);
+      if (e != kMixZeroE) atomicMax(&fexp[(long long)(2 * m + (k >> 1)) * ncols + ((k & 1) ? cim : cre)], e);
+    }
+  }
+};
+
 // ---------------------------------------------------------------------------
 // Ring spectra -> m-major analysis operand (src/fft_mapping.f90:19-68,
 // fft2mat_fast, for all m at once).  For order m and kept north ring i:
@@ -140,12 +168,13 @@ __global__ __launch_bounds__(256) void k_gather(const RingGeo* __restrict__ geo,
                                                 const double2* __restrict__ tw, const int* __restrict__ slot,
                                                 const double* __restrict__ specT, double* __restrict__ F,
                                                 const long long* __restrict__ goff, int nhr, int nnorth,
-                                                int nsimp) {
+                                                int nsimp, int* __restrict__ fexp) {
   const int m = blockIdx.x;
   const int i0 = i0tab[m];
   const int ldT = NCOMP * nsimp;
-  const int col = blockIdx.z * kFoldBlk.x + threadIdx.x;  // spectrum column comp*nsimp + sim
-  if (col >= ldT) return;
+  const int colx = blockIdx.z * kFoldBlk.x + threadIdx.x;  // spectrum column comp*nsimp + sim
+  const bool colok = colx < ldT;
+  const int col = colok ? colx : ldT - 1;
   const int comp = col / nsimp, sim = col - comp * nsimp;
   const int nr = nhr + 1 - i0;
   const int ncols = 2 * NCOMP * nsimp;
@@ -153,8 +182,9 @@ __global__ __launch_bounds__(256) void k_gather(const RingGeo* __restrict__ geo,
   const long long odd = (long long)nr * ncols;
   const int cre = (comp * 2) * nsimp + sim, cim = cre + nsimp;
   const long long mrow = (long long)m * nnorth;
-  for (int i = i0 + blockIdx.y * kGatherRings + threadIdx.y; i <= nhr && i < i0 + ((int)blockIdx.y + 1) * kGatherRings;
-       i += kFoldBlk.y) {
+  ColMax cm;
+  for (int i = i0 + blockIdx.y * kGatherRings + threadIdx.y;
+       colok && i <= nhr && i < i0 + ((int)blockIdx.y + 1) * kGatherRings; i += kFoldBlk.y) {
     const int sinfo = slot[mrow + i];
     const int rel = sinfo >> 2, cls = sinfo & 3;
     const double sg = cls == 1 ? 1.0 : (cls == 2 ? -1.0 : 0.0);
@@ -169,11 +199,14 @@ __global__ __launch_bounds__(256) void k_gather(const RingGeo* __restrict__ geo,
     const double Si = sg * yi * cs.x - yr * cs.y;
     const double fs = (g.i1S != g.i1N) ? 1.0 : 0.0;
     const long long row = fb + (long long)(i - i0) * ncols;
-    F[row + cre] = fma(fs, Sr, Nr);
-    F[row + cim] = fma(fs, Si, Ni);
-    F[row + odd + cre] = fma(-fs, Sr, Nr);
-    F[row + odd + cim] = fma(-fs, Si, Ni);
+    const double v0 = fma(fs, Sr, Nr), v1 = fma(fs, Si, Ni), v2 = fma(-fs, Sr, Nr), v3 = fma(-fs, Si, Ni);
+    F[row + cre] = v0;
+    F[row + cim] = v1;
+    F[row + odd + cre] = v2;
+    F[row + odd + cim] = v3;
+    cm.add(v0, v1, v2, v3);
   }
+  if (fexp) cm.publish(fexp, m, ncols, cre, cim, colok);
 }
 
 // ---------------------------------------------------------------------------
@@ -196,12 +229,14 @@ __global__ __launch_bounds__(256) void k_gather_prefix(const RingGeo* __restrict
                                                        const double2* __restrict__ tw, const int* __restrict__ slot,
                                                        const double* __restrict__ G, const double* __restrict__ specT,
                                                        double* __restrict__ F, const long long* __restrict__ goff,
-                                                       int nhr, int nnorth, int nsimp, int lmax, int full) {
+                                                       int nhr, int nnorth, int nsimp, int lmax, int full,
+                                                       int* __restrict__ fexp) {
   const int m = blockIdx.x;
   const int i0 = i0tab[m];
   const int ldT = NCOMP * nsimp;
-  const int col = blockIdx.z * kFoldBlk.x + threadIdx.x;
-  if (col >= ldT) return;
+  const int colx = blockIdx.z * kFoldBlk.x + threadIdx.x;
+  const bool colok = colx < ldT;
+  const int col = colok ? colx : ldT - 1;
   const int comp = col / nsimp, sim = col - comp * nsimp;
   const int nr = nhr + 1 - i0;
   const int ncols = 2 * NCOMP * nsimp;
@@ -209,8 +244,9 @@ __global__ __launch_bounds__(256) void k_gather_prefix(const RingGeo* __restrict
   const long long odd = (long long)nr * ncols;
   const int cre = (comp * 2) * nsimp + sim, cim = cre + nsimp;
   const long long mrow = (long long)m * nnorth;
-  for (int i = i0 + blockIdx.y * kGatherRings + threadIdx.y; i <= nhr && i < i0 + ((int)blockIdx.y + 1) * kGatherRings;
-       i += kFoldBlk.y) {
+  ColMax cm;
+  for (int i = i0 + blockIdx.y * kGatherRings + threadIdx.y;
+       colok && i <= nhr && i < i0 + ((int)blockIdx.y + 1) * kGatherRings; i += kFoldBlk.y) {
     const RingGeo g = geo[i];
     const int n = g.nfft;
     const int pos = m % n;
@@ -285,38 +321,41 @@ __global__ __launch_bounds__(256) void k_gather_prefix(const RingGeo* __restrict
     const double Si = sg * yi * cs.x - yr * cs.y;
     const double fs = pair ? 1.0 : 0.0;
     const long long row = fb + (long long)(i - i0) * ncols;
-    F[row + cre] = fma(fs, Sr, Nr);
-    F[row + cim] = fma(fs, Si, Ni);
-    F[row + odd + cre] = fma(-fs, Sr, Nr);
-    F[row + odd + cim] = fma(-fs, Si, Ni);
+    const double v0 = fma(fs, Sr, Nr), v1 = fma(fs, Si, Ni), v2 = fma(-fs, Sr, Nr), v3 = fma(-fs, Si, Ni);
+    F[row + cre] = v0;
+    F[row + cim] = v1;
+    F[row + odd + cre] = v2;
+    F[row + odd + cim] = v3;
+    cm.add(v0, v1, v2, v3);
   }
+  if (fexp) cm.publish(fexp, m, ncols, cre, cim, colok);
 }
 
-hipError_t launch_gather_prefix(Handle* h, const double* G, const double* specT, double* F, int full) {
+hipError_t launch_gather_prefix(Handle* h, const double* G, const double* specT, double* F, int full, int* fexp) {
   const int ncomp = h->spin ? 2 : 1;
   dim3 block(kFoldBlk.x, kFoldBlk.y);
   dim3 grid(h->lmax + 1, (h->nnorth + kGatherRings - 1) / kGatherRings,
             (ncomp * h->nsimp + kFoldBlk.x - 1) / kFoldBlk.x);
   if (ncomp == 2)
     hipLaunchKernelGGL(k_gather_prefix<2>, grid, block, 0, h->stream, h->d_geo, h->d_i0, h->d_tw, h->d_slot, G,
-                       specT, F, h->d_goff, h->nhr, h->nnorth, h->nsimp, h->lmax, full);
+                       specT, F, h->d_goff, h->nhr, h->nnorth, h->nsimp, h->lmax, full, fexp);
   else
     hipLaunchKernelGGL(k_gather_prefix<1>, grid, block, 0, h->stream, h->d_geo, h->d_i0, h->d_tw, h->d_slot, G,
-                       specT, F, h->d_goff, h->nhr, h->nnorth, h->nsimp, h->lmax, full);
+                       specT, F, h->d_goff, h->nhr, h->nnorth, h->nsimp, h->lmax, full, fexp);
   return hipGetLastError();
 }
 
-hipError_t launch_gather(Handle* h, const double* specT, double* F) {
+hipError_t launch_gather(Handle* h, const double* specT, double* F, int* fexp) {
   const int ncomp = h->spin ? 2 : 1;
   dim3 block(kFoldBlk.x, kFoldBlk.y);
   dim3 grid(h->lmax + 1, (h->nnorth + kGatherRings - 1) / kGatherRings,
             (ncomp * h->nsimp + kFoldBlk.x - 1) / kFoldBlk.x);
   if (ncomp == 2)
     hipLaunchKernelGGL(k_gather<2>, grid, block, 0, h->stream, h->d_geo, h->d_i0, h->d_tw, h->d_slot, specT, F,
-                       h->d_goff, h->nhr, h->nnorth, h->nsimp);
+                       h->d_goff, h->nhr, h->nnorth, h->nsimp, fexp);
   else
     hipLaunchKernelGGL(k_gather<1>, grid, block, 0, h->stream, h->d_geo, h->d_i0, h->d_tw, h->d_slot, specT, F,
-                       h->d_goff, h->nhr, h->nnorth, h->nsimp);
+                       h->d_goff, h->nhr, h->nnorth, h->nsimp, fexp);
   return hipGetLastError();
 }
 

@@ -516,8 +516,7 @@ static std::vector<LegItem> xcd_order(const std::vector<LegItem>& v, const std::
   return out;
 }
 
-static hipError_t upload_items(const std::vector<LegItem>& v, const std::vector<double>& cost, LegItem** d,
-                               int* n) {
+hipError_t upload_items(const std::vector<LegItem>& v, const std::vector<double>& cost, LegItem** d, int* n) {
   std::vector<LegItem> o = xcd_order(v, cost);
   *n = (int)o.size();
   if (o.empty()) return hipSuccess;

@@ -80,7 +80,13 @@ class SHT:
     signature compatibility; workspace is sized by the library itself.
     """
 
-    def __init__(self, nside, lmax, nsim, niter=0, pol=False, all_buff=False):
+    def __init__(self, nside, lmax, nsim, niter=0, pol=False, all_buff=False, *, precision="fp64"):
+        """precision="mixed" runs the temperature transforms (t2alm, alm2t) on
+        int8-slice MFMA Legendre contractions refined in FP64 (about 1e-10 of
+        the FP64 result); spin-2 transforms stay FP64."""
+        if precision not in ("fp64", "mixed"):
+            raise ValueError("precision must be 'fp64' or 'mixed'")
+        self.precision = precision
         self.nside = int(nside)
         self.lmax = int(lmax)
         self.nsim = int(nsim)
@@ -96,6 +102,8 @@ class SHT:
         h = ctypes.c_void_p()
         _lib.check(lib.fsht_create(self.nside, self.lmax, self.nsim, self.niter, self.pol, ctypes.byref(h)))
         self._h = h
+        if precision == "mixed":
+            self._call(lib.fsht_set_precision, _lib.FSHT_PREC_MIXED)
 
     def __del__(self):
         h = getattr(self, "_h", None)

@@ -62,8 +62,24 @@ int fsht_get_stream(fsht_handle* h, void** stream);
 int fsht_set_stream(fsht_handle* h, void* stream);
 int fsht_synchronize(fsht_handle* h);
 
-/* out[0..n) = {nside, lmax, nsim, niter, pol, npix, nsim padded to 16, m-major buffer size}. */
+/* out[0..n) = {nside, lmax, nsim, niter, pol, npix, nsim padded to 16, m-major
+ * buffer size, precision, device bytes held for mixed precision}. */
 int fsht_info(const fsht_handle* h, long long* out, int n);
+
+#define FSHT_PREC_FP64 0  /* every Legendre contraction in FP64 (the default) */
+#define FSHT_PREC_MIXED 1 /* temperature transforms on int8-slice MFMA, refined in FP64 */
+
+/* Precision of the temperature transforms (fsht_t2alm, fsht_t2alm_iter,
+ * fsht_alm2t) on this handle.  FSHT_PREC_MIXED evaluates each Legendre
+ * contraction as exact int8 products of 7-bit fixed-point digits (4 to 6 per
+ * operand, v_mfma_i32_32x32x32_i8) and keeps the ring FFTs, the Jacobi
+ * residual and the alm accumulation in FP64; t2alm(niter) then matches the
+ * FP64 result to about 1e-10 (max |diff| / max |alm|) and alm2t to about
+ * 1e-11.  No counterpart in the reference, whose GEMMs (src/sht_core.f90:30-168)
+ * are FP64 only.  The first switch to FSHT_PREC_MIXED builds the sliced
+ * Legendre tables (about 12 bytes per (m, l, ring) term plus sliced-operand
+ * workspace) and returns when they are ready; spin-2 transforms stay FP64. */
+int fsht_set_precision(fsht_handle* h, int precision);
 
 /* The ring-skip table used (first kept north ring per m, m = 0..lmax), as read
  * from SHT_ring_skip_table/NNNN.bin at SHT.py:31-32. */
@@ -133,8 +149,8 @@ int fsht_host_free(void* p);
  * fsht_profile(h, 1) starts recording (and clears earlier records); classes:
  * 0 ring FFT forward, 1 ring FFT inverse, 2 transposes, 3 Legendre analysis,
  * 4 Legendre synthesis, 5 alias fold / Jacobi residual, 6 spectrum gather
- * into the m-major analysis operand.
- * fsht_profile_read fills out[2c] = launches, out[2c+1] = total ms (c < 7). */
+ * into the m-major analysis operand, 7 operand slicing (mixed precision).
+ * fsht_profile_read fills out[2c] = launches, out[2c+1] = total ms (c < 8). */
 int fsht_profile(fsht_handle* h, int enable);
 int fsht_profile_read(fsht_handle* h, double* out, int n);
 

@@ -58,7 +58,10 @@ def test_sht_api_surface():
 
     from fastsht_amd import SHT
     sig = inspect.signature(SHT.__init__)
-    assert list(sig.parameters)[1:] == ["nside", "lmax", "nsim", "niter", "pol", "all_buff"]
+    pos = [n for n, p in sig.parameters.items() if p.kind != inspect.Parameter.KEYWORD_ONLY]
+    assert pos[1:] == ["nside", "lmax", "nsim", "niter", "pol", "all_buff"]
+    # extras are keyword-only, so every positional call of the reference still binds the same way
+    assert [n for n, p in sig.parameters.items() if p.kind == inspect.Parameter.KEYWORD_ONLY] == ["precision"]
     assert sig.parameters["niter"].default == 0 and sig.parameters["pol"].default is False
     for name in ["set_maps", "set_alms", "t2alm", "alm2t", "qu2eb", "qu2e", "qu2b", "eb2qu", "fix_eb",
                  "convert_alm_healpy", "get_alms"]:

@@ -487,3 +487,173 @@ def test_t2alm_iter_differs_from_jacobi_only_through_aliases():
     e_old = np.abs(R.t2alm(maps, niter=5) - alm).max()
     e_new = np.abs(R.t2alm_iter(maps, niter=5) - alm).max()
     assert e_old < 1e-3 * e_new
+
+
+# ---------------------------------------------------------------------------
+# spin-2 E/B convention against a brute-force spin-weighted direct sum
+# ---------------------------------------------------------------------------
+def _wigner_d_hp(j, mp_, m, beta):
+    """Wigner d^j_{mp,m}(beta) from the explicit sum, in 40-digit arithmetic
+    (the alternating sum cancels to ~1e-8 in double precision at j ~ 16)."""
+    import mpmath as mpm
+    with mpm.workdps(40):
+        f = mpm.factorial
+        b = mpm.mpf(float(beta))
+        c, s_ = mpm.cos(b / 2), mpm.sin(b / 2)
+        tot = mpm.mpf(0)
+        for s in range(0, 2 * j + 1):
+            a1, a2, a3, a4 = j + m - s, s, mp_ - m + s, j - mp_ - s
+            if min(a1, a2, a3, a4) < 0:
+                continue
+            tot += ((-1) ** (mp_ - m + s) * mpm.sqrt(f(j + mp_) * f(j - mp_) * f(j + m) * f(j - m))
+                    / (f(a1) * f(a2) * f(a3) * f(a4)) * c ** (2 * j + m - mp_ - 2 * s) * s_ ** (mp_ - m + 2 * s))
+        return float(tot)
+
+
+def _spin_lambda(s, l, m, thetas):
+    """sY_lm(theta, 0) = sqrt((2l+1)/4pi) d^l_{m,-s}(theta) (Goldberg et al. 1967 up to
+    the (-1)^m that makes 0Y_lm the Condon-Shortley Y_lm of scipy and healpy);
+    0 for l < |s|."""
+    if l < abs(s) or abs(m) > l:
+        return np.zeros(len(thetas))
+    return np.array([math.sqrt((2 * l + 1) / (4 * math.pi)) * _wigner_d_hp(l, m, -s, t) for t in thetas])
+
+
+def _spin_tables(ns, lmax):
+    """Per ring: theta, and lambda_{+-2}(l, m) for -lmax <= m <= lmax."""
+    R = O.OracleSHT(ns, lmax, 1)
+    th_ring = np.arctan2(R.s, R.z)
+    th_ring = np.concatenate([th_ring, np.pi - th_ring[: 2 * ns - 1][::-1]])
+    lam = {}
+    for s in (2, -2):
+        for l in range(lmax + 1):
+            for m in range(-l, l + 1):
+                lam[s, l, m] = _spin_lambda(s, l, m, th_ring)
+    return R, lam
+
+
+def _ring_of_pixel(ns):
+    i1, nfft = O.ring_info(ns)
+    return np.repeat(np.arange(len(nfft)), nfft)
+
+
+def brute_qu2eb(Q, U, ns, lmax, lam):
+    """HEALPix convention: +-2 a_lm = 4pi/npix sum_p (Q +- iU) conj(+-2 Y_lm(p)),
+    a_E = -(2a + -2a)/2, a_B = i (2a - -2a)/2; reference square packing."""
+    th, ph = O.pixel_angles(ns)
+    ring = _ring_of_pixel(ns)
+    npix, nsim = Q.shape
+    E = np.zeros((nsim, lmax + 1, lmax + 1), order="F")
+    B = np.zeros_like(E)
+    P = Q + 1j * U
+    M = Q - 1j * U
+    for m in range(lmax + 1):
+        ph_m = np.exp(-1j * m * ph)
+        for l in range(max(m, 2), lmax + 1):
+            y2 = lam[2, l, m][ring] * ph_m
+            ym2 = lam[-2, l, m][ring] * ph_m
+            a2 = (4 * np.pi / npix) * (y2[None, :] @ P)[0]
+            am2 = (4 * np.pi / npix) * (ym2[None, :] @ M)[0]
+            aE = -(a2 + am2) / 2
+            aB = 1j * (a2 - am2) / 2
+            E[:, l, m] = aE.real
+            B[:, l, m] = aB.real
+            if m > 0:
+                E[:, l - m, lmax + 1 - m] = aE.imag
+                B[:, l - m, lmax + 1 - m] = aB.imag
+    return E, B
+
+
+def brute_eb2qu(E, B, ns, lam):
+    """Q + iU = sum_{l, |m| <= l} 2a_lm 2Y_lm with 2a = -(a_E + i a_B) and
+    a_{l,-m} = (-1)^m conj(a_lm) for both fields."""
+    th, ph = O.pixel_angles(ns)
+    ring = _ring_of_pixel(ns)
+    nsim, L1, _ = E.shape
+    lmax = L1 - 1
+    out = np.zeros((12 * ns * ns, nsim), dtype=complex)
+
+    def cplx(A, l, m):
+        if m == 0:
+            return A[:, l, 0].astype(complex)
+        return A[:, l, m] + 1j * A[:, l - m, lmax + 1 - m]
+
+    for l in range(2, lmax + 1):
+        for m in range(0, l + 1):
+            aE, aB = cplx(E, l, m), cplx(B, l, m)
+            for mm, eE, eB in ((m, aE, aB), (-m, (-1) ** m * np.conj(aE), (-1) ** m * np.conj(aB))):
+                if m == 0 and mm == -m and eE is not aE:
+                    continue
+                y2 = lam[2, l, mm][ring] * np.exp(1j * mm * ph)
+                out += np.outer(y2, -(eE + 1j * eB))
+                if m == 0:
+                    break
+    return out.real, out.imag
+
+
+@pytest.fixture(scope="module")
+def spin_tables():
+    return {(ns, lmax): _spin_tables(ns, lmax) for ns, lmax in [(2, 4), (4, 8)]}
+
+
+@pytest.mark.parametrize("ns,lmax", [(2, 4), (4, 8)])
+def test_qu2eb_eb2qu_vs_spin_weighted_direct_sum(spin_tables, ns, lmax):
+    """Pins the spin-2 sign convention (the alpha/swap table of
+    sht_data_init.f90:606-632 and the type sequences of sht_main.f90:409-430,
+    1102-1124) to the HEALPix definitions built from Wigner d alone, so a
+    flipped B or a flipped U <-> E term fails even though it would survive
+    every round trip."""
+    R, lam = spin_tables[(ns, lmax)]
+    rng = np.random.default_rng(ns * 11 + lmax)
+    nsim = 2
+    Q = np.asfortranarray(rng.standard_normal((12 * ns * ns, nsim)))
+    U = np.asfortranarray(rng.standard_normal((12 * ns * ns, nsim)))
+    S = O.OracleSHT(ns, lmax, nsim)
+    E, B = S.qu2eb(Q, U)
+    Eb, Bb = brute_qu2eb(Q, U, ns, lmax, lam)
+    assert maxrel(E, Eb) < 1e-12 and maxrel(B, Bb) < 1e-12
+    Ea, Ba = O.random_alm(lmax, nsim, seed=5), O.random_alm(lmax, nsim, seed=6)
+    Qs, Us = S.eb2qu(Ea, Ba)
+    Qb, Ub = brute_eb2qu(Ea, Ba, ns, lam)
+    assert maxrel(Qs, Qb) < 1e-12 and maxrel(Us, Ub) < 1e-12
+
+
+def test_spin2_planted_sign_mistakes_are_caught(spin_tables):
+    """Two plausible slips that round trips cannot see: the sign of every F2
+    term flipped in both directions (the U <-> E and Q <-> B couplings, i.e.
+    the other handedness), and B negated in both directions.  The direct sum
+    catches both; the band-limited round trip does not."""
+    ns, lmax, nsim = 4, 8, 2
+    R, lam = spin_tables[(ns, lmax)]
+    rng = np.random.default_rng(9)
+    Q = np.asfortranarray(rng.standard_normal((12 * ns * ns, nsim)))
+    U = np.asfortranarray(rng.standard_normal((12 * ns * ns, nsim)))
+    Eb, Bb = brute_qu2eb(Q, U, ns, lmax, lam)
+
+    class FlipF2(O.OracleSHT):
+        def plm(self, m, pol=False):
+            P = super().plm(m, pol)
+            return (P[0], P[1], -P[2]) if pol else P
+
+    class FlipB(O.OracleSHT):
+        def qu2eb(self, Q, U, niter=0):
+            E, B = super().qu2eb(Q, U, niter)
+            return E, -B
+
+        def eb2qu(self, E, B):
+            return super().eb2qu(E, -B)
+
+    for cls in (FlipF2, FlipB):
+        S = cls(ns, lmax, nsim)
+        E, B = S.qu2eb(Q, U)
+        assert max(maxrel(E, Eb), maxrel(B, Bb)) > 1e-2, cls.__name__
+        # a band-limited round trip does not notice the slip
+        Ea, Ba = O.random_alm(lmax, nsim, seed=1), O.random_alm(lmax, nsim, seed=2)
+        for a in (Ea, Ba):
+            a[:, :2, :] = 0.0
+            a[:, :, lmax:] = 0.0
+            a[:, 0, lmax] = 0.0
+        Ea[:, :, 0][:, :2] = 0.0
+        Qr, Ur = S.eb2qu(Ea, Ba)
+        E3, B3 = S.qu2eb(Qr, Ur, niter=3)
+        assert maxrel(E3, Ea) < 1e-2 and maxrel(B3, Ba) < 1e-2, cls.__name__
