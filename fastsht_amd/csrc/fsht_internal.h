@@ -220,10 +220,13 @@ struct Handle {
   signed char *d_TA = nullptr, *d_TS = nullptr, *d_FQ = nullptr, *d_AQ = nullptr;
   int *d_TAe = nullptr, *d_TSe = nullptr, *d_FQe = nullptr, *d_AQe = nullptr;
   int *d_firstA = nullptr, *d_firstS = nullptr;
-  // slice-GEMM work lists for workgroups of 2 and of 4 row tiles
-  LegItem* d_anaq_items[2] = {nullptr, nullptr};
-  LegItem* d_synq_items[2] = {nullptr, nullptr};
-  int n_anaq_items[2] = {0, 0}, n_synq_items[2] = {0, 0};
+  // slice-GEMM work lists: [0] full blocks of 4 column tiles (4 row tiles
+  // each), [1] the remaining column tiles one by one (8 row tiles each), [2]
+  // the remaining column tiles as one padded block of 4 (for six slices,
+  // whose 8-row-tile stages would not fit in LDS)
+  LegItem* d_anaq_items[3] = {nullptr, nullptr, nullptr};
+  LegItem* d_synq_items[3] = {nullptr, nullptr, nullptr};
+  int n_anaq_items[3] = {0, 0, 0}, n_synq_items[3] = {0, 0, 0};
   long long mix_bytes = 0;  // device memory held by the mixed-precision tables and operands
   int mix_nrb_max = 0;  // 256-row blocks of the longest order (grid of the operand slicer)
   // slices per pass: a pass whose error the remaining Jacobi iterations of

@@ -312,20 +312,21 @@ __global__ __launch_bounds__(256) void k_quant_A(const MixOff* __restrict__ mo, 
 }
 
 // ---------------------------------------------------------------------------
-// The slice GEMM.  Workgroup = (m, class, NA row tiles, 4 column tiles) with
-// 2 NA waves; wave w owns row tile w/2 and column tiles 2(w%2), 2(w%2)+1.
-// Per 32-wide step of the contraction the workgroup needs (NA + 4) K blocks
-// of 1 KB (K slices of its A and B tiles); the waves copy them global -> LDS
-// with direct-to-LDS loads (global_load_lds_dwordx4: 1 KB per wave
-// instruction, lane order kept) into a ring of NS stages, NS - 1 steps ahead
-// of the MFMAs.  Each step every wave waits for its own loads of the step
-// (vmcnt), one barrier publishes them, the stage that every wave finished
-// with is refilled, and the wave reads its operands from LDS and issues
-// K(K+1)/2 digit products per B tile into one int32 accumulator per digit sum.
+// The slice GEMM.  Workgroup = 8 waves (two per SIMD) on (m, class, NA row
+// tiles, NB column tiles): NB = 4 with NA = 4 (wave w owns row tile w/2 and
+// column tiles 2(w%2), 2(w%2)+1), or NB = 1 with NA = 8 (wave w owns row tile
+// w and the column tile) for the last column tiles of a batch whose tile
+// count is not a multiple of 4.  Per 32-wide step of the contraction the
+// workgroup needs (NA + NB) K blocks of 1 KB (K slices of its A and B tiles);
+// the waves copy them global -> LDS with direct-to-LDS loads
+// (global_load_lds_dwordx4: 1 KB per wave instruction, lane order kept) into a
+// ring of NS stages, NS - 1 steps ahead of the MFMAs.  Each step every wave
+// waits for its own loads of the step (vmcnt), one barrier publishes them,
+// the stage that every wave finished with is refilled, and the wave reads its
+// operands from LDS and issues K(K+1)/2 digit products per B tile into one
+// int32 accumulator per digit sum.
 //   SYN = 0 (analysis): A = TA (rows l), B = FQ (columns), out = alm
 //   SYN = 1 (synthesis): A = TS (rows ring), B = AQ (columns), out = G
-// NA = 4 (8 waves, two per SIMD) serves K <= 5; six slices do not leave the
-// registers for two waves per SIMD and run with NA = 2.
 // ---------------------------------------------------------------------------
 constexpr int kLdsBudget = 147456;  // bytes of LDS the slice GEMM may use
 
@@ -340,18 +341,6 @@ __device__ inline void wait_vm() {
 __device__ inline void raw_barrier() { asm volatile("s_barrier" ::: "memory"); }
 
 template <int K>
-__device__ inline void slice_products(const v4i (&a)[K], const v4i (&b0)[K], const v4i (&b1)[K], v16i (&D0)[K],
-                                      v16i (&D1)[K]) {
-#pragma unroll
-  for (int i = 0; i < K; ++i)
-#pragma unroll
-    for (int j = 0; j < K - i; ++j) {
-      D0[i + j] = mfma_i8(a[i], b0[j], D0[i + j]);
-      D1[i + j] = mfma_i8(a[i], b1[j], D1[i + j]);
-    }
-}
-
-template <int K>
 __device__ inline double combine(const v16i (&D)[K], int r) {
   double v = 0.0;
 #pragma unroll
@@ -359,8 +348,8 @@ __device__ inline double combine(const v16i (&D)[K], int r) {
   return v;
 }
 
-template <int K, int SYN, int NA>
-__global__ __launch_bounds__(128 * NA, 1) void k_slice_gemm(const LegItem* __restrict__ items,
+template <int K, int SYN, int NB>
+__global__ __launch_bounds__(512, 1) void k_slice_gemm(const LegItem* __restrict__ items,
                                                        const MixOff* __restrict__ mo,
                                                        const signed char* __restrict__ TP,
                                                        const int* __restrict__ TPe,
@@ -369,8 +358,11 @@ __global__ __launch_bounds__(128 * NA, 1) void k_slice_gemm(const LegItem* __res
                                                        const long long* __restrict__ goff,
                                                        const int* __restrict__ i0tab, int nsim, int nsimp, int lmax,
                                                        int nhr, int nct, double pa, double beta) {
-  constexpr int NW = 2 * NA;                           // waves
-  constexpr int NBLK = (NA + 4) * K;                    // 1 KB blocks per step
+  static_assert(NB == 4 || NB == 1, "column tiles per workgroup");
+  constexpr int NW = 8;                                 // waves
+  constexpr int NA = NB == 4 ? 4 : 8;                   // row tiles
+  constexpr int TPW = NB == 4 ? 2 : 1;                  // column tiles per wave
+  constexpr int NBLK = (NA + NB) * K;                   // 1 KB blocks per step
   constexpr int PER = (NBLK + NW - 1) / NW;             // loads per wave per step (the last may repeat)
   constexpr int NS = kLdsBudget / (NBLK * 1024) < 4 ? kLdsBudget / (NBLK * 1024) : 4;  // stages
   static_assert(NS >= 3, "pipeline needs three stages");
@@ -399,15 +391,17 @@ __global__ __launch_bounds__(128 * NA, 1) void k_slice_gemm(const LegItem* __res
         g = baseA + (((long long)tile * nsteps + s) * kMixK + b % K) * 1024;
       } else {
         const int bb = b - NA * K;
-        const int tile = min(it.colblk * 4 + bb / K, nct - 1);
+        const int tile = min(it.colblk * NB + bb / K, nct - 1);
         g = baseB + (((long long)tile * nsteps + s) * K + bb % K) * 1024;
       }
       __builtin_amdgcn_global_load_lds((const void*)(g + L * 16), (lds_ptr)(stage + b * 1024), 16, 0, 0);
     }
   };
 
-  const int ta = it.blk * NA + (w >> 1);
-  const int tb0 = it.colblk * 4 + (w & 1) * 2;
+  const int wa = NB == 4 ? (w >> 1) : w;         // this wave's row tile within the workgroup
+  const int wb = NB == 4 ? (w & 1) * 2 : 0;      // its first column tile within the workgroup
+  const int ta = it.blk * NA + wa;
+  const int tb0 = it.colblk * NB + wb;
   const bool act = ta < ntA && tb0 < nct;
   v16i D0[K], D1[K];
 #pragma unroll
@@ -426,15 +420,24 @@ __global__ __launch_bounds__(128 * NA, 1) void k_slice_gemm(const LegItem* __res
     raw_barrier();
     if (s + NS - 1 < nsteps) issue(s + NS - 1);  // the stage read in step s - 1
     if (act) {
+      // A slices stay in registers for the step; B slices are read one at a
+      // time (slice j of the B tiles meets A slices 0 .. K-1-j)
       const signed char* sb = st[s % NS];
-      v4i a[K], b0[K], b1[K];
+      const signed char* pb = sb + (NA * K + wb * K) * 1024 + L * 16;
+      v4i a[K];
 #pragma unroll
-      for (int sl = 0; sl < K; ++sl) {
-        a[sl] = *reinterpret_cast<const v4i*>(sb + ((w >> 1) * K + sl) * 1024 + L * 16);
-        b0[sl] = *reinterpret_cast<const v4i*>(sb + (NA * K + ((w & 1) * 2) * K + sl) * 1024 + L * 16);
-        b1[sl] = *reinterpret_cast<const v4i*>(sb + (NA * K + ((w & 1) * 2 + 1) * K + sl) * 1024 + L * 16);
+      for (int sl = 0; sl < K; ++sl) a[sl] = *reinterpret_cast<const v4i*>(sb + (wa * K + sl) * 1024 + L * 16);
+#pragma unroll
+      for (int j = 0; j < K; ++j) {
+        const v4i b0 = *reinterpret_cast<const v4i*>(pb + j * 1024);
+        v4i b1 = b0;
+        if (TPW == 2) b1 = *reinterpret_cast<const v4i*>(pb + (K + j) * 1024);
+#pragma unroll
+        for (int i = 0; i < K - j; ++i) {
+          D0[i + j] = mfma_i8(a[i], b0, D0[i + j]);
+          if (TPW == 2) D1[i + j] = mfma_i8(a[i], b1, D1[i + j]);
+        }
       }
-      slice_products<K>(a, b0, b1, D0, D1);
     }
   }
   if (!act) return;
@@ -443,7 +446,7 @@ __global__ __launch_bounds__(128 * NA, 1) void k_slice_gemm(const LegItem* __res
   const int h = L >> 5;
   const int* pe = TPe + (SYN ? o.tse : o.tae) + (long long)ta * 32 + 16 * h;  // accumulator order
 #pragma unroll
-  for (int t = 0; t < 2; ++t) {
+  for (int t = 0; t < TPW; ++t) {
     const int tb = tb0 + t;
     if (tb >= nct) break;
     const int col = tb * 32 + (L & 31);
@@ -480,19 +483,20 @@ __global__ __launch_bounds__(128 * NA, 1) void k_slice_gemm(const LegItem* __res
 static int cdiv(long long a, long long b) { return (int)((a + b - 1) / b); }
 
 void mixed_free(Handle* h) {
-  void* ptrs[] = {h->d_mix_off, h->d_TA,     h->d_TS,     h->d_FQ,           h->d_AQ,
-                  h->d_TAe,     h->d_TSe,    h->d_FQe,    h->d_AQe,          h->d_firstA,
-                  h->d_firstS,  h->d_anaq_items[0], h->d_anaq_items[1], h->d_synq_items[0], h->d_synq_items[1]};
+  void* ptrs[] = {h->d_mix_off, h->d_TA, h->d_TS, h->d_FQ, h->d_AQ, h->d_TAe, h->d_TSe, h->d_FQe, h->d_AQe,
+                  h->d_firstA,  h->d_firstS};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
+  for (int k = 0; k < 3; ++k) {
+    if (h->d_anaq_items[k]) (void)hipFree(h->d_anaq_items[k]);
+    if (h->d_synq_items[k]) (void)hipFree(h->d_synq_items[k]);
+    h->d_anaq_items[k] = h->d_synq_items[k] = nullptr;
+    h->n_anaq_items[k] = h->n_synq_items[k] = 0;
+  }
   h->d_mix_off = nullptr;
   h->d_TA = h->d_TS = h->d_FQ = h->d_AQ = nullptr;
   h->d_TAe = h->d_TSe = h->d_FQe = h->d_AQe = nullptr;
   h->d_firstA = h->d_firstS = nullptr;
-  for (int k = 0; k < 2; ++k) {
-    h->d_anaq_items[k] = h->d_synq_items[k] = nullptr;
-    h->n_anaq_items[k] = h->n_synq_items[k] = 0;
-  }
   h->mix_ready = 0;
   h->mix_bytes = 0;
 }
@@ -593,31 +597,39 @@ hipError_t mixed_setup(Handle* h) {
     mixed_free(h);
     return e;
   }
-  // work lists (workgroups of NA = 2 and 4 row tiles): every (m, class, row
-  // tiles, column block) writes its outputs, starting at the first step
-  // with a nonzero P digit of its row tiles
-  const int ncb = cdiv(nct, 4);
-  for (int k = 0; k < 2; ++k) {
-    const int NA = k ? 4 : 2;
+  // work lists: every (m, class, row tiles, column tiles) writes its
+  // outputs, starting at the first step with a nonzero P digit of its row tiles
+  const int nfull = nct / 4, nrem = nct % 4;
+  for (int k = 0; k < 3; ++k) {
+    if (k > 0 && nrem == 0) break;
+    const int NA = k == 1 ? 8 : 4;
     std::vector<LegItem> ana, syn;
     std::vector<double> ana_cost, syn_cost;
+    // column blocks of this list (colblk counts blocks of NB tiles) and their tile counts
+    std::vector<int> cbs, cnt;
+    if (k == 0)
+      for (int cb = 0; cb < nfull; ++cb) cbs.push_back(cb), cnt.push_back(4);
+    else if (k == 1)
+      for (int r = 0; r < nrem; ++r) cbs.push_back(4 * nfull + r), cnt.push_back(1);
+    else
+      cbs.push_back(nfull), cnt.push_back(4);
     for (int m = 0; m <= lmax; ++m)
       for (int c = 0; c < 2; ++c) {
         const MixOff& o = h->mix_off[2 * m + c];
         for (int b = 0; NA * b < o.nlt; ++b) {
           int r0 = o.nrs;
           for (int t = NA * b; t < std::min(o.nlt, NA * b + NA); ++t) r0 = std::min(r0, firstA[o.fa + t]);
-          for (int cb = 0; cb < ncb; ++cb) {
-            ana.push_back(LegItem{m, c, b, cb, r0});
-            ana_cost.push_back(1.0 + (double)(o.nrs - r0) * std::min(4, nct - 4 * cb));
+          for (size_t q = 0; q < cbs.size(); ++q) {
+            ana.push_back(LegItem{m, c, b, cbs[q], r0});
+            ana_cost.push_back(1.0 + (double)(o.nrs - r0) * NA * cnt[q]);
           }
         }
         for (int b = 0; NA * b < o.nrs; ++b) {
           int l0 = o.nlt;
           for (int t = NA * b; t < std::min(o.nrs, NA * b + NA); ++t) l0 = std::min(l0, firstS[o.fs + t]);
-          for (int cb = 0; cb < ncb; ++cb) {
-            syn.push_back(LegItem{m, c, b, cb, l0});
-            syn_cost.push_back(1.0 + (double)(o.nlt - l0) * std::min(4, nct - 4 * cb));
+          for (size_t q = 0; q < cbs.size(); ++q) {
+            syn.push_back(LegItem{m, c, b, cbs[q], l0});
+            syn_cost.push_back(1.0 + (double)(o.nlt - l0) * NA * cnt[q]);
           }
         }
       }
@@ -669,26 +681,40 @@ hipError_t launch_quant_A(Handle* h, const double* alm, int K) {
 
 template <int SYN>
 static hipError_t launch_gemm(Handle* h, double* out, double beta, int K) {
-  const int k = K <= 5 ? 1 : 0;  // NA = 4 for K <= 5, NA = 2 for six slices
-  const int n = SYN ? h->n_synq_items[k] : h->n_anaq_items[k];
-  if (n == 0) return hipSuccess;
   const int nct = h->nsimp / 16;
-  const LegItem* items = SYN ? h->d_synq_items[k] : h->d_anaq_items[k];
   const signed char* TP = SYN ? h->d_TS : h->d_TA;
   const int* TPe = SYN ? h->d_TSe : h->d_TAe;
   const signed char* OQ = SYN ? h->d_AQ : h->d_FQ;
   const int* OQe = SYN ? h->d_AQe : h->d_FQe;
-#define FSHT_GQ(KV, NAV)                                                                                           \
-  hipLaunchKernelGGL((k_slice_gemm<KV, SYN, NAV>), dim3(n), dim3(128 * NAV), 0, h->stream, items, h->d_mix_off, TP, \
-                     TPe, OQ, OQe, out, h->d_goff, h->d_i0, h->nsim, h->nsimp, h->lmax, h->nhr, nct, h->pa, beta)
-  switch (K) {
-    case 4: FSHT_GQ(4, 4); break;
-    case 5: FSHT_GQ(5, 4); break;
-    case 6: FSHT_GQ(6, 2); break;
-    default: return hipErrorInvalidValue;
-  }
+  // full column blocks, then the remaining tiles (one by one; six slices: one padded block)
+  const int lists[2] = {0, K <= 5 ? 1 : 2};
+  for (int q = 0; q < 2; ++q) {
+    const int k = lists[q];
+    const int n = SYN ? h->n_synq_items[k] : h->n_anaq_items[k];
+    if (n == 0) continue;
+    const LegItem* items = SYN ? h->d_synq_items[k] : h->d_anaq_items[k];
+#define FSHT_GQ(KV, NBV)                                                                                          \
+  hipLaunchKernelGGL((k_slice_gemm<KV, SYN, NBV>), dim3(n), dim3(512), 0, h->stream, items, h->d_mix_off, TP, TPe, \
+                     OQ, OQe, out, h->d_goff, h->d_i0, h->nsim, h->nsimp, h->lmax, h->nhr, nct, h->pa, beta)
+    if (k == 1) {
+      switch (K) {
+        case 4: FSHT_GQ(4, 1); break;
+        case 5: FSHT_GQ(5, 1); break;
+        default: return hipErrorInvalidValue;
+      }
+    } else {
+      switch (K) {
+        case 4: FSHT_GQ(4, 4); break;
+        case 5: FSHT_GQ(5, 4); break;
+        case 6: FSHT_GQ(6, 4); break;
+        default: return hipErrorInvalidValue;
+      }
+    }
 #undef FSHT_GQ
-  return hipGetLastError();
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
 }
 
 hipError_t launch_analysis_q(Handle* h, double* alm, double beta, int K) { return launch_gemm<0>(h, alm, beta, K); }
