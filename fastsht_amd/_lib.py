@@ -27,6 +27,7 @@ EXPORTS = (
     "fsht_version", "fsht_last_error", "fsht_create", "fsht_destroy", "fsht_get_stream",
     "fsht_set_stream", "fsht_synchronize", "fsht_info", "fsht_ring_tab", "fsht_set_precision", "fsht_t2alm", "fsht_t2alm_iter",
     "fsht_get_alm", "fsht_alm2t", "fsht_qu2eb", "fsht_qu2e", "fsht_qu2b", "fsht_eb2qu", "fsht_fix_eb",
+    "fsht_fix_eb_noalloc",
     "fsht_profile", "fsht_profile_read", "fsht_norm_alm", "fsht_filt_alm", "fsht_convert_alm_healpy", "fsht_convert_alm_healpy_inv",
     "fsht_alm2cl", "fsht_cl2alm", "fsht_host_alloc", "fsht_host_free",
 )
@@ -75,6 +76,7 @@ def load():
     lib.fsht_qu2b.argtypes = [H, dp, dp, dp, i]
     lib.fsht_eb2qu.argtypes = [H, dp, dp, dp, dp, i]
     lib.fsht_fix_eb.argtypes = [H, dp, dp, dp, dp, dp, dp, dp, i, i]
+    lib.fsht_fix_eb_noalloc.argtypes = [H, dp, dp, dp, dp, dp, dp, dp, i]
     lib.fsht_profile.argtypes = [H, i]
     lib.fsht_profile_read.argtypes = [H, ctypes.POINTER(ctypes.c_double), i]
     lib.fsht_norm_alm.argtypes = [dp, dp, dp, dp, d, i, i, vp]

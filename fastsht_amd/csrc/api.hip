@@ -10,6 +10,7 @@
 #include <string>
 
 #include <algorithm>
+#include <mutex>
 #include <vector>
 #include <thread>
 
@@ -33,6 +34,31 @@ struct fsht_handle {
 };
 
 static thread_local std::string g_err;
+
+// Scratch of the allocation-free entry point (fsht_fix_eb_noalloc): a
+// file-scope device array of kScratchBytes, one copy per device that loads
+// the library, held until the process ends.  Host code never uses the array's
+// name as an address: its device address is fetched once per device with
+// hipGetSymbolAddress (with that device current) and kept in g_scratch.
+constexpr size_t kScratchBytes = 128u << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static void* g_scratch[kMaxDevices] = {};
+
+// Device address of the current device's scratch (device `dev` is current).
+static hipError_t scratch_address(int dev, void** out) {
+  if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch[dev] = p;
+  }
+  *out = g_scratch[dev];
+  return hipSuccess;
+}
 
 static int fail(Handle* h, int code, const std::string& msg) {
   if (h) h->err = msg;
@@ -102,6 +128,7 @@ int fsht_create(int nside, int lmax, int nsim, int niter, int pol, fsht_handle**
   h->nsimp = (nsim + 15) / 16 * 16;
   h->pa = 4.0 * 3.141592653589793238462643383279502884 / (double)h->npix;
   hipError_t e = hipGetDevice(&h->device);
+  if (e == hipSuccess) e = scratch_address(h->device, &h->scratch);
   if (e == hipSuccess) e = hipStreamCreate(&h->own_stream);
   h->stream = h->own_stream;
   if (e == hipSuccess) e = fsht::setup_tables(h);
@@ -648,6 +675,26 @@ struct DevBuf {
 };
 }  // namespace
 
+// The fix_eb sequence on device arrays with given temporaries: alm sets dE,
+// dB2, dZ and map sets dQ2, dU2 (ab, bt, cf: almB, the template, the fit).
+static int fix_eb_device(fsht_handle* H, double* q, double* u, const double* mk, double* bm, double* ab, double* bt,
+                         double* cf, int niter, double* dE, double* dB2, double* dZ, double* dQ2, double* dU2) {
+  Handle* h = &H->h;
+  int rc;
+  HIPCHK(h, fsht::launch_zero(dZ, alm_count(h), h->stream));
+  HIPCHK(h, fsht::launch_mask_maps(q, mk, h->npix, h->nsim, h->stream));
+  HIPCHK(h, fsht::launch_mask_maps(u, mk, h->npix, h->nsim, h->stream));
+  if ((rc = fsht_qu2eb(H, q, u, dE, ab, niter, FSHT_DEVICE))) return rc;
+  if ((rc = fsht_eb2qu(H, dE, dZ, dQ2, dU2, FSHT_DEVICE))) return rc;
+  HIPCHK(h, fsht::launch_mask_maps(dQ2, mk, h->npix, h->nsim, h->stream));
+  HIPCHK(h, fsht::launch_mask_maps(dU2, mk, h->npix, h->nsim, h->stream));
+  if ((rc = fsht_qu2eb(H, dQ2, dU2, dE, dB2, niter, FSHT_DEVICE))) return rc;
+  if ((rc = fsht_alm2t(H, ab, dQ2, FSHT_DEVICE))) return rc;  // B map with leakage
+  if ((rc = fsht_alm2t(H, dB2, bt, FSHT_DEVICE))) return rc;  // leakage template
+  HIPCHK(h, fsht::launch_linfit_resi(bt, dQ2, mk, bm, cf, h->npix, h->nsim, h->stream));
+  return FSHT_OK;
+}
+
 int fsht_fix_eb(fsht_handle* H, double* Q, double* U, const double* mask, double* Bmap, double* almB, double* Btpl,
                 double* coef, int niter, int loc) {
   int rc = check_common(H, loc);
@@ -662,65 +709,99 @@ int fsht_fix_eb(fsht_handle* H, double* Q, double* U, const double* mask, double
   // on the error paths too
   DevBuf dQ, dU, dmask, dB, dE, dB2, dZ, dQ2, dU2, dBmap, dBtpl, dcoef;
   rc = [&]() -> int {
-  auto alloc = [&](DevBuf& b, size_t bytes) -> hipError_t { return hipMalloc((void**)&b.p, std::max<size_t>(bytes, 8)); };
-  double *q = Q, *u = U, *bm = Bmap, *bt = Btpl, *ab = almB, *cf = coef;
-  const double* mk = mask;
-  if (!loc) {
-    HIPCHK(h, alloc(dQ, mbytes));
-    HIPCHK(h, alloc(dU, mbytes));
-    HIPCHK(h, alloc(dmask, (size_t)h->npix * sizeof(double)));
-    HIPCHK(h, alloc(dBmap, mbytes));
-    HIPCHK(h, hipMemcpyAsync(dQ.p, Q, mbytes, hipMemcpyHostToDevice, h->stream));
-    HIPCHK(h, hipMemcpyAsync(dU.p, U, mbytes, hipMemcpyHostToDevice, h->stream));
-    HIPCHK(h, hipMemcpyAsync(dmask.p, mask, (size_t)h->npix * sizeof(double), hipMemcpyHostToDevice, h->stream));
-    q = dQ.p;
-    u = dU.p;
-    mk = dmask.p;
-    bm = dBmap.p;
-    bt = nullptr;
-    ab = nullptr;
-    cf = nullptr;
-  }
-  if (!bt) {
-    HIPCHK(h, alloc(dBtpl, mbytes));
-    bt = dBtpl.p;
-  }
-  if (!ab) {
-    HIPCHK(h, alloc(dB, abytes));
-    ab = dB.p;
-  }
-  if (!cf) {
-    HIPCHK(h, alloc(dcoef, 2 * (size_t)h->nsim * sizeof(double)));
-    cf = dcoef.p;
-  }
-  HIPCHK(h, alloc(dE, abytes));
-  HIPCHK(h, alloc(dB2, abytes));
-  HIPCHK(h, alloc(dZ, abytes));
-  HIPCHK(h, alloc(dQ2, mbytes));
-  HIPCHK(h, alloc(dU2, mbytes));
-  HIPCHK(h, hipMemsetAsync(dZ.p, 0, abytes, h->stream));
-  HIPCHK(h, fsht::launch_mask_maps(q, mk, h->npix, h->nsim, h->stream));
-  HIPCHK(h, fsht::launch_mask_maps(u, mk, h->npix, h->nsim, h->stream));
-  if ((rc = fsht_qu2eb(H, q, u, dE.p, ab, niter, FSHT_DEVICE))) return rc;
-  if ((rc = fsht_eb2qu(H, dE.p, dZ.p, dQ2.p, dU2.p, FSHT_DEVICE))) return rc;
-  HIPCHK(h, fsht::launch_mask_maps(dQ2.p, mk, h->npix, h->nsim, h->stream));
-  HIPCHK(h, fsht::launch_mask_maps(dU2.p, mk, h->npix, h->nsim, h->stream));
-  if ((rc = fsht_qu2eb(H, dQ2.p, dU2.p, dE.p, dB2.p, niter, FSHT_DEVICE))) return rc;
-  if ((rc = fsht_alm2t(H, ab, dQ2.p, FSHT_DEVICE))) return rc;  // B map with leakage
-  if ((rc = fsht_alm2t(H, dB2.p, bt, FSHT_DEVICE))) return rc;  // leakage template
-  HIPCHK(h, fsht::launch_linfit_resi(bt, dQ2.p, mk, bm, cf, h->npix, h->nsim, h->stream));
-  if (!loc) {
-    HIPCHK(h, hipMemcpyAsync(Q, dQ.p, mbytes, hipMemcpyDeviceToHost, h->stream));
-    HIPCHK(h, hipMemcpyAsync(U, dU.p, mbytes, hipMemcpyDeviceToHost, h->stream));
-    HIPCHK(h, hipMemcpyAsync(Bmap, bm, mbytes, hipMemcpyDeviceToHost, h->stream));
-    if (Btpl) HIPCHK(h, hipMemcpyAsync(Btpl, bt, mbytes, hipMemcpyDeviceToHost, h->stream));
-    if (almB) HIPCHK(h, hipMemcpyAsync(almB, ab, abytes, hipMemcpyDeviceToHost, h->stream));
-    if (coef) HIPCHK(h, hipMemcpyAsync(coef, cf, 2 * (size_t)h->nsim * sizeof(double), hipMemcpyDeviceToHost, h->stream));
-  }
-  return FSHT_OK;
+    auto alloc = [&](DevBuf& b, size_t bytes) -> hipError_t { return hipMalloc((void**)&b.p, std::max<size_t>(bytes, 8)); };
+    double *q = Q, *u = U, *bm = Bmap, *bt = Btpl, *ab = almB, *cf = coef;
+    const double* mk = mask;
+    if (!loc) {
+      HIPCHK(h, alloc(dQ, mbytes));
+      HIPCHK(h, alloc(dU, mbytes));
+      HIPCHK(h, alloc(dmask, (size_t)h->npix * sizeof(double)));
+      HIPCHK(h, alloc(dBmap, mbytes));
+      HIPCHK(h, hipMemcpyAsync(dQ.p, Q, mbytes, hipMemcpyHostToDevice, h->stream));
+      HIPCHK(h, hipMemcpyAsync(dU.p, U, mbytes, hipMemcpyHostToDevice, h->stream));
+      HIPCHK(h, hipMemcpyAsync(dmask.p, mask, (size_t)h->npix * sizeof(double), hipMemcpyHostToDevice, h->stream));
+      q = dQ.p;
+      u = dU.p;
+      mk = dmask.p;
+      bm = dBmap.p;
+      bt = nullptr;
+      ab = nullptr;
+      cf = nullptr;
+    }
+    if (!bt) {
+      HIPCHK(h, alloc(dBtpl, mbytes));
+      bt = dBtpl.p;
+    }
+    if (!ab) {
+      HIPCHK(h, alloc(dB, abytes));
+      ab = dB.p;
+    }
+    if (!cf) {
+      HIPCHK(h, alloc(dcoef, 2 * (size_t)h->nsim * sizeof(double)));
+      cf = dcoef.p;
+    }
+    HIPCHK(h, alloc(dE, abytes));
+    HIPCHK(h, alloc(dB2, abytes));
+    HIPCHK(h, alloc(dZ, abytes));
+    HIPCHK(h, alloc(dQ2, mbytes));
+    HIPCHK(h, alloc(dU2, mbytes));
+    int rc2 = fix_eb_device(H, q, u, mk, bm, ab, bt, cf, niter, dE.p, dB2.p, dZ.p, dQ2.p, dU2.p);
+    if (rc2) return rc2;
+    if (!loc) {
+      HIPCHK(h, hipMemcpyAsync(Q, dQ.p, mbytes, hipMemcpyDeviceToHost, h->stream));
+      HIPCHK(h, hipMemcpyAsync(U, dU.p, mbytes, hipMemcpyDeviceToHost, h->stream));
+      HIPCHK(h, hipMemcpyAsync(Bmap, bm, mbytes, hipMemcpyDeviceToHost, h->stream));
+      if (Btpl) HIPCHK(h, hipMemcpyAsync(Btpl, bt, mbytes, hipMemcpyDeviceToHost, h->stream));
+      if (almB) HIPCHK(h, hipMemcpyAsync(almB, ab, abytes, hipMemcpyDeviceToHost, h->stream));
+      if (coef) HIPCHK(h, hipMemcpyAsync(coef, cf, 2 * (size_t)h->nsim * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    }
+    return FSHT_OK;
   }();
   const hipError_t se = hipStreamSynchronize(h->stream);
   if (rc == FSHT_OK && se != hipSuccess) return fail(h, FSHT_ERR_HIP, std::string("fix_eb: ") + hipGetErrorString(se));
+  return rc;
+}
+
+int fsht_fix_eb_noalloc(fsht_handle* H, double* Q, double* U, const double* mask, double* Bmap, double* almB,
+                        double* Btpl, double* coef, int niter) {
+  int rc = check_common(H, FSHT_DEVICE);
+  if (rc) return rc;
+  Handle* h = &H->h;
+  if (!h->pol) return fail(h, FSHT_ERR_STATE, "fix_eb needs a handle created with pol=1");
+  if (!Q || !U || !mask || !Bmap) return fail(h, FSHT_ERR_ARG, "NULL argument");
+  if (niter < 0) return fail(h, FSHT_ERR_ARG, "niter must be >= 0");
+  if (niter > 0 && !h->ws_F)
+    return fail(h, FSHT_ERR_STATE, "niter > 0 needs a handle created with niter > 0 (its workspace is set up then)");
+  if (!h->scratch) return fail(h, FSHT_ERR_STATE, "no scratch address for this device");
+  // carve the temporaries out of the scratch, 256-byte aligned
+  const size_t abytes = (size_t)alm_count(h) * sizeof(double);
+  const size_t mbytes = (size_t)h->nsim * (size_t)h->npix * sizeof(double);
+  auto up = [](size_t b) { return (b + 255) & ~size_t(255); };
+  const size_t need = 3 * up(abytes) + 2 * up(mbytes) + (almB ? 0 : up(abytes)) + (Btpl ? 0 : up(mbytes)) +
+                      (coef ? 0 : up(2 * (size_t)h->nsim * sizeof(double)));
+  if (need > kScratchBytes)
+    return fail(h, FSHT_ERR_ARG,
+                "fix_eb_noalloc: this configuration needs " + std::to_string(need) + " bytes of scratch, more than the " +
+                    std::to_string(kScratchBytes) + " available");
+  char* p = static_cast<char*>(h->scratch);
+  auto take = [&](size_t b) {
+    double* r = reinterpret_cast<double*>(p);
+    p += up(b);
+    return r;
+  };
+  double* dE = take(abytes);
+  double* dB2 = take(abytes);
+  double* dZ = take(abytes);
+  double* dQ2 = take(mbytes);
+  double* dU2 = take(mbytes);
+  double* ab = almB ? almB : take(abytes);
+  double* bt = Btpl ? Btpl : take(mbytes);
+  double* cf = coef ? coef : take(2 * (size_t)h->nsim * sizeof(double));
+  // no profiling events either: recording them would create events
+  const int prof = h->profiling;
+  h->profiling = 0;
+  rc = fix_eb_device(H, Q, U, mask, Bmap, ab, bt, cf, niter, dE, dB2, dZ, dQ2, dU2);
+  h->profiling = prof;
   return rc;
 }
 

@@ -130,6 +130,7 @@ hipError_t launch_analysis_q(Handle* h, double* alm, double beta, int K);
 hipError_t launch_synthesis_q(Handle* h, double* G, int K);
 // map-domain helpers (k_maps.hip)
 hipError_t launch_mask_maps(double* maps, const double* mask, long long npix, int nmaps, hipStream_t s);
+hipError_t launch_zero(double* p, long long n, hipStream_t s);
 hipError_t launch_linfit_resi(const double* x, const double* y, const double* mask, double* resi, double* coef,
                               long long npix, int nmaps, hipStream_t s);
 
@@ -145,6 +146,7 @@ struct Handle {
   hipStream_t stream = nullptr;      // stream all work is enqueued on
   hipStream_t own_stream = nullptr;  // stream created (and destroyed) by the handle
   int device = 0;
+  void* scratch = nullptr;  // this device's copy of the static scratch array (fsht_fix_eb_noalloc)
   std::string err;
 
   // host tables

@@ -19,6 +19,19 @@ __global__ __launch_bounds__(256) void k_mask_maps(double* __restrict__ maps, co
   }
 }
 
+// p[0..n) = 0 with a kernel (a memset node recorded into a captured HIP graph
+// is not reliable on this platform; a kernel is)
+__global__ __launch_bounds__(256) void k_zero(double* __restrict__ p, long long n) {
+  for (long long i = blockIdx.x * 256LL + threadIdx.x; i < n; i += (long long)gridDim.x * 256) p[i] = 0.0;
+}
+
+hipError_t launch_zero(double* p, long long n, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  const long long nb = std::min<long long>((n + 255) / 256, 256LL * 64);
+  hipLaunchKernelGGL(k_zero, dim3((unsigned)nb), dim3(256), 0, s, p, n);
+  return hipGetLastError();
+}
+
 hipError_t launch_mask_maps(double* maps, const double* mask, long long npix, int nmaps, hipStream_t s) {
   const long long n = npix * nmaps;
   const long long nb = std::min<long long>((n + 255) / 256, 256LL * 64);

@@ -138,6 +138,19 @@ int fsht_eb2qu(fsht_handle* h, const double* E, const double* B, double* Q, doub
 int fsht_fix_eb(fsht_handle* h, double* Q, double* U, const double* mask, double* Bmap, double* almB, double* Btpl,
                 double* coef, int niter, int loc);
 
+/* fsht_fix_eb on device arrays (as loc = FSHT_DEVICE) without any device
+ * allocation.  No counterpart in the reference; it is for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops).  Its
+ * temporaries (3 alm sets and 2 map sets, plus almB / Btpl / coef when they
+ * are NULL, each rounded up to 256 bytes) come from a static device array of
+ * 128 MiB that every device holds from library load to process end; larger
+ * configurations return FSHT_ERR_ARG.  niter > 0 needs a handle created with
+ * niter > 0.  Asynchronous on the handle's stream.  All calls on one GPU share
+ * the array (it keeps the last call's contents; what must start at zero is
+ * cleared by the call), so calls that overlap on one GPU race. */
+int fsht_fix_eb_noalloc(fsht_handle* h, double* Q, double* U, const double* mask, double* Bmap, double* almB,
+                        double* Btpl, double* coef, int niter);
+
 /* Page-locked host memory for map / alm arrays: host-pointer calls move
  * such arrays by DMA directly (other host arrays go through the handle's
  * pinned staging buffers).  Replaces the numba.cuda.pinned_array outputs of
