@@ -1,13 +1,21 @@
 """Benchmark: batched temperature SHT round trip on one MI355X.
 
-One step = t2alm(niter=3) + alm2t over nsim maps at nside=1024, lmax=2047
-(FP64, inputs resident in HBM), on synthetic maps built as alm2t of Gaussian
-alm with C_l = 1/(l(l+1)) (l >= 2) plus white noise of variance 1e-3.
-Prints one JSON line: throughput, the dominant Legendre kernel's FP64-MFMA
-roofline fraction (HIP events on the library stream) and a bounded CPU
-baseline from the CPU restatement of the algorithm.
+One step = t2alm(niter=3) + alm2t over nsim maps at nside=1024, lmax=2047,
+inputs resident in HBM, on synthetic maps built as alm2t of Gaussian alm with
+C_l = 1/(l(l+1)) (l >= 2) plus white noise of variance 1e-3.
+
+The value is the mixed-precision rate (int8-slice MFMA Legendre contractions,
+FP64 FFTs, FP64 Jacobi refinement).  The same handle is also timed in plain
+FP64 (FP64 MFMA Legendre); both rates, their ratio, p50/p99 step latencies,
+the mixed alm error against the FP64 path for every niter 0..3 and the
+alm2t map error are printed on the one JSON line, with the dominant Legendre
+kernel's roofline (int8 MFMA for mixed, FP64 MFMA for FP64; HIP events on the
+library stream) and the CPU baseline (the CPU restatement of the algorithm:
+the full nside=64 configuration at 1 thread and at all threads, median of 5,
+and a bounded sample of the nside=1024 workload).
 
   python bench.py [--steps K] [--warmup W] [--nside 1024 --lmax 2047 --nsim 200 --niter 3]
+  python bench.py --only mixed|fp64      # time one precision only (profiling runs)
   python bench.py --pol [--steps K]      # spin-2 qu2eb+eb2qu (separate line, not the value)
 """
 from __future__ import annotations
@@ -17,17 +25,21 @@ import ctypes
 import json
 import math
 import os
+import statistics
 import sys
 import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-# MI355X published peaks
+# MI355X published dense peaks
 PEAK_FP64_MFMA_TFLOPS = 78.6
+PEAK_INT8_MFMA_TOPS = 5000.0
 PEAK_HBM_GBS = 8000.0
 KERNEL_CLASSES = ["ring_fft_fwd", "ring_fft_inv", "transpose", "legendre_analysis", "legendre_synthesis",
-                  "alias_fold", "spectrum_gather"]
+                  "alias_fold", "spectrum_gather", "operand_slicing"]
+NCLS = len(KERNEL_CLASSES)
+LEGENDRE = ("legendre_analysis", "legendre_synthesis")
 
 
 def legendre_flops_per_map(nside, lmax, i0):
@@ -39,6 +51,28 @@ def legendre_flops_per_map(nside, lmax, i0):
         g = 1 if m == 0 else 2
         tot += g * 2 * (lmax + 1 - m) * (2 * nside - int(i0[m]))
     return float(tot)
+
+
+def slice_products(k):
+    """int8 slice products of one mixed-precision contraction with k slices per
+    operand: the digit pairs (i, j) with i + j < k."""
+    return k * (k + 1) // 2
+
+
+def slice_ladder(info, niter):
+    """Slices per pass of a mixed t2alm(niter) + alm2t (the library's rule,
+    reported by fsht_info): analysis pass k is damped by niter - k later
+    Jacobi iterations, residual synthesis k by niter - k - 1; passes damped at
+    least damp_min times use k_lo slices, the others k_hi; alm2t and an
+    unrefined t2alm (niter = 0) use k_out."""
+    k_lo, k_hi, dmin, k_out = (int(v) for v in info[10:14])
+
+    def ks(d):
+        return k_lo if d >= dmin else k_hi
+    ana = [ks(niter - k) for k in range(niter + 1)] if niter > 0 else [k_out]
+    syn = [ks(niter - k - 1) for k in range(niter)]
+    return {"analysis": ana, "residual_synthesis": syn, "alm2t": k_out,
+            "k_lo": k_lo, "k_hi": k_hi, "damp_min": dmin}
 
 
 def make_inputs(sht, seed):
@@ -69,16 +103,74 @@ def make_inputs(sht, seed):
     return alm, maps
 
 
-def cpu_baseline(nsim=32, mstride=16):
+def _cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def _all_threads():
+    """Threads for the all-cores CPU leg: the host's logical CPUs, capped by
+    OMP_NUM_THREADS when the environment sets it (the CPU share given to this
+    process)."""
+    n = os.cpu_count() or 1
+    env = os.environ.get("OMP_NUM_THREADS")
+    if env and env.isdigit() and int(env) > 0:
+        n = min(n, int(env))
+    return n
+
+
+def cpu_c0(threads, reps=5):
+    """The oracle on the full nside=64 configuration: lmax=191, nsim=100 maps
+    = alm2t of Gaussian alm (C_l = 1/(l(l+1)), PCG64 seed 0) + N(0, 1e-3)
+    noise; t2alm niter=0, t2alm niter=3 and alm2t, each timed whole (wall
+    clock, one warm-up call, median of `reps`), per thread count."""
+    import numpy as np
+    from threadpoolctl import threadpool_limits
+
+    from oracle import oracle as O
+
+    nside, lmax, nsim = 64, 191, 100
+    S = O.OracleSHT(nside, lmax, nsim)
+    alm = O.random_alm(lmax, nsim, seed=0)
+    rng = np.random.default_rng(1)
+    maps = np.asfortranarray(S.alm2t(alm) + rng.standard_normal((S.npix, nsim)) * math.sqrt(1e-3))
+    out = {}
+    for nt in threads:
+        with threadpool_limits(nt):
+            S.t2alm(maps, niter=0)
+            S.alm2t(alm)
+            res = {}
+            for name, fn in (("t2alm_niter0", lambda: S.t2alm(maps, niter=0)),
+                             ("t2alm_niter3", lambda: S.t2alm(maps, niter=3)),
+                             ("alm2t", lambda: S.alm2t(alm))):
+                ts = []
+                for _ in range(reps):
+                    t0 = time.perf_counter()
+                    fn()
+                    ts.append(time.perf_counter() - t0)
+                res[name] = statistics.median(ts)
+        out[str(nt)] = {
+            "maps_per_s": {k: nsim / v for k, v in res.items()} | {
+                "t2alm_niter3+alm2t": nsim / (res["t2alm_niter3"] + res["alm2t"])},
+            "median_s": res,
+        }
+    return out
+
+
+def cpu_sample(nsim=32, mstride=16):
     """The oracle (CPU FP64 restatement of the reference algorithm: numpy ring
     FFTs, OpenBLAS dgemm per m, C Legendre recursion) on a bounded sample of the
-    same workload (nside=1024, lmax=2047): `nsim` maps, the full forward and
-    inverse ring FFTs, and the per-m analysis and synthesis passes for every
+    nside=1024, lmax=2047 workload: `nsim` maps, the full forward and inverse
+    ring FFTs, and the per-m analysis and synthesis passes for every
     `mstride`-th order m.  The per-m pass times are scaled to all m by the
     per-m work (lmax+1-m) * (4*nside-1 - 2*i0(m)); the round trip is
     fft_fwd + 4 analysis passes + 4 synthesis passes + fft_inv."""
     import numpy as np
-    from threadpoolctl import threadpool_info
 
     from oracle import oracle as O
 
@@ -91,28 +183,28 @@ def cpu_baseline(nsim=32, mstride=16):
     w = (lmax + 1 - np.arange(lmax + 1)) * (S.nring - 2 * S.rtab)
     scale = float(w.sum() / w[ms].sum())
     total = t["fft_fwd"] + t["fft_inv"] + 4.0 * scale * (t["analysis"] + t["synthesis"])
-    blas = [p.get("num_threads", 1) for p in threadpool_info() if p.get("user_api") == "blas"]
-    return {
-        "value": nsim / total,
-        "unit": "maps/s",
-        "cores": int(max(blas) if blas else 1),
-        "kind": "port",
-        "sample": (f"oracle (numpy FFT + OpenBLAS dgemm + C recursion) at nside=1024 lmax=2047, {nsim} maps: "
-                   f"ring FFT fwd {t['fft_fwd']:.1f}s + inv {t['fft_inv']:.1f}s, analysis {t['analysis']:.1f}s and "
-                   f"synthesis {t['synthesis']:.1f}s over every {mstride}th m scaled x{scale:.2f} by per-m work; "
-                   f"round trip = FFTs + 4 analysis + 4 synthesis passes"),
-        "cpu": _cpu_model(),
-    }
+    return nsim / total, (f"oracle (numpy FFT + OpenBLAS dgemm + C recursion) at nside=1024 lmax=2047, {nsim} maps: "
+                          f"ring FFT fwd {t['fft_fwd']:.1f}s + inv {t['fft_inv']:.1f}s, analysis {t['analysis']:.1f}s "
+                          f"and synthesis {t['synthesis']:.1f}s over every {mstride}th m scaled x{scale:.2f} by per-m "
+                          f"work; round trip = FFTs + 4 analysis + 4 synthesis passes")
 
 
-def _cpu_model():
-    try:
-        for line in open("/proc/cpuinfo"):
-            if line.startswith("model name"):
-                return line.split(":", 1)[1].strip()
-    except OSError:
-        pass
-    return "unknown"
+def cpu_baseline(c0=True):
+    """CPU legs, on rank 0's host: the value is the nside=1024 sample (the
+    benchmark's workload) at all threads; `c0` holds the full nside=64
+    configuration at 1 thread and at all threads."""
+    from threadpoolctl import threadpool_limits
+
+    nt = _all_threads()
+    with threadpool_limits(nt):
+        v, sample = cpu_sample()
+    out = {"value": v, "unit": "maps/s", "cores": nt, "kind": "port", "sample": sample,
+           "cpu": _cpu_model(), "host_logical_cpus": os.cpu_count(), "threads_all": nt}
+    if c0:
+        out["c0"] = {"what": "oracle, full nside=64 lmax=191 nsim=100 (alm2t of seeded Gaussian alm + N(0,1e-3)), "
+                             "no extrapolation; wall clock, 1 warm-up, median of 5",
+                     "threads": cpu_c0([1, nt]) if nt > 1 else cpu_c0([1])}
+    return out
 
 
 def make_eb(sht, seed):
@@ -139,11 +231,59 @@ def make_eb(sht, seed):
     return out
 
 
+def read_profile(sht, lib):
+    from fastsht_amd import _lib
+
+    prof = (ctypes.c_double * (2 * NCLS))()
+    _lib.check(lib.fsht_profile_read(sht._h, prof, 2 * NCLS), sht._h)
+    _lib.check(lib.fsht_profile(sht._h, 0), sht._h)
+    return {KERNEL_CLASSES[c]: {"launches": int(prof[2 * c]), "ms": prof[2 * c + 1]} for c in range(NCLS)}
+
+
+def timed_steps(sht, lib, step, steps, warmup):
+    """W untimed steps, then exactly K steps bracketed by synchronisation, with
+    one HIP event per step boundary on the library's stream (torch's current
+    stream) and the library's per-class kernel timers on."""
+    import torch
+
+    from fastsht_amd import _lib
+
+    sht._use_torch_stream()
+    for _ in range(warmup):
+        step()
+    torch.cuda.synchronize()
+    _lib.check(lib.fsht_profile(sht._h, 1), sht._h)
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(steps + 1)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ev[0].record()
+    for i in range(steps):
+        step()
+        ev[i + 1].record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms_total = ev[0].elapsed_time(ev[-1])
+    lat = sorted(ev[i].elapsed_time(ev[i + 1]) for i in range(steps))
+    return {"ms_total": ms_total, "lat": lat, "kern": read_profile(sht, lib), "wall": wall}
+
+
+def pct(lat, p):
+    return lat[min(len(lat) - 1, max(0, int(math.ceil(p / 100.0 * len(lat))) - 1))]
+
+
+def load_traffic(kernel, tag, nside, nsim):
+    tf = os.path.join(ROOT, "profiles", f"traffic_{kernel}{tag}_nside{nside}_nsim{nsim}.json")
+    if os.path.exists(tf):
+        return json.load(open(tf)).get("bytes_per_launch")
+    return None
+
+
 def pol_round_trip(args, lmax):
     """Spin-2 workload (nside 1024, lmax 2047, nsim Q/U pairs): one step =
     qu2eb(niter) + eb2qu on Q/U built as eb2qu of seeded E/B alm.  Prints one
     JSON line (Q/U pairs per second, per-niter E/B error against the input alm);
-    the T round trip stays the benchmark value."""
+    the T round trip stays the benchmark value.  Mixed precision covers the
+    temperature transforms only, so this line is FP64."""
     import torch
 
     from fastsht_amd import SHT, _lib
@@ -168,31 +308,19 @@ def pol_round_trip(args, lmax):
         _lib.check(lib.fsht_qu2eb(sht._h, p(Q), p(U), p(Eo), p(Bo), args.niter, _lib.FSHT_DEVICE), sht._h)
         _lib.check(lib.fsht_eb2qu(sht._h, p(Eo), p(Bo), p(Q), p(U), _lib.FSHT_DEVICE), sht._h)
 
-    sht._use_torch_stream()  # the events below are recorded on torch's current stream
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
-    _lib.check(lib.fsht_profile(sht._h, 1), sht._h)
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    e0.record()
-    for _ in range(args.steps):
-        step()
-    e1.record()
-    torch.cuda.synchronize()
-    ms = e0.elapsed_time(e1)
-    prof = (ctypes.c_double * 14)()
-    _lib.check(lib.fsht_profile_read(sht._h, prof, 14), sht._h)
-    kern = {KERNEL_CLASSES[c]: {"launches": int(prof[2 * c]), "ms": prof[2 * c + 1]} for c in range(7)}
+    r = timed_steps(sht, lib, step, args.steps, args.warmup)
+    kern = r["kern"]
     fpair = 4 * legendre_flops_per_map(args.nside, lmax, sht.ring_tab())
-    tf = {k: fpair * args.nsim / (kern[k]["ms"] / max(1, kern[k]["launches"]) / 1e3) / 1e12
-          for k in ("legendre_analysis", "legendre_synthesis")}
+    tf = {k: fpair * args.nsim / (kern[k]["ms"] / max(1, kern[k]["launches"]) / 1e3) / 1e12 for k in LEGENDRE}
     print(json.dumps({
         "metric": "Q/U pairs/s qu2eb+eb2qu (spin 2; not the benchmark value)",
-        "value": args.nsim * args.steps / (ms / 1e3), "unit": "QU pairs/s", "ms_per_step": ms / args.steps,
-        "steps": args.steps, "warmup": args.warmup, "dtype": "f64",
+        "value": args.nsim * args.steps / (r["ms_total"] / 1e3), "unit": "QU pairs/s",
+        "ms_per_step": r["ms_total"] / args.steps, "steps": args.steps, "warmup": args.warmup, "dtype": "f64",
+        "precision": "fp64 (mixed precision covers the temperature transforms only)",
         "config": {"workload": f"qu2eb(niter={args.niter})+eb2qu nside={args.nside} lmax={lmax} nsim={args.nsim} "
                                "Q/U from seeded E/B (C_l^BB = 0.05 C_l^EE)",
                    "nside": args.nside, "lmax": lmax, "nsim": args.nsim, "niter": args.niter},
+        "latency_ms_per_step": {"p50": pct(r["lat"], 50), "p99": pct(r["lat"], 99)},
         "legendre_tflops": tf,
         "eb_alm_rel_err_vs_input_by_niter": errs,
         "kernels_ms_per_step": {k: v["ms"] / args.steps for k, v in kern.items()},
@@ -228,8 +356,96 @@ def host_round_trip(sht, maps_dev, args):
     print(json.dumps({"metric": "PCIe-inclusive maps/s t2alm+alm2t from host arrays (not the benchmark value)",
                       "value": sht.nsim * args.steps / wall, "unit": "maps/s", "ms_per_step": 1e3 * wall / args.steps,
                       "host_bytes_moved_per_step_GB": gb, "steps": args.steps, "warmup": args.warmup,
+                      "precision": sht.precision,
                       "config": {"nside": sht.nside, "lmax": sht.lmax, "nsim": sht.nsim, "niter": args.niter,
                                  "maps_in": "pageable NumPy", "alms_and_maps_out": "pinned (fsht_host_alloc)"}}))
+
+
+def errors_by_niter(sht, lib, maps, niter):
+    """Mixed t2alm against FP64 t2alm on the same handle and maps, for every
+    niter 0..niter (max |alm_mixed - alm_fp64| / max |alm_fp64|), and the
+    alm2t map error (max |map_mixed - map_fp64| / max |map_fp64|) on the FP64
+    alm.  Outside every timed region."""
+    import torch
+
+    from fastsht_amd import _lib
+
+    p = lambda t: ctypes.c_void_p(t.data_ptr())
+    L = sht.lmax + 1
+    ref = torch.empty((L, L, sht.nsim), dtype=torch.float64, device="cuda").permute(2, 1, 0)
+    mix = torch.empty_like(ref)
+    errs = {}
+    for it in range(niter + 1):
+        sht.set_precision("fp64")
+        _lib.check(lib.fsht_t2alm(sht._h, p(maps), p(ref), it, _lib.FSHT_DEVICE), sht._h)
+        sht.set_precision("mixed")
+        _lib.check(lib.fsht_t2alm(sht._h, p(maps), p(mix), it, _lib.FSHT_DEVICE), sht._h)
+        torch.cuda.synchronize()
+        scale = float(ref.abs().max())
+        errs[str(it)] = float(mix.sub_(ref).abs_().max()) / scale
+    del mix
+    out = torch.empty((sht.nsim, sht.npix), dtype=torch.float64, device="cuda").t()
+    out_fp = torch.empty_like(out)
+    sht.set_precision("fp64")
+    _lib.check(lib.fsht_alm2t(sht._h, p(ref), p(out_fp), _lib.FSHT_DEVICE), sht._h)
+    sht.set_precision("mixed")
+    _lib.check(lib.fsht_alm2t(sht._h, p(ref), p(out), _lib.FSHT_DEVICE), sht._h)
+    torch.cuda.synchronize()
+    scale = float(out_fp.abs().max())
+    map_err = float(out.sub_(out_fp).abs_().max()) / scale
+    del ref, out, out_fp
+    torch.cuda.empty_cache()
+    return errs, map_err
+
+
+def legendre_roofline(kern, steps, work, bound_unit, peak):
+    """Roofline of the dominant Legendre class (the larger total time): the
+    algorithmic work of one pass (`work[cls]` per step / passes per step) over
+    the class's average pass duration.  A pass is one library call of the
+    class; its launches are all counted in its duration."""
+    dom = max(LEGENDRE, key=lambda k: kern[k]["ms"])
+    passes = kern[dom]["launches"] / steps
+    avg_ms = kern[dom]["ms"] / max(1, kern[dom]["launches"])
+    per_pass = work[dom] / passes
+    achieved = per_pass / (avg_ms / 1e3) / 1e12
+    return dom, {"bound": "mfma", "kernel": dom, "achieved": achieved, "peak": peak, "unit": bound_unit,
+                 "frac": achieved / peak, "work_per_launch": per_pass, "avg_launch_ms": avg_ms,
+                 "launches_per_step": passes}
+
+
+def gpu_c0(precision):
+    """The nside=64 configuration on the GPU (lmax=191, nsim=100, maps built as
+    in the CPU leg): t2alm niter=0, t2alm niter=3 and alm2t, each the median of
+    5 HIP-event-timed calls after 2 warm-up calls."""
+    import torch
+
+    from fastsht_amd import SHT
+
+    sht = SHT(64, 191, 100, niter=3)
+    alm, maps = make_inputs(sht, 1)
+    sht.set_precision(precision)
+    sht._use_torch_stream()
+    alm_out = torch.empty((192, 192, 100), dtype=torch.float64, device="cuda").permute(2, 1, 0)
+    maps_out = torch.empty((100, sht.npix), dtype=torch.float64, device="cuda").t()
+    from fastsht_amd import _lib
+    lib = _lib.load()
+    p = lambda t: ctypes.c_void_p(t.data_ptr())
+    res = {}
+    for name, fn in (("t2alm_niter0", lambda: lib.fsht_t2alm(sht._h, p(maps), p(alm_out), 0, _lib.FSHT_DEVICE)),
+                     ("t2alm_niter3", lambda: lib.fsht_t2alm(sht._h, p(maps), p(alm_out), 3, _lib.FSHT_DEVICE)),
+                     ("alm2t", lambda: lib.fsht_alm2t(sht._h, p(alm), p(maps_out), _lib.FSHT_DEVICE))):
+        for _ in range(2):
+            _lib.check(fn(), sht._h)
+        ts = []
+        for _ in range(5):
+            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            a.record()
+            _lib.check(fn(), sht._h)
+            b.record()
+            torch.cuda.synchronize()
+            ts.append(a.elapsed_time(b) / 1e3)
+        res[name] = statistics.median(ts)
+    return {k: 100 / v for k, v in res.items()} | {"t2alm_niter3+alm2t": 100 / (res["t2alm_niter3"] + res["alm2t"])}
 
 
 def main():
@@ -242,6 +458,8 @@ def main():
     ap.add_argument("--nsim", type=int, default=200)
     ap.add_argument("--niter", type=int, default=3)
     ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--only", choices=("mixed", "fp64"), default=None,
+                    help="time one precision only and skip the error and nside=64 legs (profiling runs)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-only", action="store_true", help="only the CPU baseline leg")
     ap.add_argument("--host", action="store_true",
@@ -283,10 +501,12 @@ def main():
         return
     sht = SHT(args.nside, lmax, args.nsim, niter=args.niter)
     alm, maps = make_inputs(sht, args.seed)
-    alm_out = torch.empty_like(alm.permute(2, 1, 0).contiguous()).permute(2, 1, 0)
+    L = lmax + 1
+    alm_out = torch.empty((L, L, args.nsim), dtype=torch.float64, device="cuda").permute(2, 1, 0)
     maps_out = torch.empty((args.nsim, sht.npix), dtype=torch.float64, device="cuda").t()
     lib = _lib.load()
     if args.host:
+        sht.set_precision(args.only or "mixed")
         host_round_trip(sht, maps, args)
         return
 
@@ -294,76 +514,99 @@ def main():
         sht.t2alm(maps, alm_out)
         sht.alm2t(alm_out, maps_out)
 
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
-    _lib.check(lib.fsht_profile(sht._h, 1), sht._h)
-    # one event per step boundary on the stream the library works on: the
-    # whole-run time and the per-step (one request of nsim maps) latencies
-    ev = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps + 1)]
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    ev[0].record()
-    for i in range(args.steps):
-        step()
-        ev[i + 1].record()
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    ms_total = ev[0].elapsed_time(ev[-1])
-    lat = sorted(ev[i].elapsed_time(ev[i + 1]) for i in range(args.steps))
+    sht.set_precision("mixed")  # builds the int8 tables before any timing
+    info = (ctypes.c_longlong * 14)()
+    _lib.check(lib.fsht_info(sht._h, info, 14), sht._h)
+    ladder = slice_ladder(list(info), args.niter)
+    runs = {}
+    for prec in ("fp64", "mixed"):
+        if args.only and prec != args.only:
+            continue
+        sht.set_precision(prec)
+        runs[prec] = timed_steps(sht, lib, step, args.steps, args.warmup)
+    rates = {k: args.nsim * args.steps / (r["ms_total"] / 1e3) for k, r in runs.items()}
+    head = "mixed" if "mixed" in runs else "fp64"
 
-    def pct(p):
-        return lat[min(len(lat) - 1, int(math.ceil(p / 100.0 * len(lat))) - 1)]
-    prof = (ctypes.c_double * 14)()
-    _lib.check(lib.fsht_profile_read(sht._h, prof, 14), sht._h)
-    _lib.check(lib.fsht_profile(sht._h, 0), sht._h)
-    kern = {KERNEL_CLASSES[c]: {"launches": int(prof[2 * c]), "ms": prof[2 * c + 1]} for c in range(7)}
-    ms_per_step = ms_total / args.steps
-    maps_per_s = args.nsim * args.steps / (ms_total / 1e3)
+    # round trip against the input alm (noise-limited), with the last timed precision's alm
+    rel = float(((alm_out - alm).abs().max() / alm.abs().max()).item())
+    del alm
+    torch.cuda.empty_cache()
 
-    # roofline of the dominant kernel (largest total time)
     i0 = sht.ring_tab()
     fpm = legendre_flops_per_map(args.nside, lmax, i0)
-    dom = max(("legendre_analysis", "legendre_synthesis"), key=lambda k: kern[k]["ms"])
-    avg_ms = kern[dom]["ms"] / max(1, kern[dom]["launches"])
-    flops_launch = fpm * args.nsim
-    achieved = flops_launch / (avg_ms / 1e3) / 1e12
-    traffic = None
-    tf = os.path.join(ROOT, "profiles", f"traffic_{dom}_nside{args.nside}_nsim{args.nsim}.json")
-    if os.path.exists(tf):
-        traffic = json.load(open(tf)).get("bytes_per_launch")
+    fp64_work = {"legendre_analysis": fpm * args.nsim * (args.niter + 1),
+                 "legendre_synthesis": fpm * args.nsim * (args.niter + 1)}
+    int8_work = {"legendre_analysis": fpm * args.nsim * sum(slice_products(k) for k in ladder["analysis"]),
+                 "legendre_synthesis": fpm * args.nsim * (sum(slice_products(k) for k in ladder["residual_synthesis"])
+                                                          + slice_products(ladder["alm2t"]))}
+    roofs = {}
+    for prec, r in runs.items():
+        if prec == "mixed":
+            dom, roof = legendre_roofline(r["kern"], args.steps, int8_work, "TOP/s", PEAK_INT8_MFMA_TOPS)
+            fp = legendre_roofline(r["kern"], args.steps, fp64_work, "TFLOP/s", PEAK_FP64_MFMA_TFLOPS)[1]
+            roof["fp64_equivalent_tflops"] = fp["achieved"]
+            roof["traffic"] = load_traffic(dom, "_mixed", args.nside, args.nsim)
+        else:
+            dom, roof = legendre_roofline(r["kern"], args.steps, fp64_work, "TFLOP/s", PEAK_FP64_MFMA_TFLOPS)
+            roof["traffic"] = load_traffic(dom, "", args.nside, args.nsim)
+        roofs[prec] = roof
 
-    # sanity: the round trip reproduces the inputs' alm (noise-limited)
-    rel = float(((alm_out - alm).abs().max() / alm.abs().max()).item())
+    def summary(prec):
+        r = runs[prec]
+        return {"maps_per_s": rates[prec], "ms_per_step": r["ms_total"] / args.steps,
+                "latency_ms_per_step": {"p50": pct(r["lat"], 50), "p99": pct(r["lat"], 99), "min": r["lat"][0],
+                                        "max": r["lat"][-1]},
+                "kernels_ms_per_step": {k: v["ms"] / args.steps for k, v in r["kern"].items()},
+                "roofline": roofs[prec], "wall_s": r["wall"]}
 
+    hr = runs[head]
     out = {
         "metric": "maps/s t2alm+alm2t @nside=1024 mixed vs FP64 at 1e-10 err; p50/p99 latency",
-        "value": maps_per_s,
+        "value": rates[head],
         "unit": "maps/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
+        "ms_per_step": hr["ms_total"] / args.steps,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "variant": head,
+        "dtype": "i8/f64" if head == "mixed" else "f64",
         "data": "synthetic: seeded Gaussian alm (C_l=1/(l(l+1))) -> alm2t + N(0,1e-3) noise, resident in HBM",
         "config": {"workload": f"t2alm(niter={args.niter})+alm2t nside={args.nside} lmax={lmax} nsim={args.nsim} T maps",
                    "nside": args.nside, "lmax": lmax, "nsim": args.nsim, "niter": args.niter,
-                   "precision": "plain FP64 path (FP64 MFMA Legendre, FP64 FFT)",
+                   "precision": ("mixed: int8-slice MFMA Legendre (exact int32 accumulation, FP64 combine), "
+                                 "FP64 FFT, FP64 residual refinement" if head == "mixed"
+                                 else "plain FP64 path (FP64 MFMA Legendre, FP64 FFT)"),
+                   "slices_per_pass": ladder,
                    "parallelism": "1 GPU (maps shard with no communication)"},
-        "roofline": {"bound": "mfma", "kernel": dom, "achieved": achieved, "peak": PEAK_FP64_MFMA_TFLOPS,
-                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_MFMA_TFLOPS, "traffic": traffic,
-                     "flops_per_launch": flops_launch, "avg_launch_ms": avg_ms},
-        "latency_ms_per_step": {"p50": pct(50), "p99": pct(99), "min": lat[0], "max": lat[-1],
+        "roofline": roofs[head],
+        "latency_ms_per_step": {"p50": pct(hr["lat"], 50), "p99": pct(hr["lat"], 99), "min": hr["lat"][0],
+                                "max": hr["lat"][-1],
                                 "what": f"one t2alm(niter={args.niter})+alm2t request of {args.nsim} maps"},
-        "kernels_ms_per_step": {k: v["ms"] / args.steps for k, v in kern.items()},
-        "wall_s": wall,
+        "kernels_ms_per_step": {k: v["ms"] / args.steps for k, v in hr["kern"].items()},
+        "wall_s": hr["wall"],
         "roundtrip_alm_rel_err_vs_input": rel,
     }
+    if "mixed" in runs:
+        out["mixed_maps_per_s"] = rates["mixed"]
+        out["mixed"] = summary("mixed")
+    if "fp64" in runs:
+        out["fp64_maps_per_s"] = rates["fp64"]
+        out["fp64"] = summary("fp64")
+    if len(runs) == 2:
+        out["mixed_vs_fp64"] = rates["mixed"] / rates["fp64"]
+    if args.only is None:
+        errs, map_err = errors_by_niter(sht, lib, maps, args.niter)
+        out["mixed_alm_rel_err_vs_fp64_by_niter"] = errs
+        out["mixed_alm2t_map_rel_err_vs_fp64"] = map_err
+        out["mixed_err_target"] = 1e-10
+        del maps, alm_out, maps_out, step, sht  # frees the handle's device memory
+        torch.cuda.empty_cache()
+        out["gpu_c0_maps_per_s"] = {p: gpu_c0(p) for p in ("fp64", "mixed")}
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline()
+        out["cpu_baseline"] = cpu_baseline(c0=args.only is None)
     print(json.dumps(out))
 
 

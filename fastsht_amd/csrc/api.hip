@@ -200,9 +200,10 @@ int fsht_set_stream(fsht_handle* H, void* stream) {
 int fsht_info(const fsht_handle* H, long long* out, int n) {
   if (!H || !out) return fail(nullptr, FSHT_ERR_ARG, "NULL argument");
   const Handle* h = &H->h;
-  long long v[10] = {h->nside, h->lmax, h->nsim,      h->niter,     h->pol,
-                     h->npix,  h->nsimp, h->G_size, h->precision, h->mix_bytes};
-  for (int i = 0; i < n && i < 10; ++i) out[i] = v[i];
+  long long v[14] = {h->nside,    h->lmax,     h->nsim,     h->niter,        h->pol,
+                     h->npix,     h->nsimp,    h->G_size,   h->precision,    h->mix_bytes,
+                     h->mix_k_lo, h->mix_k_hi, h->mix_damp_min, h->mix_k_out};
+  for (int i = 0; i < n && i < 14; ++i) out[i] = v[i];
   return FSHT_OK;
 }
 
@@ -481,7 +482,8 @@ static int analysis_iter_mixed(Handle* h, double* a0, int niter, int method) {
   auto kslices = [&](int damping) { return damping >= h->mix_damp_min ? h->mix_k_lo : h->mix_k_hi; };
   HIPCHK(h, fsht::launch_fexp_reset(h));
   HIPCHK(h, timed(h, fsht::KC_GATHER, [&] { return fsht::launch_gather(h, h->ws_specT, h->ws_G, h->d_FQe); }));
-  int ka = kslices(niter);
+  // a single unrefined analysis carries the full slice count
+  int ka = niter == 0 ? h->mix_k_out : kslices(niter);
   HIPCHK(h, timed(h, fsht::KC_SLICE, [&] { return fsht::launch_quant_F(h, h->ws_G, ka); }));
   HIPCHK(h, timed(h, fsht::KC_ANALYSIS, [&] { return fsht::launch_analysis_q(h, a0, 0.0, ka); }));
   if (niter > 0 && !h->ws_F)

@@ -103,7 +103,16 @@ class SHT:
         _lib.check(lib.fsht_create(self.nside, self.lmax, self.nsim, self.niter, self.pol, ctypes.byref(h)))
         self._h = h
         if precision == "mixed":
-            self._call(lib.fsht_set_precision, _lib.FSHT_PREC_MIXED)
+            self.set_precision("mixed")
+
+    def set_precision(self, precision):
+        """Switch the temperature transforms between "fp64" and "mixed" (the
+        first switch to "mixed" builds the handle's int8 Legendre tables)."""
+        if precision not in ("fp64", "mixed"):
+            raise ValueError("precision must be 'fp64' or 'mixed'")
+        code = _lib.FSHT_PREC_MIXED if precision == "mixed" else _lib.FSHT_PREC_FP64
+        self._call(_lib.load().fsht_set_precision, code)
+        self.precision = precision
 
     def __del__(self):
         h = getattr(self, "_h", None)
