@@ -62,17 +62,16 @@ def slice_products(k):
 def slice_ladder(info, niter):
     """Slices per pass of a mixed t2alm(niter) + alm2t (the library's rule,
     reported by fsht_info): analysis pass k is damped by niter - k later
-    Jacobi iterations, residual synthesis k by niter - k - 1; passes damped at
-    least damp_min times use k_lo slices, the others k_hi; alm2t and an
-    unrefined t2alm (niter = 0) use k_out."""
-    k_lo, k_hi, dmin, k_out = (int(v) for v in info[10:14])
-
-    def ks(d):
-        return k_lo if d >= dmin else k_hi
-    ana = [ks(niter - k) for k in range(niter + 1)] if niter > 0 else [k_out]
-    syn = [ks(niter - k - 1) for k in range(niter)]
+    Jacobi iterations, residual synthesis k by niter - k - 1; the first
+    analysis and syntheses damped at least damp_min times and the later
+    analyses damped at least damp_min_a times use k_lo slices, the others
+    k_hi; alm2t and an unrefined t2alm (niter = 0) use k_out."""
+    k_lo, k_hi, dmin, k_out, dmin_a = (int(v) for v in info[10:15])
+    ana = [k_lo if niter - k >= (dmin if k == 0 else dmin_a) else k_hi
+           for k in range(niter + 1)] if niter > 0 else [k_out]
+    syn = [k_lo if niter - k - 1 >= dmin else k_hi for k in range(niter)]
     return {"analysis": ana, "residual_synthesis": syn, "alm2t": k_out,
-            "k_lo": k_lo, "k_hi": k_hi, "damp_min": dmin}
+            "k_lo": k_lo, "k_hi": k_hi, "damp_min": dmin, "damp_min_a": dmin_a}
 
 
 def make_inputs(sht, seed):
@@ -515,8 +514,8 @@ def main():
         sht.alm2t(alm_out, maps_out)
 
     sht.set_precision("mixed")  # builds the int8 tables before any timing
-    info = (ctypes.c_longlong * 14)()
-    _lib.check(lib.fsht_info(sht._h, info, 14), sht._h)
+    info = (ctypes.c_longlong * 15)()
+    _lib.check(lib.fsht_info(sht._h, info, 15), sht._h)
     ladder = slice_ladder(list(info), args.niter)
     runs = {}
     for prec in ("fp64", "mixed"):

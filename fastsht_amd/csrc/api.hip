@@ -200,10 +200,10 @@ int fsht_set_stream(fsht_handle* H, void* stream) {
 int fsht_info(const fsht_handle* H, long long* out, int n) {
   if (!H || !out) return fail(nullptr, FSHT_ERR_ARG, "NULL argument");
   const Handle* h = &H->h;
-  long long v[14] = {h->nside,    h->lmax,     h->nsim,     h->niter,        h->pol,
+  long long v[15] = {h->nside,    h->lmax,     h->nsim,     h->niter,        h->pol,
                      h->npix,     h->nsimp,    h->G_size,   h->precision,    h->mix_bytes,
-                     h->mix_k_lo, h->mix_k_hi, h->mix_damp_min, h->mix_k_out};
-  for (int i = 0; i < n && i < 14; ++i) out[i] = v[i];
+                     h->mix_k_lo, h->mix_k_hi, h->mix_damp_min, h->mix_k_out, h->mix_damp_min_a};
+  for (int i = 0; i < n && i < 15; ++i) out[i] = v[i];
   return FSHT_OK;
 }
 
@@ -226,8 +226,10 @@ int fsht_set_precision(fsht_handle* H, int precision) {
       return fail(h, FSHT_ERR_NOMEM, "mixed precision: the int8 Legendre tables and sliced operands do not fit");
     if (e != hipSuccess) return fail(h, FSHT_ERR_HIP, std::string("mixed precision set-up: ") + hipGetErrorString(e));
   }
-  // slice counts: 4 for passes damped at least twice and 5 otherwise while
-  // lmax <= 2.5 nside (Jacobi contracts by ~0.1 per iteration there); stronger
+  // slice counts: 4 for the first analysis and residual syntheses damped at
+  // least twice and for the later analyses (of residuals) damped at least
+  // once, 5 otherwise, while lmax <= 2.5 nside (Jacobi contracts by ~0.1 per
+  // iteration there); stronger
   // aliasing contracts less, so 5 everywhere up to lmax = 3.5 nside and 6
   // beyond; 6 everywhere for nside <= 16
   if (h->nside <= 16) {  // coarse grids contract least; their passes are cheap
@@ -240,7 +242,8 @@ int fsht_set_precision(fsht_handle* H, int precision) {
   } else {
     h->mix_k_lo = h->mix_k_hi = 6;
   }
-  h->mix_damp_min = 2;
+  h->mix_damp_min = 2;    // residual syntheses
+  h->mix_damp_min_a = 1;  // analyses (after the first, they act on residuals)
   h->mix_k_out = 6;
   h->precision = precision;
   return FSHT_OK;
@@ -475,30 +478,34 @@ static int check_common(fsht_handle* H, int loc) {
 // Mixed precision (temperature): the same passes with int8-slice Legendre
 // contractions.  Slices per pass follow how strongly the later Jacobi
 // iterations damp that pass's error: the last analysis and the last two
-// residual syntheses carry more.
+// residual syntheses carry more.  An unrefined analysis (niter = 0) carries
+// the full slice count.
 static int analysis_iter_mixed(Handle* h, double* a0, int niter, int method) {
   // error of analysis k is damped by the niter - k later iterations, that of
   // synthesis k (the residual of iteration k) by niter - k - 1
   auto kslices = [&](int damping) { return damping >= h->mix_damp_min ? h->mix_k_lo : h->mix_k_hi; };
+  auto kslices_a = [&](int damping) { return damping >= h->mix_damp_min_a ? h->mix_k_lo : h->mix_k_hi; };
   HIPCHK(h, fsht::launch_fexp_reset(h));
   HIPCHK(h, timed(h, fsht::KC_GATHER, [&] { return fsht::launch_gather(h, h->ws_specT, h->ws_G, h->d_FQe); }));
   // a single unrefined analysis carries the full slice count
-  int ka = niter == 0 ? h->mix_k_out : kslices(niter);
+  int ka = niter == 0 ? h->mix_k_out : kslices(niter);  // the full maps: as damped as a synthesis
   HIPCHK(h, timed(h, fsht::KC_SLICE, [&] { return fsht::launch_quant_F(h, h->ws_G, ka); }));
-  HIPCHK(h, timed(h, fsht::KC_ANALYSIS, [&] { return fsht::launch_analysis_q(h, a0, 0.0, ka); }));
+  // each analysis but the last also records alm's column exponents for the
+  // next residual synthesis
+  HIPCHK(h, timed(h, fsht::KC_ANALYSIS, [&] { return fsht::launch_analysis_q(h, a0, 0.0, ka, niter > 0); }));
   if (niter > 0 && !h->ws_F)
     HIPCHK(h, hipMalloc((void**)&h->ws_F, (size_t)(h->G_size / (h->pol ? 1 : 2)) * sizeof(double)));
   for (int it = 0; it < niter; ++it) {
     const int ks = kslices(niter - it - 1);
-    HIPCHK(h, timed(h, fsht::KC_SLICE, [&] { return fsht::launch_quant_A(h, a0, ks); }));
+    HIPCHK(h, timed(h, fsht::KC_SLICE, [&] { return fsht::launch_quant_A(h, a0, ks, 1); }));
     HIPCHK(h, timed(h, fsht::KC_SYNTHESIS, [&] { return fsht::launch_synthesis_q(h, h->ws_G, ks); }));
     HIPCHK(h, fsht::launch_fexp_reset(h));
     HIPCHK(h, timed(h, fsht::KC_FOLD, [&] {
       return fsht::launch_gather_prefix(h, h->ws_G, h->ws_specT, h->ws_F, method == 0 ? 1 : 0, h->d_FQe);
     }));
-    ka = kslices(niter - it - 1);
+    ka = kslices_a(niter - it - 1);
     HIPCHK(h, timed(h, fsht::KC_SLICE, [&] { return fsht::launch_quant_F(h, h->ws_F, ka); }));
-    HIPCHK(h, timed(h, fsht::KC_ANALYSIS, [&] { return fsht::launch_analysis_q(h, a0, 1.0, ka); }));
+    HIPCHK(h, timed(h, fsht::KC_ANALYSIS, [&] { return fsht::launch_analysis_q(h, a0, 1.0, ka, it + 1 < niter); }));
   }
   return FSHT_OK;
 }
@@ -581,7 +588,7 @@ int fsht_alm2t(fsht_handle* H, const double* alms, double* maps, int loc) {
     h->have_dev_alm = 0;
   }
   if (h->precision == FSHT_PREC_MIXED) {
-    HIPCHK(h, timed(h, fsht::KC_SLICE, [&] { return fsht::launch_quant_A(h, a, h->mix_k_out); }));
+    HIPCHK(h, timed(h, fsht::KC_SLICE, [&] { return fsht::launch_quant_A(h, a, h->mix_k_out, 0); }));
     HIPCHK(h, timed(h, fsht::KC_SYNTHESIS, [&] { return fsht::launch_synthesis_q(h, h->ws_G, h->mix_k_out); }));
   } else {
     HIPCHK(h, timed(h, fsht::KC_SYNTHESIS, [&] { return fsht::launch_synthesis(h, a, nullptr, h->ws_G); }));

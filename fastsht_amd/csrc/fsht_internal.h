@@ -125,8 +125,10 @@ hipError_t mixed_setup(Handle* h);        // P tables + work lists (allocates; s
 void mixed_free(Handle* h);
 hipError_t launch_fexp_reset(Handle* h);                          // FQe <- kMixZeroE
 hipError_t launch_quant_F(Handle* h, const double* F, int K);      // m-major FP64 operand -> FQ (FQe set)
-hipError_t launch_quant_A(Handle* h, const double* alm, int K);    // packed alm -> AQ
-hipError_t launch_analysis_q(Handle* h, double* alm, double beta, int K);
+// packed alm -> AQ; have_exp: AQe already holds alm's column exponents (set by
+// the analysis that wrote alm with exps = 1), else they are computed here
+hipError_t launch_quant_A(Handle* h, const double* alm, int K, int have_exp);
+hipError_t launch_analysis_q(Handle* h, double* alm, double beta, int K, int exps);
 hipError_t launch_synthesis_q(Handle* h, double* G, int K);
 // map-domain helpers (k_maps.hip)
 hipError_t launch_mask_maps(double* maps, const double* mask, long long npix, int nmaps, hipStream_t s);
@@ -231,11 +233,13 @@ struct Handle {
   int n_anaq_items[3] = {0, 0, 0}, n_synq_items[3] = {0, 0, 0};
   long long mix_bytes = 0;  // device memory held by the mixed-precision tables and operands
   int mix_nrb_max = 0;  // 256-row blocks of the longest order (grid of the operand slicer)
-  // slices per pass: a pass whose error the remaining Jacobi iterations of
-  // t2alm damp at least mix_damp_min times uses mix_k_lo, the others mix_k_hi
-  // (set from lmax / nside: the stronger the aliasing, the weaker the damping);
-  // alm2t uses mix_k_out
-  int mix_k_lo = 4, mix_k_hi = 5, mix_damp_min = 2, mix_k_out = 6;
+  // slices per pass: the first analysis and a residual synthesis whose error
+  // the remaining Jacobi iterations of t2alm damp at least mix_damp_min
+  // times, and a later analysis (of a residual) damped at least
+  // mix_damp_min_a times, use mix_k_lo, the others mix_k_hi
+  // (set from lmax / nside: the stronger the aliasing, the weaker the
+  // damping); alm2t and an unrefined t2alm use mix_k_out
+  int mix_k_lo = 4, mix_k_hi = 5, mix_damp_min = 2, mix_k_out = 6, mix_damp_min_a = 1;
 
   // optional per-kernel-class timing with HIP events on `stream`
   int profiling = 0;

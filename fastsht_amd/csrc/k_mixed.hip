@@ -260,12 +260,13 @@ __global__ __launch_bounds__(256) void k_quant_F(const MixOff* __restrict__ mo, 
 
 // ---------------------------------------------------------------------------
 // Synthesis operand: packed alm -> AQ slices with one exponent per column
-// (set here into AQe, index (2 m + class) * ncols + col).  Columns as in the
-// m-major buffer: col < nsimp is Re a_lm of map col, nsimp <= col < 2 nsimp
-// is Im.  Workgroup = (m, class, 32-column tile): the column maxima first,
-// then the digits (second read of the same alm, mostly from cache).
+// (AQe, index (2 m + class) * ncols + col).  Columns as in the m-major
+// buffer: col < nsimp is Re a_lm of map col, nsimp <= col < 2 nsimp is Im.
+// Workgroup = (m, class, 32-column tile).  HAVE_EXP = 0: the column maxima
+// first (setting AQe), then the digits (second read of the same alm, mostly
+// from cache); HAVE_EXP = 1: AQe was set by the analysis that wrote alm.
 // ---------------------------------------------------------------------------
-template <int K>
+template <int K, int HAVE_EXP>
 __global__ __launch_bounds__(256) void k_quant_A(const MixOff* __restrict__ mo, const double* __restrict__ alm,
                                                  signed char* __restrict__ AQ, int* __restrict__ AQe, int nsim,
                                                  int nsimp, int lmax) {
@@ -285,22 +286,25 @@ __global__ __launch_bounds__(256) void k_quant_A(const MixOff* __restrict__ mo, 
     const int l = m + cls + 2 * (lst * 32 + j);
     return (ok && l <= lmax) ? src[(long long)nsim * l] : 0.0;
   };
-  double mx = 0.0;
-  for (int lst = q; lst < o.nlt; lst += 8)
+  int* pe = AQe + (long long)(2 * m + cls) * ncols + col;
+  if (!HAVE_EXP) {
+    double mx = 0.0;
+    for (int lst = q; lst < o.nlt; lst += 8)
 #pragma unroll 8
-    for (int j = 0; j < 32; ++j) mx = fmax(mx, fabs(val(lst, j)));
-  red[q][cl] = mx;
-  __syncthreads();
-  if (q == 0) {
-    double M = red[0][cl];
+      for (int j = 0; j < 32; ++j) mx = fmax(mx, fabs(val(lst, j)));
+    red[q][cl] = mx;
+    __syncthreads();
+    if (q == 0) {
+      double M = red[0][cl];
 #pragma unroll
-    for (int k = 1; k < 8; ++k) M = fmax(M, red[k][cl]);
-    const int e = mix_exp(M);
-    ex[cl] = e;
-    AQe[(long long)(2 * m + cls) * ncols + col] = e;
+      for (int k = 1; k < 8; ++k) M = fmax(M, red[k][cl]);
+      const int e = mix_exp(M);
+      ex[cl] = e;
+      *pe = e;
+    }
+    __syncthreads();
   }
-  __syncthreads();
-  const int e = ex[cl];
+  const int e = HAVE_EXP ? *pe : ex[cl];
   for (int lst = q; lst < o.nlt; lst += 8) {
     double v[32];
 #pragma unroll
@@ -357,7 +361,8 @@ __global__ __launch_bounds__(512, 1) void k_slice_gemm(const LegItem* __restrict
                                                        const int* __restrict__ OQe, double* __restrict__ out,
                                                        const long long* __restrict__ goff,
                                                        const int* __restrict__ i0tab, int nsim, int nsimp, int lmax,
-                                                       int nhr, int nct, double pa, double beta) {
+                                                       int nhr, int nct, double pa, double beta,
+                                                       int* __restrict__ aqe) {
   static_assert(NB == 4 || NB == 1, "column tiles per workgroup");
   constexpr int NW = 8;                                 // waves
   constexpr int NA = NB == 4 ? 4 : 8;                   // row tiles
@@ -379,23 +384,37 @@ __global__ __launch_bounds__(512, 1) void k_slice_gemm(const LegItem* __restrict
   const signed char* baseA = TP + (SYN ? o.ts : o.ta);
   const signed char* baseB = OQ + (SYN ? o.aq : o.fq);
 
-  auto issue = [&](int s) {
-    signed char* stage = st[s % NS];
+  const int s0 = it.r0;
+  // this wave's PER source streams: wave-uniform pointers at step s0 that
+  // advance by one step's stride per issue (A tiles keep kMixK slices per
+  // step, B tiles K), and their blocks within a stage
+  const signed char* gsrc[PER];
+  int gstep[PER], lofs[PER];
+#pragma unroll
+  for (int k = 0; k < PER; ++k) {
+    int b = w + NW * k;
+    if (b >= NBLK) b -= NW;  // this wave's previous block again (same data, same place)
+    lofs[k] = b * 1024;
+    if (b < NA * K) {
+      const int tile = min(it.blk * NA + b / K, ntA - 1);
+      gsrc[k] = baseA + (((long long)tile * nsteps + s0) * kMixK + b % K) * 1024;
+      gstep[k] = kMixK * 1024;
+    } else {
+      const int bb = b - NA * K;
+      const int tile = min(it.colblk * NB + bb / K, nct - 1);
+      gsrc[k] = baseB + (((long long)tile * nsteps + s0) * K + bb % K) * 1024;
+      gstep[k] = K * 1024;
+    }
+  }
+  int fill = 0;  // stage the next issue fills (steps s0, s0 + 1, ... go to stages 0, 1, ..., NS - 1, 0, ...)
+  auto issue = [&]() {
+    signed char* stage = st[fill];
 #pragma unroll
     for (int k = 0; k < PER; ++k) {
-      int b = w + NW * k;
-      if (b >= NBLK) b -= NW;  // this wave's previous block again (same data, same place)
-      const signed char* g;
-      if (b < NA * K) {
-        const int tile = min(it.blk * NA + b / K, ntA - 1);
-        g = baseA + (((long long)tile * nsteps + s) * kMixK + b % K) * 1024;
-      } else {
-        const int bb = b - NA * K;
-        const int tile = min(it.colblk * NB + bb / K, nct - 1);
-        g = baseB + (((long long)tile * nsteps + s) * K + bb % K) * 1024;
-      }
-      __builtin_amdgcn_global_load_lds((const void*)(g + L * 16), (lds_ptr)(stage + b * 1024), 16, 0, 0);
+      __builtin_amdgcn_global_load_lds((const void*)(gsrc[k] + L * 16), (lds_ptr)(stage + lofs[k]), 16, 0, 0);
+      gsrc[k] += gstep[k];
     }
+    fill = fill + 1 == NS ? 0 : fill + 1;
   };
 
   const int wa = NB == 4 ? (w >> 1) : w;         // this wave's row tile within the workgroup
@@ -406,10 +425,10 @@ __global__ __launch_bounds__(512, 1) void k_slice_gemm(const LegItem* __restrict
   v16i D0[K], D1[K];
 #pragma unroll
   for (int d = 0; d < K; ++d) D0[d] = D1[d] = v16i{};
-  const int s0 = it.r0;
 #pragma unroll
   for (int k = 0; k < NS - 1; ++k)
-    if (s0 + k < nsteps) issue(s0 + k);
+    if (s0 + k < nsteps) issue();
+  int use = 0;  // stage holding step s
   for (int s = s0; s < nsteps; ++s) {
     // this wave's loads of step s are done once at most the later issued
     // steps (up to NS - 2 of them) are outstanding
@@ -418,15 +437,18 @@ __global__ __launch_bounds__(512, 1) void k_slice_gemm(const LegItem* __restrict
     else if (ahead >= 1) wait_vm<PER>();
     else wait_vm<0>();
     raw_barrier();
-    if (s + NS - 1 < nsteps) issue(s + NS - 1);  // the stage read in step s - 1
+    const signed char* sb = st[use];
+    use = use + 1 == NS ? 0 : use + 1;
+    // A slices stay in registers for the step; B slices are read one at a
+    // time (slice j of the B tiles meets A slices 0 .. K-1-j)
+    const signed char* pb = sb + (NA * K + wb * K) * 1024 + L * 16;
+    v4i a[K];
     if (act) {
-      // A slices stay in registers for the step; B slices are read one at a
-      // time (slice j of the B tiles meets A slices 0 .. K-1-j)
-      const signed char* sb = st[s % NS];
-      const signed char* pb = sb + (NA * K + wb * K) * 1024 + L * 16;
-      v4i a[K];
 #pragma unroll
       for (int sl = 0; sl < K; ++sl) a[sl] = *reinterpret_cast<const v4i*>(sb + (wa * K + sl) * 1024 + L * 16);
+    }
+    if (s + NS - 1 < nsteps) issue();  // into the stage read in step s - 1
+    if (act) {
 #pragma unroll
       for (int j = 0; j < K; ++j) {
         const v4i b0 = *reinterpret_cast<const v4i*>(pb + j * 1024);
@@ -456,15 +478,28 @@ __global__ __launch_bounds__(512, 1) void k_slice_gemm(const LegItem* __restrict
       const int sim = col - reim * nsimp;
       if (sim >= nsim || (m == 0 && reim)) continue;
       const long long L1 = lmax + 1;
+      // this column's entries: sim + nsim (l + L1 m) for Re, sim + nsim (l - m + L1 (L1 - m)) for Im
+      double* oc = out + sim + (long long)nsim * (reim ? L1 * (L1 - m) - m : L1 * m);
+      // the previous values (beta != 0) are all loaded before any store, so
+      // the read-modify-write costs one memory latency, not sixteen
+      double prev[16];
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int l = m + cls + 2 * (ta * 32 + drow(r, h));
+        prev[r] = (beta != 0.0 && l <= lmax) ? oc[(long long)nsim * l] : 0.0;
+      }
+      double mx = 0.0;
 #pragma unroll
       for (int r = 0; r < 16; ++r) {
         const int l = m + cls + 2 * (ta * 32 + drow(r, h));
         if (l > lmax) continue;
-        const long long idx =
-            reim ? (long long)sim + nsim * ((l - m) + L1 * (L1 - m)) : (long long)sim + nsim * (l + L1 * m);
-        const double v = pa * __builtin_ldexp(combine<K>(t ? D1 : D0, r), pe[r] + eB);
-        out[idx] = (beta != 0.0) ? v + beta * out[idx] : v;
+        const double v = pa * __builtin_ldexp(combine<K>(t ? D1 : D0, r), pe[r] + eB) + beta * prev[r];
+        oc[(long long)nsim * l] = v;
+        mx = fmax(mx, fabs(v));
       }
+      // the synthesis operand's exponent of this column: max over every
+      // workgroup writing it (k_quant_A<K, 1> reads it)
+      if (aqe) atomicMax(aqe + (long long)(2 * m + cls) * ncols + col, mix_exp(mx));
     } else {
       const int R = nhr + 1 - i0tab[m];
       const long long gbase = goff[m] / 2 + (long long)cls * R * ncols;
@@ -663,16 +698,19 @@ hipError_t launch_quant_F(Handle* h, const double* F, int K) {
   return hipGetLastError();
 }
 
-hipError_t launch_quant_A(Handle* h, const double* alm, int K) {
+hipError_t launch_quant_A(Handle* h, const double* alm, int K, int have_exp) {
   const int nct = h->nsimp / 16;
   dim3 grid(h->lmax + 1, 2, nct);
-#define FSHT_QA(KV)                                                                                              \
-  hipLaunchKernelGGL(k_quant_A<KV>, grid, dim3(256), 0, h->stream, h->d_mix_off, alm, h->d_AQ, h->d_AQe, h->nsim, \
+#define FSHT_QA(KV, E)                                                                                              \
+  hipLaunchKernelGGL((k_quant_A<KV, E>), grid, dim3(256), 0, h->stream, h->d_mix_off, alm, h->d_AQ, h->d_AQe, h->nsim, \
                      h->nsimp, h->lmax)
-  switch (K) {
-    case 4: FSHT_QA(4); break;
-    case 5: FSHT_QA(5); break;
-    case 6: FSHT_QA(6); break;
+  switch (K * 2 + (have_exp ? 1 : 0)) {
+    case 8: FSHT_QA(4, 0); break;
+    case 9: FSHT_QA(4, 1); break;
+    case 10: FSHT_QA(5, 0); break;
+    case 11: FSHT_QA(5, 1); break;
+    case 12: FSHT_QA(6, 0); break;
+    case 13: FSHT_QA(6, 1); break;
     default: return hipErrorInvalidValue;
   }
 #undef FSHT_QA
@@ -680,7 +718,7 @@ hipError_t launch_quant_A(Handle* h, const double* alm, int K) {
 }
 
 template <int SYN>
-static hipError_t launch_gemm(Handle* h, double* out, double beta, int K) {
+static hipError_t launch_gemm(Handle* h, double* out, double beta, int K, int* aqe) {
   const int nct = h->nsimp / 16;
   const signed char* TP = SYN ? h->d_TS : h->d_TA;
   const int* TPe = SYN ? h->d_TSe : h->d_TAe;
@@ -695,7 +733,7 @@ static hipError_t launch_gemm(Handle* h, double* out, double beta, int K) {
     const LegItem* items = SYN ? h->d_synq_items[k] : h->d_anaq_items[k];
 #define FSHT_GQ(KV, NBV)                                                                                          \
   hipLaunchKernelGGL((k_slice_gemm<KV, SYN, NBV>), dim3(n), dim3(512), 0, h->stream, items, h->d_mix_off, TP, TPe, \
-                     OQ, OQe, out, h->d_goff, h->d_i0, h->nsim, h->nsimp, h->lmax, h->nhr, nct, h->pa, beta)
+                     OQ, OQe, out, h->d_goff, h->d_i0, h->nsim, h->nsimp, h->lmax, h->nhr, nct, h->pa, beta, aqe)
     if (k == 1) {
       switch (K) {
         case 4: FSHT_GQ(4, 1); break;
@@ -717,7 +755,12 @@ static hipError_t launch_gemm(Handle* h, double* out, double beta, int K) {
   return hipSuccess;
 }
 
-hipError_t launch_analysis_q(Handle* h, double* alm, double beta, int K) { return launch_gemm<0>(h, alm, beta, K); }
-hipError_t launch_synthesis_q(Handle* h, double* G, int K) { return launch_gemm<1>(h, G, 0.0, K); }
+hipError_t launch_analysis_q(Handle* h, double* alm, double beta, int K, int exps) {
+  if (!exps) return launch_gemm<0>(h, alm, beta, K, nullptr);
+  // alm column exponents for the next synthesis operand
+  hipError_t e = fill_int(h, h->d_AQe, 2LL * (h->lmax + 1) * 2 * h->nsimp, kMixZeroE);
+  return e != hipSuccess ? e : launch_gemm<0>(h, alm, beta, K, h->d_AQe);
+}
+hipError_t launch_synthesis_q(Handle* h, double* G, int K) { return launch_gemm<1>(h, G, 0.0, K, nullptr); }
 
 }  // namespace fsht

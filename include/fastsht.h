@@ -64,11 +64,12 @@ int fsht_synchronize(fsht_handle* h);
 
 /* out[0..n) = {nside, lmax, nsim, niter, pol, npix, nsim padded to 16, m-major
  * buffer size, precision, device bytes held for mixed precision, and the
- * mixed-precision slice counts: k_lo, k_hi, damp_min, k_out}.  In mixed
- * precision a t2alm(niter) pass whose error the later Jacobi iterations damp
- * at least damp_min times (analysis k: niter - k, synthesis k: niter - k - 1)
- * uses k_lo slices per operand, the others k_hi; alm2t and t2alm with
- * niter = 0 use k_out. */
+ * mixed-precision slice counts: k_lo, k_hi, damp_min, k_out, damp_min_a}.  In
+ * mixed precision the first analysis of a t2alm(niter) and a residual
+ * synthesis whose error the later Jacobi iterations damp at least damp_min
+ * times (analysis k: niter - k, synthesis k: niter - k - 1), and a later
+ * analysis damped at least damp_min_a times, use k_lo slices per operand, the
+ * others k_hi; alm2t and t2alm with niter = 0 use k_out. */
 int fsht_info(const fsht_handle* h, long long* out, int n);
 
 #define FSHT_PREC_FP64 0  /* every Legendre contraction in FP64 (the default) */
