@@ -293,27 +293,36 @@ class SHT:
                                                                   and Q.dtype == np.float64))
         up, uloc, uk = self._arg(U, self._mshape, "U", out=dev or (isinstance(U, np.ndarray) and U.flags.f_contiguous
                                                                    and U.dtype == np.float64))
+        if len({loc, uloc}) != 1:
+            raise ValueError("all arrays must be host arrays or all GPU tensors")
+        for name, arr in (("Btpl_out", Btpl_out), ("almB_out", almB_out)):
+            if arr is not None and _is_torch(arr) != dev:
+                raise ValueError(f"{name} must be {'a GPU tensor' if dev else 'a host array'} like Q and U")
+        # the mask goes where Q is: a host mask (NumPy or CPU tensor) is copied
+        # to Q's GPU, a GPU mask is brought to the host for host Q
         if dev:
             torch = _torch()
-            mk = mask if _is_torch(mask) else torch.as_tensor(np.asarray(mask, dtype=np.float64), device="cuda")
-            mk = mk.to(torch.float64).contiguous()
+            mk = torch.as_tensor(mask, dtype=torch.float64, device=Q.device).reshape(-1).contiguous()
             if mk.numel() != self.npix:
                 raise ValueError(f"mask must have npix={self.npix} entries")
             mp = ctypes.c_void_p(mk.data_ptr())
         else:
-            mk = np.ascontiguousarray(np.asarray(mask, dtype=np.float64).reshape(-1))
+            host_mask = mask.detach().cpu().numpy() if _is_torch(mask) else mask
+            mk = np.ascontiguousarray(np.asarray(host_mask, dtype=np.float64).reshape(-1))
             if mk.size != self.npix:
                 raise ValueError(f"mask must have npix={self.npix} entries")
             mp = ctypes.c_void_p(mk.ctypes.data)
         Bmap = self._like(Q, "map")
         bp, bloc, bk = self._arg(Bmap, self._mshape, "Bmap", out=True)
         tp = ap = None
-        if Btpl_out is not None:
-            tp = self._arg(Btpl_out, self._mshape, "Btpl_out", out=True)[0]
-        if almB_out is not None:
-            ap = self._arg(almB_out, self._ashape, "almB_out", out=True)[0]
-        if len({loc, uloc, bloc}) != 1:
-            raise ValueError("all arrays must be host arrays or all GPU tensors")
+        for name, arr, shape in (("Btpl_out", Btpl_out, self._mshape), ("almB_out", almB_out, self._ashape)):
+            if arr is None:
+                continue
+            ptr = self._arg(arr, shape, name, out=True)[0]
+            if name == "Btpl_out":
+                tp = ptr
+            else:
+                ap = ptr
         self._call(lib.fsht_fix_eb, qp, up, mp, bp, ap, tp, None, self.niter, loc)
         return Bmap
 

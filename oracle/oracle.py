@@ -528,9 +528,10 @@ class OracleSHT:
         t4 = time.perf_counter()
         return {"fft_fwd": t1 - t0, "analysis": t2 - t1, "synthesis": t3 - t2, "fft_inv": t4 - t3}
 
-    def _qu_forward(self, bq, bu, afE, afB, beta, want_e=True, want_b=True):
-        """The per-m body shared by QU2EB (:409-430) and qu2eb_iter_old (:596-617, :645-666)."""
-        for m in range(self.lmax + 1):
+    def _qu_forward(self, bq, bu, afE, afB, beta, want_e=True, want_b=True, ms=None):
+        """The per-m body shared by QU2EB (:409-430) and qu2eb_iter_old (:596-617, :645-666);
+        `ms` restricts it to some orders."""
+        for m in (range(self.lmax + 1) if ms is None else ms):
             _, P1, P2 = self.plm(m, pol=True)
             c1, c2 = self.fft2mat(bq, m)
             if want_e:
@@ -564,6 +565,17 @@ class OracleSHT:
                 self.alm2mat(afB, P1, m, 1, 1.0, c1, c2)
                 self.mat2fft_iter(bu, b4, m, c1, c2)
             self._qu_forward(b3, b4, afE, afB, 1.0)
+        return E, B
+
+    def qu2eb_orders(self, Q, U, ms):
+        """QU2EB (sht_main.f90:383-436) for the orders `ms` only (each order of
+        an unrefined transform is independent); the other orders stay 0."""
+        nsim = Q.shape[1]
+        bq = self.map2fft(Q)
+        bu = self.map2fft(U)
+        E, afE = self._new_alm(nsim)
+        B, afB = self._new_alm(nsim)
+        self._qu_forward(bq, bu, afE, afB, 0.0, ms=ms)
         return E, B
 
     def qu2e(self, Q, U):

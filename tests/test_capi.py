@@ -89,3 +89,21 @@ def test_product_does_not_import_oracle():
             if f.endswith((".py", ".hip", ".h", ".cpp")):
                 txt = open(os.path.join(dirpath, f)).read()
                 assert "oracle" not in txt.replace("reference's", "").lower() or f == "__init__.py", f
+
+
+def test_fix_eb_rejects_outputs_on_the_other_side():
+    """fix_eb with host Q/U and a torch Btpl_out / almB_out (or the reverse)
+    raises before any transfer or kernel; checked without a GPU on an object
+    that has no device handle."""
+    torch = pytest.importorskip("torch")
+    from fastsht_amd import SHT
+    S = object.__new__(SHT)
+    S.nside, S.lmax, S.nsim, S.niter, S.pol = 2, 4, 1, 0, 1
+    S.npix, S.nring = 48, 7
+    S._h, S._stream = None, None
+    Q = np.zeros((48, 1), order="F")
+    U = np.zeros((48, 1), order="F")
+    with pytest.raises(ValueError, match="Btpl_out must be a host array"):
+        S.fix_eb(Q, U, np.ones(48), Btpl_out=torch.zeros((1, 48), dtype=torch.float64).t())
+    with pytest.raises(ValueError, match="almB_out must be a host array"):
+        S.fix_eb(Q, U, np.ones(48), almB_out=torch.zeros((5, 5, 1), dtype=torch.float64).permute(2, 1, 0))

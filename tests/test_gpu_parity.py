@@ -383,6 +383,13 @@ def test_fix_eb_device_tensors(torch_cuda):
     torch.cuda.synchronize()
     assert maxrel(Bd.cpu().numpy(), ref["Bmap"]) < TOL
     assert maxrel(Qd.cpu().numpy(), ref["Qm"]) < 1e-15
+    # a host mask (NumPy or CPU tensor) with GPU maps is copied to the device
+    for host_mask in (mask, torch.from_numpy(mask)):
+        Qd.copy_(torch.from_numpy(np.asarray(Q)))
+        Ud.copy_(torch.from_numpy(np.asarray(U)))
+        Bh = sht.fix_eb(Qd, Ud, host_mask)
+        torch.cuda.synchronize()
+        assert torch.equal(Bh, Bd)
 
 
 def test_alm2cl_vs_oracle(torch_cuda):

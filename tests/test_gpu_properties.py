@@ -64,7 +64,7 @@ def test_t_roundtrip_converges_nside1024(torch_cuda):
 def test_qu_roundtrip_converges_nside1024(torch_cuda):
     torch = torch_cuda
     from fastsht_amd import SHT, _lib
-    nside, lmax, nsim = 1024, 2047, 4
+    nside, lmax, nsim = 1024, 2047, 200  # the polarised benchmark configuration
     print(f"case QU convergence nside={nside} lmax={lmax} nsim={nsim}", flush=True)
     S = SHT(nside, lmax, nsim, pol=True)
     lib = _lib.load()

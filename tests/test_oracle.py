@@ -657,3 +657,24 @@ def test_spin2_planted_sign_mistakes_are_caught(spin_tables):
         Qr, Ur = S.eb2qu(Ea, Ba)
         E3, B3 = S.qu2eb(Qr, Ur, niter=3)
         assert maxrel(E3, Ea) < 1e-2 and maxrel(B3, Ba) < 1e-2, cls.__name__
+
+
+def test_qu2eb_orders_matches_full_transform():
+    """The per-order spin-2 entry (used by the nside 1024 GPU check) equals the
+    full QU2EB on the orders it computes and leaves the others zero."""
+    ns, lmax, nsim = 8, 20, 2
+    rng = np.random.default_rng(4)
+    Q = np.asfortranarray(rng.standard_normal((12 * ns * ns, nsim)))
+    U = np.asfortranarray(rng.standard_normal((12 * ns * ns, nsim)))
+    R = O.OracleSHT(ns, lmax, nsim)
+    E, B = R.qu2eb(Q, U)
+    ms = [0, 3, 20]
+    Es, Bs = R.qu2eb_orders(Q, U, ms)
+    L = lmax + 1
+    mask = np.zeros((L, L), dtype=bool)  # storage entries of the orders in ms
+    for m in ms:
+        mask[m:, m] = True                # Re a_lm, l >= m
+        if m > 0:
+            mask[:L - m, L - m] = True    # Im a_lm at row l - m
+    assert np.array_equal(Es[:, mask], E[:, mask]) and np.array_equal(Bs[:, mask], B[:, mask])
+    assert not Es[:, ~mask].any() and not Bs[:, ~mask].any()
