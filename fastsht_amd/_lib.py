@@ -81,8 +81,8 @@ def load():
     lib.fsht_profile_read.argtypes = [H, ctypes.POINTER(ctypes.c_double), i]
     lib.fsht_norm_alm.argtypes = [dp, dp, dp, dp, d, i, i, vp]
     lib.fsht_filt_alm.argtypes = [dp, dp, i, i, vp]
-    lib.fsht_convert_alm_healpy.argtypes = [dp, dp, i, i, vp]
-    lib.fsht_convert_alm_healpy_inv.argtypes = [dp, dp, i, i, vp]
+    lib.fsht_convert_alm_healpy.argtypes = [dp, dp, i, i, i, vp]
+    lib.fsht_convert_alm_healpy_inv.argtypes = [dp, dp, i, i, i, vp]
     lib.fsht_alm2cl.argtypes = [dp, dp, dp, dp, i, i, i, vp]
     lib.fsht_host_alloc.argtypes = [ctypes.c_ulonglong, ctypes.POINTER(ctypes.c_void_p)]
     lib.fsht_host_free.argtypes = [vp]

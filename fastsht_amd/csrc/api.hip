@@ -870,16 +870,37 @@ int fsht_filt_alm(double* alm, const double* win, int lmax, int nsim, void* stre
   return e == hipSuccess ? FSHT_OK : fail(nullptr, FSHT_ERR_HIP, hipGetErrorString(e));
 }
 
-int fsht_convert_alm_healpy(const double* alm, double* alm_hp, int lmax, int nsim, void* stream) {
-  if (!alm || !alm_hp || lmax < 0 || nsim < 1) return fail(nullptr, FSHT_ERR_ARG, "bad argument");
-  hipError_t e = fsht::launch_convert_healpy(alm, alm_hp, lmax, nsim, 0, (hipStream_t)stream);
+// Square packing <-> healpy order.  Host arrays (loc = FSHT_HOST) are staged
+// through device buffers allocated for the call; the call then returns with
+// the result in `dst`.
+static int convert_healpy(const double* src, double* dst, int lmax, int nsim, int inverse, int loc, void* stream) {
+  if (!src || !dst || lmax < 0 || nsim < 1 || (loc != FSHT_HOST && loc != FSHT_DEVICE))
+    return fail(nullptr, FSHT_ERR_ARG, "bad argument");
+  const size_t L = (size_t)lmax + 1;
+  const size_t n_sq = L * L * (size_t)nsim, n_hp = L * (L + 1) * (size_t)nsim;  // doubles
+  const size_t n_src = inverse ? n_hp : n_sq, n_dst = inverse ? n_sq : n_hp;
+  hipStream_t st = (hipStream_t)stream;
+  hipError_t e;
+  if (loc == FSHT_DEVICE) {
+    e = fsht::launch_convert_healpy(src, dst, lmax, nsim, inverse, st);
+  } else {
+    DevBuf ds, dd;
+    e = hipMalloc((void**)&ds.p, n_src * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc((void**)&dd.p, n_dst * sizeof(double));
+    if (e == hipSuccess) e = hipMemcpyAsync(ds.p, src, n_src * sizeof(double), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = fsht::launch_convert_healpy(ds.p, dd.p, lmax, nsim, inverse, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(dst, dd.p, n_dst * sizeof(double), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+  }
   return e == hipSuccess ? FSHT_OK : fail(nullptr, FSHT_ERR_HIP, hipGetErrorString(e));
 }
 
-int fsht_convert_alm_healpy_inv(const double* alm_hp, double* alm, int lmax, int nsim, void* stream) {
-  if (!alm || !alm_hp || lmax < 0 || nsim < 1) return fail(nullptr, FSHT_ERR_ARG, "bad argument");
-  hipError_t e = fsht::launch_convert_healpy(alm_hp, alm, lmax, nsim, 1, (hipStream_t)stream);
-  return e == hipSuccess ? FSHT_OK : fail(nullptr, FSHT_ERR_HIP, hipGetErrorString(e));
+int fsht_convert_alm_healpy(const double* alm, double* alm_hp, int lmax, int nsim, int loc, void* stream) {
+  return convert_healpy(alm, alm_hp, lmax, nsim, 0, loc, stream);
+}
+
+int fsht_convert_alm_healpy_inv(const double* alm_hp, double* alm, int lmax, int nsim, int loc, void* stream) {
+  return convert_healpy(alm_hp, alm, lmax, nsim, 1, loc, stream);
 }
 
 int fsht_alm2cl(const double* almT, const double* almE, const double* almB, double* cl, int lmin, int lmax, int nsim,

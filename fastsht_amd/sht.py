@@ -328,23 +328,27 @@ class SHT:
 
     def convert_alm_healpy(self, alms):
         """Square packing -> healpy complex (nalm, nsim) (SHT.py:204-210,
-        convert_alm.f90:34-56), computed on the GPU."""
-        torch = _torch()
+        convert_alm.f90:34-56), computed on the GPU (host arrays are staged
+        through the device by the library)."""
         lib = _lib.load()
         L = self.lmax + 1
         nalm = L * (L + 1) // 2
-        host = not _is_torch(alms)
-        a = torch.from_numpy(np.asfortranarray(alms, dtype=np.float64)).cuda() if host else alms
-        if host:  # from_numpy keeps the strides: make the memory Fortran order on the device
-            a = a.permute(2, 1, 0).contiguous().permute(2, 1, 0)
-        if tuple(a.stride()) != _fortran_strides(self._ashape):
+        if not _is_torch(alms):
+            a = np.asfortranarray(alms, dtype=np.float64)
+            if a.shape != self._ashape:
+                raise ValueError(f"alms: expected shape {self._ashape}, got {a.shape}")
+            out = np.empty((2, nalm, self.nsim), order="F")
+            _lib.check(lib.fsht_convert_alm_healpy(a.ctypes.data, out.ctypes.data, self.lmax, self.nsim,
+                                                   _lib.FSHT_HOST, None))
+            return out[0] + 1j * out[1]
+        torch = _torch()
+        if tuple(alms.stride()) != _fortran_strides(self._ashape):
             raise ValueError("alms must be Fortran-order (nsim, lmax+1, lmax+1)")
-        out = torch.empty((self.nsim, nalm, 2), dtype=torch.float64, device=a.device)
+        out = torch.empty((self.nsim, nalm, 2), dtype=torch.float64, device=alms.device)
         s = torch.cuda.current_stream().cuda_stream
-        _lib.check(lib.fsht_convert_alm_healpy(ctypes.c_void_p(a.data_ptr()), ctypes.c_void_p(out.data_ptr()),
-                                               self.lmax, self.nsim, ctypes.c_void_p(s)))
-        res = torch.complex(out[..., 0], out[..., 1]).t()  # (nalm, nsim)
-        return res.cpu().numpy() if host else res
+        _lib.check(lib.fsht_convert_alm_healpy(ctypes.c_void_p(alms.data_ptr()), ctypes.c_void_p(out.data_ptr()),
+                                               self.lmax, self.nsim, _lib.FSHT_DEVICE, ctypes.c_void_p(s)))
+        return torch.complex(out[..., 0], out[..., 1]).t()  # (nalm, nsim)
 
     def get_alms(self, alms_in=None):
         """Fetch the alms kept on the GPU by t2alm(get_alm=False) (SHT.py:214-227)."""

@@ -182,10 +182,12 @@ int fsht_norm_alm(const double* alm_in, double* alm_out, const double* fac, cons
 int fsht_filt_alm(double* alm, const double* win, int lmax, int nsim, void* stream);
 /* Square packing -> healpy m-major order: alm_hp (2, (lmax+1)(lmax+2)/2, nsim)
  * Fortran order (Re, Im interleaved).  Replaces convert_alm_healpy1
- * (src/convert_alm.f90:34-56) used by SHT.convert_alm_healpy (SHT.py:204-210). */
-int fsht_convert_alm_healpy(const double* alm, double* alm_hp, int lmax, int nsim, void* stream);
+ * (src/convert_alm.f90:34-56) used by SHT.convert_alm_healpy (SHT.py:204-210).
+ * loc = FSHT_DEVICE: device arrays, asynchronous on `stream`; loc = FSHT_HOST:
+ * host arrays, staged through device buffers, returns with the result. */
+int fsht_convert_alm_healpy(const double* alm, double* alm_hp, int lmax, int nsim, int loc, void* stream);
 /* Inverse of the above.  Replaces convert_alm_healpy_inv1 (src/convert_alm.f90:87-107). */
-int fsht_convert_alm_healpy_inv(const double* alm_hp, double* alm, int lmax, int nsim, void* stream);
+int fsht_convert_alm_healpy_inv(const double* alm_hp, double* alm, int lmax, int nsim, int loc, void* stream);
 
 /* Angular power spectra of packed alm (device arrays): T only (almE = almB =
  * NULL): cl (nsim, lmax+1); or all of TT EE BB TE TB EB: cl (nsim, lmax+1, 6),

@@ -181,9 +181,17 @@ def test_alm_utilities_vs_oracle(torch_cuda):
     np.testing.assert_allclose(d.cpu().numpy().transpose(2, 1, 0), O.filt_alm(alm, win.cpu().numpy()), rtol=1e-15)
     hpd = torch.from_numpy(np.stack([hp.real.T, hp.imag.T], axis=-1).copy()).cuda()  # (nsim, nalm, 2)
     back = torch.empty_like(d)
-    _lib.check(lib.fsht_convert_alm_healpy_inv(p(hpd), p(back), lmax, nsim, None))
+    _lib.check(lib.fsht_convert_alm_healpy_inv(p(hpd), p(back), lmax, nsim, _lib.FSHT_DEVICE, None))
     torch.cuda.synchronize()
     np.testing.assert_array_equal(back.cpu().numpy().transpose(2, 1, 0), alm)
+    # host arrays through the C ABI (the reference's convert_alm_healpy1 call)
+    out_h = np.empty((2, L * (L + 1) // 2, nsim), order="F")
+    _lib.check(lib.fsht_convert_alm_healpy(alm.ctypes.data, out_h.ctypes.data, lmax, nsim, _lib.FSHT_HOST, None))
+    np.testing.assert_array_equal(out_h[0] + 1j * out_h[1], O.convert_alm_healpy(alm))
+    back_h = np.empty_like(alm)
+    _lib.check(lib.fsht_convert_alm_healpy_inv(out_h.ctypes.data, back_h.ctypes.data, lmax, nsim, _lib.FSHT_HOST,
+                                               None))
+    np.testing.assert_array_equal(back_h, alm)
 
 
 def _single_order_closed_form(nside, lmax, m, a):
