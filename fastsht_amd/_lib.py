@@ -25,7 +25,7 @@ FSHT_DEVICE = 1
 # every symbol declared in include/fastsht.h
 EXPORTS = (
     "fsht_version", "fsht_last_error", "fsht_create", "fsht_destroy", "fsht_get_stream",
-    "fsht_set_stream", "fsht_synchronize", "fsht_info", "fsht_ring_tab", "fsht_set_precision", "fsht_t2alm", "fsht_t2alm_iter",
+    "fsht_set_stream", "fsht_synchronize", "fsht_info", "fsht_ring_tab", "fsht_set_precision", "fsht_set_batch", "fsht_t2alm", "fsht_t2alm_iter",
     "fsht_get_alm", "fsht_alm2t", "fsht_qu2eb", "fsht_qu2e", "fsht_qu2b", "fsht_eb2qu", "fsht_fix_eb",
     "fsht_fix_eb_noalloc",
     "fsht_profile", "fsht_profile_read", "fsht_norm_alm", "fsht_filt_alm", "fsht_convert_alm_healpy", "fsht_convert_alm_healpy_inv",
@@ -67,6 +67,7 @@ def load():
     lib.fsht_info.argtypes = [H, ctypes.POINTER(ctypes.c_longlong), i]
     lib.fsht_ring_tab.argtypes = [H, ip]
     lib.fsht_set_precision.argtypes = [H, i]
+    lib.fsht_set_batch.argtypes = [H, i]
     lib.fsht_t2alm.argtypes = [H, dp, dp, i, i]
     lib.fsht_t2alm_iter.argtypes = [H, dp, dp, i, i]
     lib.fsht_get_alm.argtypes = [H, dp, i]

@@ -119,6 +119,7 @@ int fsht_create(int nside, int lmax, int nsim, int niter, int pol, fsht_handle**
   h->nside = nside;
   h->lmax = lmax;
   h->nsim = nsim;
+  h->nsim_cap = nsim;
   h->niter = niter;
   h->pol = pol ? 1 : 0;
   h->nring = 4 * nside - 1;
@@ -197,13 +198,24 @@ int fsht_set_stream(fsht_handle* H, void* stream) {
   return FSHT_OK;
 }
 
+int fsht_set_batch(fsht_handle* H, int nsim) {
+  if (!H) return fail(nullptr, FSHT_ERR_ARG, "NULL handle");
+  Handle* h = &H->h;
+  if (nsim < 1 || nsim > h->nsim_cap)
+    return fail(h, FSHT_ERR_ARG, "batch must hold 1.." + std::to_string(h->nsim_cap) + " maps");
+  h->nsim = nsim;
+  h->have_dev_alm = 0;  // kept alms were laid out for the previous batch
+  return FSHT_OK;
+}
+
 int fsht_info(const fsht_handle* H, long long* out, int n) {
   if (!H || !out) return fail(nullptr, FSHT_ERR_ARG, "NULL argument");
   const Handle* h = &H->h;
-  long long v[15] = {h->nside,    h->lmax,     h->nsim,     h->niter,        h->pol,
+  long long v[16] = {h->nside,    h->lmax,     h->nsim,     h->niter,        h->pol,
                      h->npix,     h->nsimp,    h->G_size,   h->precision,    h->mix_bytes,
-                     h->mix_k_lo, h->mix_k_hi, h->mix_damp_min, h->mix_k_out, h->mix_damp_min_a};
-  for (int i = 0; i < n && i < 15; ++i) out[i] = v[i];
+                     h->mix_k_lo, h->mix_k_hi, h->mix_damp_min, h->mix_k_out, h->mix_damp_min_a,
+                     h->nsim_cap};
+  for (int i = 0; i < n && i < 16; ++i) out[i] = v[i];
   return FSHT_OK;
 }
 
@@ -284,8 +296,9 @@ static hipError_t timed(Handle* h, int cls, F&& f) {
 // ---------------------------------------------------------------------------
 static long long alm_count(const Handle* h) { return (long long)h->nsim * (h->lmax + 1) * (h->lmax + 1); }
 
-static int ensure_dev_alm(Handle* h, int k) {
-  if (!h->ws_alm[k]) HIPCHK(h, hipMalloc((void**)&h->ws_alm[k], (size_t)alm_count(h) * sizeof(double)));
+static int ensure_dev_alm(Handle* h, int k) {  // sized for the capacity
+  if (!h->ws_alm[k])
+    HIPCHK(h, hipMalloc((void**)&h->ws_alm[k], (size_t)h->nsim_cap * (h->lmax + 1) * (h->lmax + 1) * sizeof(double)));
   return FSHT_OK;
 }
 

@@ -143,7 +143,8 @@ struct Handle {
   int nside = 0, lmax = 0, nsim = 0, niter = 0, pol = 0;
   int nring = 0, nhr = 0, nnorth = 0;  // nnorth = 2*nside rings incl. equator
   long long npix = 0;
-  int nsimp = 0;  // nsim rounded up to 16
+  int nsimp = 0;  // capacity rounded up to 16
+  int nsim_cap = 0;  // maps the handle was created for (nsim <= nsim_cap: fsht_set_batch)
   double pa = 0;
   hipStream_t stream = nullptr;      // stream all work is enqueued on
   hipStream_t own_stream = nullptr;  // stream created (and destroyed) by the handle
@@ -252,6 +253,18 @@ struct Handle {
 };
 
 // kernel classes reported by fsht_profile_read
+// Batches smaller than the handle's capacity: the m-major buffers keep nsimp
+// columns (the capacity rounded to 16) per Re/Im and component segment, of
+// which the first nsim hold maps.  True if a column of [c0, c1) holds a map.
+__device__ inline bool cols_active(int c0, int c1, int nsim, int nsimp) {
+  for (int c = c0; c < c1;) {
+    const int seg = c / nsimp, s = c - seg * nsimp;
+    if (s < nsim) return true;
+    c = (seg + 1) * nsimp;
+  }
+  return false;
+}
+
 enum KernelClass {
   KC_FFT_FWD = 0,
   KC_FFT_INV,

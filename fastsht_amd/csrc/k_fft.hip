@@ -631,6 +631,7 @@ __global__ __launch_bounds__(256) void k_to_T(const double* __restrict__ spec, d
   __shared__ double tile[TC][TP + 1];
   const long long p0 = (long long)blockIdx.x * TP;
   const int c0 = blockIdx.y * TC;  // column in [0, ncomp*nsimp)
+  if (!cols_active(c0, c0 + TC, nsim, nsimp)) return;  // no map in the tile
   const int ld = ncomp * nsimp;
   for (int e = threadIdx.x; e < TP * TC; e += 256) {
     const int r = e / TP, x = e % TP;  // column r, pixel x
@@ -656,6 +657,7 @@ __global__ __launch_bounds__(256) void k_from_T(const double* __restrict__ specT
   __shared__ double tile[TC][TP + 1];
   const long long p0 = (long long)blockIdx.x * TP;
   const int c0 = blockIdx.y * TC;
+  if (!cols_active(c0, c0 + TC, nsim, nsimp)) return;  // no map in the tile
   const int ld = ncomp * nsimp;
   for (int e = threadIdx.x; e < TP * TC; e += 256) {
     const int x = e / TC, r = e % TC;

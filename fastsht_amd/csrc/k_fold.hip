@@ -38,7 +38,8 @@ __global__ __launch_bounds__(256) void k_fold(const RingGeo* __restrict__ geo, c
                                               const double2* __restrict__ tw, const double* __restrict__ G,
                                               const long long* __restrict__ goff, const double* __restrict__ Xt,
                                               double* __restrict__ out, int residual, int lmax, int nhr,
-                                              int nnorth, int nsimp) {
+                                              int nnorth, int nsim, int nsimp) {
+  if (!cols_active(blockIdx.z * kFoldBlk.x, (blockIdx.z + 1) * kFoldBlk.x, nsim, nsimp)) return;  // no map here
   const int i = blockIdx.x;  // north ring
   const int kk = blockIdx.y * kFoldBlk.y + threadIdx.y;
   const int col = blockIdx.z * kFoldBlk.x + threadIdx.x;  // spectrum column: comp*nsimp + sim
@@ -168,7 +169,8 @@ __global__ __launch_bounds__(256) void k_gather(const RingGeo* __restrict__ geo,
                                                 const double2* __restrict__ tw, const int* __restrict__ slot,
                                                 const double* __restrict__ specT, double* __restrict__ F,
                                                 const long long* __restrict__ goff, int nhr, int nnorth,
-                                                int nsimp, int* __restrict__ fexp) {
+                                                int nsim, int nsimp, int* __restrict__ fexp) {
+  if (!cols_active(blockIdx.z * kFoldBlk.x, (blockIdx.z + 1) * kFoldBlk.x, nsim, nsimp)) return;  // no map here
   const int m = blockIdx.x;
   const int i0 = i0tab[m];
   const int ldT = NCOMP * nsimp;
@@ -229,8 +231,9 @@ __global__ __launch_bounds__(256) void k_gather_prefix(const RingGeo* __restrict
                                                        const double2* __restrict__ tw, const int* __restrict__ slot,
                                                        const double* __restrict__ G, const double* __restrict__ specT,
                                                        double* __restrict__ F, const long long* __restrict__ goff,
-                                                       int nhr, int nnorth, int nsimp, int lmax, int full,
+                                                       int nhr, int nnorth, int nsim, int nsimp, int lmax, int full,
                                                        int* __restrict__ fexp) {
+  if (!cols_active(blockIdx.z * kFoldBlk.x, (blockIdx.z + 1) * kFoldBlk.x, nsim, nsimp)) return;  // no map here
   const int m = blockIdx.x;
   const int i0 = i0tab[m];
   const int ldT = NCOMP * nsimp;
@@ -338,10 +341,10 @@ hipError_t launch_gather_prefix(Handle* h, const double* G, const double* specT,
             (ncomp * h->nsimp + kFoldBlk.x - 1) / kFoldBlk.x);
   if (ncomp == 2)
     hipLaunchKernelGGL(k_gather_prefix<2>, grid, block, 0, h->stream, h->d_geo, h->d_i0, h->d_tw, h->d_slot, G,
-                       specT, F, h->d_goff, h->nhr, h->nnorth, h->nsimp, h->lmax, full, fexp);
+                       specT, F, h->d_goff, h->nhr, h->nnorth, h->nsim, h->nsimp, h->lmax, full, fexp);
   else
     hipLaunchKernelGGL(k_gather_prefix<1>, grid, block, 0, h->stream, h->d_geo, h->d_i0, h->d_tw, h->d_slot, G,
-                       specT, F, h->d_goff, h->nhr, h->nnorth, h->nsimp, h->lmax, full, fexp);
+                       specT, F, h->d_goff, h->nhr, h->nnorth, h->nsim, h->nsimp, h->lmax, full, fexp);
   return hipGetLastError();
 }
 
@@ -352,10 +355,10 @@ hipError_t launch_gather(Handle* h, const double* specT, double* F, int* fexp) {
             (ncomp * h->nsimp + kFoldBlk.x - 1) / kFoldBlk.x);
   if (ncomp == 2)
     hipLaunchKernelGGL(k_gather<2>, grid, block, 0, h->stream, h->d_geo, h->d_i0, h->d_tw, h->d_slot, specT, F,
-                       h->d_goff, h->nhr, h->nnorth, h->nsimp, fexp);
+                       h->d_goff, h->nhr, h->nnorth, h->nsim, h->nsimp, fexp);
   else
     hipLaunchKernelGGL(k_gather<1>, grid, block, 0, h->stream, h->d_geo, h->d_i0, h->d_tw, h->d_slot, specT, F,
-                       h->d_goff, h->nhr, h->nnorth, h->nsimp, fexp);
+                       h->d_goff, h->nhr, h->nnorth, h->nsim, h->nsimp, fexp);
   return hipGetLastError();
 }
 
@@ -366,10 +369,10 @@ hipError_t launch_fold(Handle* h, const double* G, const double* specT_t, double
   dim3 grid(h->nnorth, (maxk + 1 + kFoldBlk.y - 1) / kFoldBlk.y, (ncomp * h->nsimp + kFoldBlk.x - 1) / kFoldBlk.x);
   if (ncomp == 2)
     hipLaunchKernelGGL(k_fold<2>, grid, block, 0, h->stream, h->d_geo, h->d_i0, h->d_tw, G, h->d_goff, specT_t,
-                       specT_out, residual, h->lmax, h->nhr, h->nnorth, h->nsimp);
+                       specT_out, residual, h->lmax, h->nhr, h->nnorth, h->nsim, h->nsimp);
   else
     hipLaunchKernelGGL(k_fold<1>, grid, block, 0, h->stream, h->d_geo, h->d_i0, h->d_tw, G, h->d_goff, specT_t,
-                       specT_out, residual, h->lmax, h->nhr, h->nnorth, h->nsimp);
+                       specT_out, residual, h->lmax, h->nhr, h->nnorth, h->nsim, h->nsimp);
   return hipGetLastError();
 }
 

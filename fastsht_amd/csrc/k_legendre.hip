@@ -159,6 +159,7 @@ __global__ __launch_bounds__(256, SPIN ? FSHT_ANA_MINBLK2 : FSHT_ANA_MINBLK0) vo
   __shared__ double2 scf[SPIN ? kAnaLB + 32 : 1];
   const LegItem it = items[blockIdx.x];
   if (it.m < 0) return;
+  if (!cols_active(it.colblk * 64 / KS, (it.colblk + 1) * 64 / KS, nsim, nsimp)) return;  // no map in the block
   const int m = it.m, par = it.par;
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // uniform: operand bases in scalar registers
@@ -614,6 +615,7 @@ __global__ __launch_bounds__(256, SPIN ? FSHT_SYN_MINBLK2 : FSHT_SYN_MINBLK0) vo
   __shared__ SynShared<SPIN> sh;
   const LegItem it = items[blockIdx.x];
   if (it.m < 0) return;
+  if (!cols_active(it.colblk * 16, it.colblk * 16 + 64, nsim, nsimp)) return;  // no map in the block
   const int m = it.m;
   const int i0 = i0tab[m];
   const int nr = nhr + 1 - i0;

@@ -237,8 +237,9 @@ template <int K>
 __global__ __launch_bounds__(256) void k_quant_F(const MixOff* __restrict__ mo, const int* __restrict__ i0tab,
                                                  const long long* __restrict__ goff, const double* __restrict__ F,
                                                  const int* __restrict__ FQe, signed char* __restrict__ FQ, int nhr,
-                                                 int ncols) {
+                                                 int ncols, int nsim) {
   const int m = blockIdx.x, cls = blockIdx.y & 1, kb = blockIdx.y >> 1, ct = blockIdx.z;
+  if (!cols_active(ct * 32, ct * 32 + 32, nsim, ncols / 2)) return;  // no map in the tile
   const MixOff o = mo[2 * m + cls];
   const int cl = threadIdx.x & 31, q = threadIdx.x >> 5;
   const int rs = kb * 8 + q;
@@ -273,6 +274,7 @@ __global__ __launch_bounds__(256) void k_quant_A(const MixOff* __restrict__ mo, 
   __shared__ double red[8][32];
   __shared__ int ex[32];
   const int m = blockIdx.x, cls = blockIdx.y, ct = blockIdx.z;
+  if (!cols_active(ct * 32, ct * 32 + 32, nsim, nsimp)) return;  // no map in the tile
   const MixOff o = mo[2 * m + cls];
   const int cl = threadIdx.x & 31, q = threadIdx.x >> 5;
   const int col = ct * 32 + cl;
@@ -374,6 +376,7 @@ __global__ __launch_bounds__(512, 1) void k_slice_gemm(const LegItem* __restrict
   __shared__ __align__(16) signed char st[NS][NBLK * 1024];
   const LegItem it = items[blockIdx.x];
   if (it.m < 0) return;  // whole workgroup
+  if (!cols_active(it.colblk * NB * 32, (it.colblk + 1) * NB * 32, nsim, nsimp)) return;  // no map in the block
   const int m = it.m, cls = it.par;
   const int w = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
   const int L = threadIdx.x & 63;
@@ -687,7 +690,7 @@ hipError_t launch_quant_F(Handle* h, const double* F, int K) {
   const int ncols = 2 * h->nsimp;
 #define FSHT_QF(KV)                                                                                              \
   hipLaunchKernelGGL(k_quant_F<KV>, grid, dim3(256), 0, h->stream, h->d_mix_off, h->d_i0, h->d_goff, F, h->d_FQe, \
-                     h->d_FQ, h->nhr, ncols)
+                     h->d_FQ, h->nhr, ncols, h->nsim)
   switch (K) {
     case 4: FSHT_QF(4); break;
     case 5: FSHT_QF(5); break;

@@ -114,6 +114,12 @@ class SHT:
         self._call(_lib.load().fsht_set_precision, code)
         self.precision = precision
 
+    def set_batch(self, nsim):
+        """Run the following calls on nsim maps (1 <= nsim <= the nsim the
+        object was created with); no counterpart in the reference."""
+        self._call(_lib.load().fsht_set_batch, int(nsim))
+        self.nsim = int(nsim)
+
     def __del__(self):
         h = getattr(self, "_h", None)
         if h is not None and h.value:

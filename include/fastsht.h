@@ -62,6 +62,14 @@ int fsht_get_stream(fsht_handle* h, void** stream);
 int fsht_set_stream(fsht_handle* h, void* stream);
 int fsht_synchronize(fsht_handle* h);
 
+/* Number of maps of the following calls: 1 <= nsim <= the nsim the handle was
+ * created with (its capacity).  Arrays passed to later calls hold nsim maps
+ * (leading dimension nsim); work on the capacity's unused 16-map column tiles
+ * is skipped.  No counterpart in the reference (one fixed nsim per
+ * configuration): it lets a serving loop pack requests of one shape into one
+ * resident handle.  Drops alms kept on the device by fsht_t2alm(alms = NULL). */
+int fsht_set_batch(fsht_handle* h, int nsim);
+
 /* out[0..n) = {nside, lmax, nsim, niter, pol, npix, nsim padded to 16, m-major
  * buffer size, precision, device bytes held for mixed precision, and the
  * mixed-precision slice counts: k_lo, k_hi, damp_min, k_out, damp_min_a}.  In
@@ -69,7 +77,8 @@ int fsht_synchronize(fsht_handle* h);
  * synthesis whose error the later Jacobi iterations damp at least damp_min
  * times (analysis k: niter - k, synthesis k: niter - k - 1), and a later
  * analysis damped at least damp_min_a times, use k_lo slices per operand, the
- * others k_hi; alm2t and t2alm with niter = 0 use k_out. */
+ * others k_hi; alm2t and t2alm with niter = 0 use k_out; then the capacity
+ * (the nsim the handle was created with). */
 int fsht_info(const fsht_handle* h, long long* out, int n);
 
 #define FSHT_PREC_FP64 0  /* every Legendre contraction in FP64 (the default) */
