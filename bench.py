@@ -17,6 +17,7 @@ and a bounded sample of the nside=1024 workload).
   python bench.py [--steps K] [--warmup W] [--nside 1024 --lmax 2047 --nsim 200 --niter 3]
   python bench.py --only mixed|fp64      # time one precision only (profiling runs)
   python bench.py --pol [--steps K]      # spin-2 qu2eb+eb2qu (separate line, not the value)
+  python bench.py --serve [--qps 1 --requests 40 --budget 200]   # serving latency (separate line)
 """
 from __future__ import annotations
 
@@ -360,6 +361,93 @@ def host_round_trip(sht, maps_dev, args):
                                  "maps_in": "pageable NumPy", "alms_and_maps_out": "pinned (fsht_host_alloc)"}}))
 
 
+def serve_stream(args):
+    """Serving benchmark: a seeded Poisson stream of requests at args.qps,
+    each with nside 64/256/1024/2048 (weights 0.4/0.3/0.2/0.1), lmax =
+    2 nside, nsim uniform in 1..64, T or TQU, forward or backward, FP64 or
+    mixed (each half and half), served by fastsht_amd.serve.Server under an
+    HBM budget.  Inputs are generated on the GPU at each request's arrival
+    (white maps, or alm with C_l = 1/(l(l+1))).  Prints one JSON line with
+    p50/p99 latency overall and per nside and the served throughput."""
+    import numpy as np
+    import torch
+
+    from fastsht_amd.serve import GB, Request, Server
+
+    rng = np.random.default_rng(args.seed)
+    n = args.requests
+    arrivals = np.cumsum(rng.exponential(1.0 / args.qps, n))
+    spec = [dict(nside=int(rng.choice([64, 256, 1024, 2048], p=[0.4, 0.3, 0.2, 0.1])),
+                 nsim=int(rng.integers(1, 65)), pol=bool(rng.random() < 0.5),
+                 direction="forward" if rng.random() < 0.5 else "backward",
+                 precision="mixed" if rng.random() < 0.5 else "fp64") for _ in range(n)]
+
+    def inputs(i):
+        sp = spec[i]
+        ns, m = sp["nside"], sp["nsim"]
+        L, npix = 2 * ns + 1, 12 * ns * ns
+        g = torch.Generator(device="cuda")
+        g.manual_seed(1000 + i)
+        k = 3 if sp["pol"] else 1
+        if sp["direction"] == "forward":
+            return tuple(torch.randn((m, npix), dtype=torch.float64, device="cuda", generator=g).t() for _ in range(k))
+        ell = torch.arange(L, dtype=torch.float64, device="cuda")
+        sd = torch.zeros(L, dtype=torch.float64, device="cuda")
+        sd[2:] = (1.0 / (ell[2:] * (ell[2:] + 1.0))).sqrt()
+        return tuple((torch.randn((L, L, m), dtype=torch.float64, device="cuda", generator=g) * sd[None, :, None])
+                     .permute(2, 1, 0) for _ in range(k))
+
+    srv = Server(hbm_budget=args.budget * GB, niter=args.niter)
+    # warm-up (untimed): one small request per shape and precision builds the
+    # lanes' resident tables, as a serving process does before taking traffic
+    for ns in (64, 256, 1024, 2048):
+        for prec in ("fp64", "mixed"):
+            x = torch.randn((1, 12 * ns * ns), dtype=torch.float64, device="cuda").t()
+            srv.submit(Request(nside=ns, lmax=2 * ns, direction="forward", inputs=(x, x, x), pol=True,
+                               precision=prec))
+            srv.run_until_idle()
+    torch.cuda.synchronize()
+    srv.batches = 0
+    srv._t0 = time.perf_counter()
+    reqs, i, done = [], 0, []
+    while i < n or srv.busy():
+        now = srv.clock()
+        while i < n and arrivals[i] <= now:
+            sp = spec[i]
+            r = srv.submit(Request(nside=sp["nside"], lmax=2 * sp["nside"], direction=sp["direction"],
+                                   inputs=inputs(i), pol=sp["pol"], precision=sp["precision"]))
+            r.arrival = float(arrivals[i])  # latency counts from the scheduled arrival
+            reqs.append(r)
+            i += 1
+        got = srv.poll()
+        for r in got:  # the client takes its result and lets the buffers go
+            r.inputs = r.outputs = None
+        done += got
+        if not got:
+            time.sleep(0.0005)
+    makespan = srv.clock()
+    srv.close()
+    lat = sorted(r.latency for r in done)
+    maps = sum(r.nsim * (3 if r.pol else 1) for r in done)
+
+    def pcts(v):
+        v = sorted(v)
+        return {"p50": pct(v, 50) * 1e3, "p99": pct(v, 99) * 1e3, "n": len(v)}
+    print(json.dumps({
+        "metric": "serving latency (ms) at a fixed request rate (not the benchmark value)",
+        "qps": args.qps, "requests": n, "budget_GB": args.budget, "niter": args.niter,
+        "latency_ms": pcts(lat), "latency_ms_by_nside": {str(k): pcts([r.latency for r in done if r.nside == k])
+                                                          for k in (64, 256, 1024, 2048)},
+        "served_maps_per_s": maps / makespan, "makespan_s": makespan, "batches": srv.batches,
+        "lane_closures": srv.evictions,
+        "mixed_served_as_fp64": sum(1 for r in done if r.precision == "mixed" and r.precision_used == "fp64"),
+        "config": {"workload": "Poisson stream: nside 64/256/1024/2048 (0.4/0.3/0.2/0.1), lmax=2 nside, nsim 1-64, "
+                               "T or TQU, forward (t2alm/qu2eb niter) or backward (alm2t/eb2qu), FP64 or mixed; "
+                               "maps counted per component"},
+        "data": "synthetic, generated on the GPU at arrival (white maps; alm with C_l = 1/(l(l+1)))",
+    }))
+
+
 def errors_by_niter(sht, lib, maps, niter):
     """Mixed t2alm against FP64 t2alm on the same handle and maps, for every
     niter 0..niter (max |alm_mixed - alm_fp64| / max |alm_fp64|), and the
@@ -467,6 +555,12 @@ def main():
     ap.add_argument("--pol", action="store_true",
                     help="time the spin-2 round trip qu2eb(niter)+eb2qu instead and report per-niter E/B errors "
                          "(reported separately from the temperature value)")
+    ap.add_argument("--serve", action="store_true",
+                    help="serve a seeded Poisson stream of mixed requests instead and report p50/p99 latency "
+                         "(reported separately from the benchmark value)")
+    ap.add_argument("--qps", type=float, default=1.0, help="request rate of --serve")
+    ap.add_argument("--requests", type=int, default=40, help="requests of --serve")
+    ap.add_argument("--budget", type=float, default=200.0, help="HBM budget of --serve, GB")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device copy of MIB MiB (HBM-counter calibration under rocprofv3)")
     args = ap.parse_args()
@@ -497,6 +591,9 @@ def main():
     torch.cuda.set_device(0)
     if args.pol:
         pol_round_trip(args, lmax)
+        return
+    if args.serve:
+        serve_stream(args)
         return
     sht = SHT(args.nside, lmax, args.nsim, niter=args.niter)
     alm, maps = make_inputs(sht, args.seed)

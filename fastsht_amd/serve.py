@@ -68,11 +68,14 @@ class Request:
     precision_used: str = None
     maps_done: int = 0
     ready: object = None  # event after which the inputs are valid on the device
+    n: int = 0            # maps (set at submission from the inputs)
 
     @property
     def nsim(self):
-        x = self.inputs[0]
-        return int(x.shape[1] if self.direction == "forward" else x.shape[0])
+        if not self.n:
+            x = self.inputs[0]
+            self.n = int(x.shape[1] if self.direction == "forward" else x.shape[0])
+        return self.n
 
     @property
     def done(self):
@@ -94,9 +97,9 @@ def estimate_lane_bytes(nside, lmax, cap, mixed=False):
     Legendre tables and operands of mixed precision); the measured footprint
     replaces it once the lane exists."""
     npix, L, nsp = 12 * nside ** 2, lmax + 1, (cap + 15) // 16 * 16
-    ws = 8 * npix * (2 * cap + 2 * nsp + 2 * 2 * 1.5 * 2 * nsp)
+    ws = 8 * npix * (2 * cap + 8 * nsp)
     io = 8 * 3 * cap * (npix + L * L)
-    tables = 64 * L * L + 8 * npix
+    tables = 64 * L * L + 8 * npix + 0.3e9
     mix = (12 * 0.5 * L * L * 2 * nside + 6 * 2 * nsp * L * (2 * nside + L / 2)) if mixed else 0
     return ws + io + tables + mix
 
@@ -233,8 +236,8 @@ class Lane:
 class Server:
     """hbm_budget: device bytes the loop may hold (lanes, batch buffers,
     request inputs and outputs).  capacity(nside, lmax) -> maps per
-    batch (default: 64, fewer where a lane's estimate would pass a quarter of
-    the budget).  mixed_max_nside: largest nside whose mixed-precision tables
+    batch (default: 64, down to 16 where a lane's estimate would pass a
+    quarter of the budget).  mixed_max_nside: largest nside whose mixed-precision tables
     are built; mixed requests above it run in FP64 (precision_used says so)."""
     hbm_budget: float = 200 * GB
     niter: int = 3
@@ -256,9 +259,9 @@ class Server:
     def cap_for(self, nside, lmax):
         if self.capacity is not None:
             return int(self.capacity(nside, lmax))
-        cap = 64
-        while cap > 1 and estimate_lane_bytes(nside, lmax, cap) > self.hbm_budget / 4:
-            cap //= 2
+        cap = 64  # halved while a lane would pass a quarter of the budget, down to 16
+        while cap > 16 and estimate_lane_bytes(nside, lmax, cap) > self.hbm_budget / 4:
+            cap //= 2  # (below 16 maps the padded workspace no longer shrinks)
         return cap
 
     def lane_bytes(self):
@@ -267,6 +270,8 @@ class Server:
     def submit(self, req: Request):
         """Queue a request; its inputs must be device tensors of its shape."""
         req.rid = next(self._ids)
+        req.n = 0
+        _ = req.nsim  # fixed from the inputs now: the client may drop them once done
         req.arrival = self.clock()
         if self.device_events:
             req.ready = _torch().cuda.Event()
