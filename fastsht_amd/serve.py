@@ -28,7 +28,9 @@ serves every shape:
 
 Every map of a batch is transformed independently of the others (the
 contractions are per column, the mixed-precision exponents per column), so a
-request's result is the same bit for bit alone and in any mix.
+request's result does not depend on what it is batched with, up to rounding:
+the FP64 analysis sums a map's rings in another order when the map lands in
+another kind of column block (about 1e-16 relative).
 """
 from __future__ import annotations
 
