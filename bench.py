@@ -18,6 +18,7 @@ and a bounded sample of the nside=1024 workload).
   python bench.py --only mixed|fp64      # time one precision only (profiling runs)
   python bench.py --pol [--steps K]      # spin-2 qu2eb+eb2qu (separate line, not the value)
   python bench.py --serve [--qps 1 --requests 40 --budget 200]   # serving latency (separate line)
+  python bench.py --group [--group-reps 2]   # uneven 24-job batch, grouped graph vs per-job (separate line)
 """
 from __future__ import annotations
 
@@ -361,6 +362,120 @@ def host_round_trip(sht, maps_dev, args):
                                  "maps_in": "pageable NumPy", "alms_and_maps_out": "pinned (fsht_host_alloc)"}}))
 
 
+def c3_jobs(seed, n=24):
+    """The uneven batch: n jobs from a seeded generator, nside in {128, 256,
+    512}, lmax in {2 nside, 3 nside - 1}, nsim uniform in 16..400, T or TQU."""
+    import numpy as np
+
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        ns = int(rng.choice([128, 256, 512]))
+        lmax = int(rng.choice([2 * ns, 3 * ns - 1]))
+        out.append((ns, lmax, int(rng.integers(16, 401)), bool(rng.integers(0, 2))))
+    return out
+
+
+def group_waves(specs, niter, budget):
+    """Consecutive jobs packed into waves whose inputs, outputs and handles
+    (one per shape, sized for its largest job) fit the byte budget."""
+    from fastsht_amd.group import estimate_handle_bytes, job_bytes
+
+    waves, cur = [], []
+
+    def need(js):
+        caps = {}
+        for ns, lmax, n, pol in js:
+            c, p = caps.get((ns, lmax), (0, False))
+            caps[(ns, lmax)] = (max(c, n), p or pol)
+        return (sum(job_bytes(*j) for j in js)
+                + sum(estimate_handle_bytes(k[0], k[1], c, p, niter) for k, (c, p) in caps.items()))
+
+    for j in specs:
+        if cur and need(cur + [j]) > budget:
+            waves.append(cur)
+            cur = []
+        cur.append(j)
+    waves.append(cur)
+    return waves, [need(w) for w in waves]
+
+
+def group_bench(args):
+    """Uneven batch: the 24 seeded jobs run forward (t2alm(niter) + qu2eb for
+    TQU jobs, FP64) on white-noise maps, as one Group per memory wave.  Times
+    each wave as per-job launches (one stream, jobs one after another, no
+    graph: what a caller looping over SHT objects gets) and as the tuned
+    group (forked streams, one HIP graph)."""
+    import torch
+
+    from fastsht_amd.group import Group, Job
+
+    specs = c3_jobs(args.seed)
+    free = torch.cuda.mem_get_info()[0]
+    waves, need = group_waves(specs, args.niter, 0.8 * free)
+    log = lambda m: print(m, file=sys.stderr, flush=True)
+    log(f"group: {len(specs)} jobs in {len(waves)} waves, estimated {[round(b / 1e9, 1) for b in need]} GB "
+        f"of {free / 1e9:.1f} GB free")
+    res = []
+    for w, wj in enumerate(waves):
+        g = torch.Generator(device="cuda")
+        g.manual_seed(args.seed + w)
+        jobs = []
+        for ns, lmax, n, pol in wj:
+            L, npix, c = lmax + 1, 12 * ns * ns, 3 if pol else 1
+            ins = tuple(torch.randn((n, npix), dtype=torch.float64, device="cuda", generator=g).t() for _ in range(c))
+            outs = tuple(torch.empty((L, L, n), dtype=torch.float64, device="cuda").permute(2, 1, 0)
+                         for _ in range(c))
+            jobs.append(Job(ns, lmax, "forward", ins, outs))
+        t0 = time.perf_counter()
+        G = Group(jobs, niter=args.niter, streams=4)
+        setup = time.perf_counter() - t0
+        log(f"wave {w}: {len(jobs)} jobs, {len(G.shapes)} shapes, handles set up in {setup:.1f} s")
+
+        def timed_eager(plan, reps):
+            G._bind_plan(plan)
+            G._enqueue(plan)  # untimed first run
+            torch.cuda.synchronize()
+            ts = []
+            for _ in range(reps):
+                a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                a.record()
+                G._enqueue(plan)
+                b.record()
+                b.synchronize()
+                ts.append(a.elapsed_time(b))
+            return sorted(ts)[len(ts) // 2]
+
+        per_job = timed_eager([list(G.shapes)], args.group_reps)
+        sums = lambda: [(float(y.sum()), float(y.abs().sum())) for j in jobs for y in j.outputs]
+        ref = sums()  # checksums of the per-job results
+        log(f"wave {w}: per-job launches {per_job:.1f} ms")
+        eager4 = timed_eager(G._plan(4), args.group_reps)
+        log(f"wave {w}: grouped, 4 streams, eager {eager4:.1f} ms")
+        tried = G.tune(reps=1, orders=("work",), log=log)
+        grouped = G.time_graph(G.graph, args.group_reps)
+        same = sums() == ref
+        log(f"wave {w}: grouped graph {grouped:.1f} ms, outputs equal to per-job: {same}")
+        maps = sum(n * (3 if pol else 1) for _, _, n, pol in wj)
+        res.append({"jobs": [list(j) for j in wj], "maps": maps, "setup_s": setup, "per_job_ms": per_job,
+                    "grouped_eager_4_streams_ms": eager4, "grouped_graph_ms": grouped,
+                    "tuner": [{"streams": s, "order": o, "ms": ms} for s, o, ms in tried],
+                    "plan": [[list(k) for k in p] for p in G.plan], "outputs_equal": same})
+        G.close()
+        del jobs, ref, G
+        torch.cuda.empty_cache()
+    maps = sum(r["maps"] for r in res)
+    t_job = sum(r["per_job_ms"] for r in res) / 1e3
+    t_grp = sum(r["grouped_graph_ms"] for r in res) / 1e3
+    print(json.dumps({
+        "metric": "uneven 24-job batch maps/s, grouped HIP graph vs per-job launches (not the benchmark value)",
+        "value": maps / t_grp, "unit": "maps/s (T, Q and U maps counted)", "per_job_maps_per_s": maps / t_job,
+        "grouped_vs_per_job": t_job / t_grp, "niter": args.niter, "dtype": "f64", "waves": res,
+        "data": "white-noise maps N(0,1), seeded", "config": {"workload": "24 seeded jobs: nside 128/256/512, "
+                                                              "lmax 2 nside or 3 nside - 1, nsim 16-400, T or TQU, "
+                                                              "t2alm(niter) + qu2eb"}}))
+
+
 def serve_stream(args):
     """Serving benchmark: a seeded Poisson stream of requests at args.qps,
     each with nside 64/256/1024/2048 (weights 0.4/0.3/0.2/0.1), lmax =
@@ -558,6 +673,9 @@ def main():
     ap.add_argument("--serve", action="store_true",
                     help="serve a seeded Poisson stream of mixed requests instead and report p50/p99 latency "
                          "(reported separately from the benchmark value)")
+    ap.add_argument("--group", action="store_true",
+                    help="uneven 24-job batch: grouped HIP graph vs per-job launches (reported separately)")
+    ap.add_argument("--group-reps", type=int, default=2, help="timed runs per schedule of --group")
     ap.add_argument("--qps", type=float, default=1.0, help="request rate of --serve")
     ap.add_argument("--requests", type=int, default=40, help="requests of --serve")
     ap.add_argument("--budget", type=float, default=200.0, help="HBM budget of --serve, GB")
@@ -594,6 +712,9 @@ def main():
         return
     if args.serve:
         serve_stream(args)
+        return
+    if args.group:
+        group_bench(args)
         return
     sht = SHT(args.nside, lmax, args.nsim, niter=args.niter)
     alm, maps = make_inputs(sht, args.seed)

@@ -194,7 +194,7 @@ int fsht_set_stream(fsht_handle* H, void* stream) {
   if (!H) return fail(nullptr, FSHT_ERR_ARG, "NULL handle");
   Handle* h = &H->h;
   HIPCHK(h, hipStreamSynchronize(h->stream));
-  h->stream = stream ? (hipStream_t)stream : h->own_stream;
+  h->stream = (hipStream_t)stream;  // NULL: the null stream, ordered with every blocking stream
   return FSHT_OK;
 }
 

@@ -393,8 +393,8 @@ __device__ inline int brev(int x, int lg) { return (int)(__brev((unsigned)x) >> 
 // in / out point at the ring's first pixel (PERM spectrum or pixels).
 template <int M>
 __device__ void ring_fft_half(double2* buf, int n, const double2* __restrict__ w, const double2* __restrict__ fl,
-                              const double2* __restrict__ tw, int Mmax, const double* __restrict__ in,
-                              double* __restrict__ out, int inverse) {
+                              const double2* __restrict__ tw, int Mmax, const double* in, double* out,
+                              int inverse) {
   const int L = n >> 1;
   const double rn = 2.0 / (double)n;
   if (!inverse) {
@@ -453,13 +453,15 @@ __device__ void ring_fft_half(double2* buf, int n, const double2* __restrict__ w
 
 // Forward: maps (sim-major, npix per map) -> spec (same layout, PERM per ring).
 // Inverse: spec -> maps.  grid.x = map index, grid.y = job within the class.
+// Maps staged from the host are transformed in place (in == out), so the map
+// and spectrum pointers are not restrict-qualified.
 template <int MAXM>
 __global__ __launch_bounds__(FftShape<MAXM>::TPB) void k_ring_fft(const FftJob* __restrict__ jobs, int job0,
                                                   const RingGeo* __restrict__ geo, const BlueInfo* __restrict__ blue,
                                                   const double2* __restrict__ chirp, const double2* __restrict__ filt,
                                                   const double2* __restrict__ tw, int Mmax, long long npix, int nsim,
-                                                  const double* __restrict__ in0, const double* __restrict__ in1,
-                                                  double* __restrict__ out0, double* __restrict__ out1, int inverse,
+                                                  const double* in0, const double* in1, double* out0,
+                                                  double* out1, int inverse,
                                                   int map0) {
   __shared__ double2 buf[MAXM + MAXM / 16];
   const FftJob jb = jobs[job0 + blockIdx.y];

@@ -94,10 +94,10 @@ class Request:
 
 
 def estimate_lane_bytes(nside, lmax, cap, mixed=False):
-    """Device bytes of a lane before it exists (an upper estimate: spin-2
-    workspace, batch buffers of three components, tables, and the int8
-    Legendre tables and operands of mixed precision); the measured footprint
-    replaces it once the lane exists."""
+    """Device bytes of a lane (an upper estimate: spin-2 workspace, batch
+    buffers of three components, tables, and the int8 Legendre tables and
+    operands of mixed precision).  A lane is charged the larger of this
+    estimate and its measured footprint, so admission and accounting agree."""
     npix, L, nsp = 12 * nside ** 2, lmax + 1, (cap + 15) // 16 * 16
     ws = 8 * npix * (2 * cap + 8 * nsp)
     io = 8 * 3 * cap * (npix + L * L)
@@ -112,15 +112,18 @@ class Lane:
         self.nside, self.lmax, self.cap = nside, lmax, cap
         self.npix, self.L = 12 * nside ** 2, lmax + 1
         self.stream = torch.cuda.Stream()
+        torch.cuda.synchronize()
         free0 = torch.cuda.mem_get_info()[0]
         self.sht = SHT(nside, lmax, cap, niter=server.niter, pol=True)
         self.lib = _lib.load()
         _lib.check(self.lib.fsht_set_stream(self.sht._h, ctypes.c_void_p(self.stream.cuda_stream)), self.sht._h)
+        torch.cuda.synchronize()
+        lib_bytes = free0 - torch.cuda.mem_get_info()[0]  # the library allocates outside torch's cache
         with torch.cuda.stream(self.stream):  # batch buffers: T, then Q/U or E/B
             self.maps = torch.empty((3, self.npix * cap), dtype=torch.float64, device="cuda")
             self.alms = torch.empty((3, cap * self.L * self.L), dtype=torch.float64, device="cuda")
-        torch.cuda.synchronize()
-        self.footprint = free0 - torch.cuda.mem_get_info()[0]
+        measured = lib_bytes + 8 * (self.maps.numel() + self.alms.numel())
+        self.footprint = max(measured, estimate_lane_bytes(nside, lmax, cap))
         self.mixed_ready = False
         self.pending = deque()  # requests (or their remaining maps) in arrival order
         self.inflight = None    # (event, chunks, precision)
@@ -135,10 +138,13 @@ class Lane:
 
     def enable_mixed(self):
         torch = _torch()
+        torch.cuda.synchronize()
         free0 = torch.cuda.mem_get_info()[0]
         self.sht.set_precision("mixed")
         torch.cuda.synchronize()
-        self.footprint += free0 - torch.cuda.mem_get_info()[0]
+        est = (estimate_lane_bytes(self.nside, self.lmax, self.cap, True)
+               - estimate_lane_bytes(self.nside, self.lmax, self.cap))
+        self.footprint += max(free0 - torch.cuda.mem_get_info()[0], est)
         self.mixed_ready = True
 
     def take_batch(self):

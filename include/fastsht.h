@@ -57,8 +57,11 @@ int fsht_destroy(fsht_handle* h);
 
 /* The handle's HIP stream (hipStream_t) and a blocking wait on it. */
 int fsht_get_stream(fsht_handle* h, void** stream);
-/* Enqueue this handle's work on a caller stream (hipStream_t; NULL restores the
- * handle's own stream).  The caller keeps ownership of the stream. */
+/* Enqueue this handle's work on a caller stream (hipStream_t; NULL is the null
+ * stream, as torch's default stream is).  A handle starts on a stream of its
+ * own, which fsht_get_stream returns before the first switch; handles on
+ * different non-null streams are not ordered with each other.  The caller
+ * keeps ownership of the stream. */
 int fsht_set_stream(fsht_handle* h, void* stream);
 int fsht_synchronize(fsht_handle* h);
 

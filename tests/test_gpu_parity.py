@@ -451,6 +451,35 @@ def test_t2alm_iter_vs_oracle(torch_cuda, ns, lmax, nsim, niter):
     assert maxrel(sht.t2alm(maps), R.t2alm(maps, niter=niter)) < TOL
 
 
+def test_sht_objects_run_on_torch_current_stream(torch_cuda):
+    """Device calls are enqueued on torch's current stream, the default
+    (null) stream included, so one SHT object's output is complete before
+    another object's call reads it: a long t2alm on one object, read at once
+    by a second object, gives what the second object gives after a full
+    synchronisation."""
+    torch = torch_cuda
+    import contextlib
+    import ctypes
+
+    from fastsht_amd import SHT, _lib
+    lib = _lib.load()
+    ns, lmax, n = 256, 511, 40
+    print(f"case stream ordering nside={ns} lmax={lmax} nsim={n}", flush=True)
+    rng = np.random.default_rng(29)
+    maps = torch.from_numpy(rng.standard_normal((n, 12 * ns * ns))).cuda().t()
+    A, B = SHT(ns, lmax, n, niter=3), SHT(ns, lmax, n, niter=0)
+    for side in (None, torch.cuda.Stream()):
+        with (torch.cuda.stream(side) if side is not None else contextlib.nullcontext()):
+            a = A.t2alm(maps)
+            m = B.alm2t(a)  # enqueued right behind A's t2alm
+            for S in (A, B):
+                st = ctypes.c_void_p()
+                _lib.check(lib.fsht_get_stream(S._h, ctypes.byref(st)), S._h)
+                assert (st.value or 0) == torch.cuda.current_stream().cuda_stream
+        torch.cuda.synchronize()
+        assert torch.equal(m, B.alm2t(a))
+
+
 def test_repeated_calls_bit_identical(torch_cuda):
     """Every kernel reads only data written before it in the same workgroup
     or launch: repeated transforms of the same device input agree bit for bit,
