@@ -362,7 +362,7 @@ def host_round_trip(sht, maps_dev, args):
                                  "maps_in": "pageable NumPy", "alms_and_maps_out": "pinned (fsht_host_alloc)"}}))
 
 
-def c3_jobs(seed, n=24):
+def c3_jobs(seed, n=24, nsides=(128, 256, 512), nsim=(16, 400)):
     """The uneven batch: n jobs from a seeded generator, nside in {128, 256,
     512}, lmax in {2 nside, 3 nside - 1}, nsim uniform in 16..400, T or TQU."""
     import numpy as np
@@ -370,9 +370,9 @@ def c3_jobs(seed, n=24):
     rng = np.random.default_rng(seed)
     out = []
     for _ in range(n):
-        ns = int(rng.choice([128, 256, 512]))
+        ns = int(rng.choice(list(nsides)))
         lmax = int(rng.choice([2 * ns, 3 * ns - 1]))
-        out.append((ns, lmax, int(rng.integers(16, 401)), bool(rng.integers(0, 2))))
+        out.append((ns, lmax, int(rng.integers(nsim[0], nsim[1] + 1)), bool(rng.integers(0, 2))))
     return out
 
 
@@ -400,20 +400,16 @@ def group_waves(specs, niter, budget):
     return waves, [need(w) for w in waves]
 
 
-def group_bench(args):
-    """Uneven batch: the 24 seeded jobs run forward (t2alm(niter) + qu2eb for
-    TQU jobs, FP64) on white-noise maps, as one Group per memory wave.  Times
-    each wave as per-job launches (one stream, jobs one after another, no
-    graph: what a caller looping over SHT objects gets) and as the tuned
-    group (forked streams, one HIP graph)."""
+def run_group_waves(specs, args, log):
+    """Each memory wave of `specs` as per-job launches (one stream, jobs one
+    after another, no graph: what a caller looping over SHT objects gets),
+    as the group eager on 4 streams, and as the tuned group graph."""
     import torch
 
     from fastsht_amd.group import Group, Job
 
-    specs = c3_jobs(args.seed)
     free = torch.cuda.mem_get_info()[0]
     waves, need = group_waves(specs, args.niter, 0.8 * free)
-    log = lambda m: print(m, file=sys.stderr, flush=True)
     log(f"group: {len(specs)} jobs in {len(waves)} waves, estimated {[round(b / 1e9, 1) for b in need]} GB "
         f"of {free / 1e9:.1f} GB free")
     res = []
@@ -449,28 +445,43 @@ def group_bench(args):
         per_job = timed_eager([list(G.shapes)], args.group_reps)
         sums = lambda: [(float(y.sum()), float(y.abs().sum())) for j in jobs for y in j.outputs]
         ref = sums()  # checksums of the per-job results
-        log(f"wave {w}: per-job launches {per_job:.1f} ms")
+        log(f"wave {w}: per-job launches {per_job:.2f} ms")
         eager4 = timed_eager(G._plan(4), args.group_reps)
-        log(f"wave {w}: grouped, 4 streams, eager {eager4:.1f} ms")
-        tried = G.tune(reps=1, orders=("work",), log=log)
+        log(f"wave {w}: grouped, 4 streams, eager {eager4:.2f} ms")
+        tried = G.tune(reps=args.group_reps, orders=("work",), log=log)
         grouped = G.time_graph(G.graph, args.group_reps)
         same = sums() == ref
-        log(f"wave {w}: grouped graph {grouped:.1f} ms, outputs equal to per-job: {same}")
+        log(f"wave {w}: grouped graph {grouped:.2f} ms, outputs equal to per-job: {same}")
         maps = sum(n * (3 if pol else 1) for _, _, n, pol in wj)
         res.append({"jobs": [list(j) for j in wj], "maps": maps, "setup_s": setup, "per_job_ms": per_job,
                     "grouped_eager_4_streams_ms": eager4, "grouped_graph_ms": grouped,
                     "tuner": [{"streams": s, "order": o, "ms": ms} for s, o, ms in tried],
                     "plan": [[list(k) for k in p] for p in G.plan], "outputs_equal": same})
         G.close()
-        del jobs, ref, G
+        del jobs, G
         torch.cuda.empty_cache()
     maps = sum(r["maps"] for r in res)
     t_job = sum(r["per_job_ms"] for r in res) / 1e3
     t_grp = sum(r["grouped_graph_ms"] for r in res) / 1e3
+    return {"maps_per_s": maps / t_grp, "per_job_maps_per_s": maps / t_job, "grouped_vs_per_job": t_job / t_grp,
+            "waves": res}
+
+
+def group_bench(args):
+    """Uneven batch: the 24 seeded jobs run forward (t2alm(niter) + qu2eb for
+    TQU jobs, FP64) on white-noise maps, as one Group per memory wave, against
+    per-job launches.  A second set of 24 small jobs (nside 32/64/128, 1-16
+    maps) shows the regime where one job does not fill the device."""
+    log = lambda m: print(m, file=sys.stderr, flush=True)
+    big = run_group_waves(c3_jobs(args.seed), args, log)
+    small = run_group_waves(c3_jobs(args.seed, nsides=(32, 64, 128), nsim=(1, 16)), args, log)
     print(json.dumps({
         "metric": "uneven 24-job batch maps/s, grouped HIP graph vs per-job launches (not the benchmark value)",
-        "value": maps / t_grp, "unit": "maps/s (T, Q and U maps counted)", "per_job_maps_per_s": maps / t_job,
-        "grouped_vs_per_job": t_job / t_grp, "niter": args.niter, "dtype": "f64", "waves": res,
+        "value": big["maps_per_s"], "unit": "maps/s (T, Q and U maps counted)",
+        "per_job_maps_per_s": big["per_job_maps_per_s"], "grouped_vs_per_job": big["grouped_vs_per_job"],
+        "niter": args.niter, "dtype": "f64", "waves": big["waves"],
+        "small_jobs": dict(small, config="24 seeded jobs: nside 32/64/128, lmax 2 nside or 3 nside - 1, "
+                                         "nsim 1-16, T or TQU"),
         "data": "white-noise maps N(0,1), seeded", "config": {"workload": "24 seeded jobs: nside 128/256/512, "
                                                               "lmax 2 nside or 3 nside - 1, nsim 16-400, T or TQU, "
                                                               "t2alm(niter) + qu2eb"}}))
@@ -675,7 +686,7 @@ def main():
                          "(reported separately from the benchmark value)")
     ap.add_argument("--group", action="store_true",
                     help="uneven 24-job batch: grouped HIP graph vs per-job launches (reported separately)")
-    ap.add_argument("--group-reps", type=int, default=2, help="timed runs per schedule of --group")
+    ap.add_argument("--group-reps", type=int, default=3, help="timed runs per schedule of --group")
     ap.add_argument("--qps", type=float, default=1.0, help="request rate of --serve")
     ap.add_argument("--requests", type=int, default=40, help="requests of --serve")
     ap.add_argument("--budget", type=float, default=200.0, help="HBM budget of --serve, GB")
