@@ -39,7 +39,7 @@ PEAK_FP64_MFMA_TFLOPS = 78.6
 PEAK_INT8_MFMA_TOPS = 5000.0
 PEAK_HBM_GBS = 8000.0
 KERNEL_CLASSES = ["ring_fft_fwd", "ring_fft_inv", "transpose", "legendre_analysis", "legendre_synthesis",
-                  "alias_fold", "spectrum_gather", "operand_slicing"]
+                  "alias_fold", "spectrum_gather", "operand_slicing", "table_generation"]
 NCLS = len(KERNEL_CLASSES)
 LEGENDRE = ("legendre_analysis", "legendre_synthesis")
 
@@ -61,19 +61,22 @@ def slice_products(k):
     return k * (k + 1) // 2
 
 
-def slice_ladder(info, niter):
-    """Slices per pass of a mixed t2alm(niter) + alm2t (the library's rule,
-    reported by fsht_info): analysis pass k is damped by niter - k later
-    Jacobi iterations, residual synthesis k by niter - k - 1; the first
-    analysis and syntheses damped at least damp_min times and the later
-    analyses damped at least damp_min_a times use k_lo slices, the others
-    k_hi; alm2t and an unrefined t2alm (niter = 0) use k_out."""
-    k_lo, k_hi, dmin, k_out, dmin_a = (int(v) for v in info[10:15])
-    ana = [k_lo if niter - k >= (dmin if k == 0 else dmin_a) else k_hi
-           for k in range(niter + 1)] if niter > 0 else [k_out]
-    syn = [k_lo if niter - k - 1 >= dmin else k_hi for k in range(niter)]
-    return {"analysis": ana, "residual_synthesis": syn, "alm2t": k_out,
-            "k_lo": k_lo, "k_hi": k_hi, "damp_min": dmin, "damp_min_a": dmin_a}
+def slice_ladder(info, niter, spin=0):
+    """Slices per pass of a mixed t2alm(niter) + alm2t (spin 0) or
+    qu2eb(niter) + eb2qu (spin 2), from the library's table (fsht_info
+    out[16..36)): pass kind k (0 the first analysis, 1 a residual synthesis, 2
+    a residual analysis) whose error d = min(2, damping) later Jacobi
+    iterations damp uses table[k][d] slices; analysis pass k is damped by
+    niter - k iterations, residual synthesis k by niter - k - 1; the output
+    synthesis and an unrefined analysis use `out`."""
+    b = 16 if spin == 0 else 26
+    tab = [[int(info[b + 3 * k + d]) for d in range(3)] for k in range(3)]
+    out = int(info[b + 9])
+    d = lambda x: min(2, x)
+    ana = [out] if niter == 0 else [tab[0][d(niter)]] + [tab[2][d(niter - k - 1)] for k in range(niter)]
+    syn = [tab[1][d(niter - k - 1)] for k in range(niter)]
+    return {"analysis": ana, "residual_synthesis": syn, ("alm2t" if spin == 0 else "eb2qu"): out,
+            "table": {"first_analysis": tab[0], "residual_synthesis": tab[1], "residual_analysis": tab[2]}}
 
 
 def make_inputs(sht, seed):
@@ -281,51 +284,119 @@ def load_traffic(kernel, tag, nside, nsim):
 
 def pol_round_trip(args, lmax):
     """Spin-2 workload (nside 1024, lmax 2047, nsim Q/U pairs): one step =
-    qu2eb(niter) + eb2qu on Q/U built as eb2qu of seeded E/B alm.  Prints one
-    JSON line (Q/U pairs per second, per-niter E/B error against the input alm);
-    the T round trip stays the benchmark value.  Mixed precision covers the
-    temperature transforms only, so this line is FP64."""
+    qu2eb(niter) + eb2qu on Q/U built as eb2qu of seeded E/B alm
+    (C_l^BB = 0.05 C_l^EE).  The same handle runs the step in FP64 and in mixed
+    precision (int8-slice F1/F2 contractions, tables generated per pass, FP64
+    refinement).  Prints one JSON line: Q/U pairs per second for both, their
+    ratio, the E/B error against the input alm per niter for both, and the
+    mixed result's error against the FP64 result per niter (qu2eb) and for
+    eb2qu (first 16 maps).  The T round trip stays the benchmark value."""
     import torch
 
     from fastsht_amd import SHT, _lib
 
     sht = SHT(args.nside, lmax, args.nsim, niter=args.niter, pol=True)
     lib = _lib.load()
+    precs = [args.only] if args.only else ["fp64", "mixed"]
     E, B = make_eb(sht, args.seed)
     Q, U = sht.eb2qu(E, B)
     Eo = torch.empty_like(E.permute(2, 1, 0).contiguous()).permute(2, 1, 0)
     Bo = torch.empty_like(Eo)
     p = lambda t: ctypes.c_void_p(t.data_ptr())
-    scale = max(float(E.abs().max()), float(B.abs().max()))
-    errs = {}
-    for it in range(args.niter + 1):  # per-niter error, outside the timed region
-        _lib.check(lib.fsht_qu2eb(sht._h, p(Q), p(U), p(Eo), p(Bo), it, _lib.FSHT_DEVICE), sht._h)
+    relmax = lambda a, b: float((a - b).abs().max()) / max(float(b.abs().max()), 1e-300)
+
+    def qu2eb(prec, it, e, b):
+        sht.set_precision(prec)
+        _lib.check(lib.fsht_qu2eb(sht._h, p(Q), p(U), p(e), p(b), it, _lib.FSHT_DEVICE), sht._h)
         torch.cuda.synchronize()
-        errs[str(it)] = max(float((Eo - E).abs().max()), float((Bo - B).abs().max())) / scale
+
+    # per-niter errors, outside the timed region (the first mixed call builds
+    # the spin-2 state: exponents, ranges of orders, chunk buffers)
+    scale = max(float(E.abs().max()), float(B.abs().max()))
+    err_in = {pr: {} for pr in precs}
+    for pr in precs:
+        for it in range(args.niter + 1):
+            qu2eb(pr, it, Eo, Bo)
+            err_in[pr][str(it)] = max(float((Eo - E).abs().max()), float((Bo - B).abs().max())) / scale
     del E, B
     torch.cuda.empty_cache()
+    err_vs = {}
+    eb2qu_err = None
+    if len(precs) == 2:
+        Ef = torch.empty_like(Eo)
+        Bf = torch.empty_like(Eo)
+        for it in range(args.niter + 1):
+            qu2eb("fp64", it, Ef, Bf)
+            qu2eb("mixed", it, Eo, Bo)
+            err_vs[str(it)] = max(relmax(Eo, Ef), relmax(Bo, Bf))
+        # eb2qu of the first 16 maps' FP64 E/B in both precisions
+        n = min(16, args.nsim)
+        cmp = lambda t: t[:n].permute(2, 1, 0).contiguous().permute(2, 1, 0)
+        En, Bn = cmp(Ef), cmp(Bf)
+        del Ef, Bf
+        _lib.check(lib.fsht_set_batch(sht._h, n), sht._h)
+        outs = {}
+        for pr in precs:
+            sht.set_precision(pr)
+            q = torch.empty((n, sht.npix), dtype=torch.float64, device="cuda").t()
+            u = torch.empty_like(q)
+            _lib.check(lib.fsht_eb2qu(sht._h, p(En), p(Bn), p(q), p(u), _lib.FSHT_DEVICE), sht._h)
+            outs[pr] = (q, u)
+        torch.cuda.synchronize()
+        eb2qu_err = max(relmax(outs["mixed"][0], outs["fp64"][0]), relmax(outs["mixed"][1], outs["fp64"][1]))
+        _lib.check(lib.fsht_set_batch(sht._h, args.nsim), sht._h)
+        del outs, En, Bn
+        torch.cuda.empty_cache()
 
     def step():  # Q/U are overwritten by their own round trip (kept in HBM)
         _lib.check(lib.fsht_qu2eb(sht._h, p(Q), p(U), p(Eo), p(Bo), args.niter, _lib.FSHT_DEVICE), sht._h)
         _lib.check(lib.fsht_eb2qu(sht._h, p(Eo), p(Bo), p(Q), p(U), _lib.FSHT_DEVICE), sht._h)
 
-    r = timed_steps(sht, lib, step, args.steps, args.warmup)
-    kern = r["kern"]
-    fpair = 4 * legendre_flops_per_map(args.nside, lmax, sht.ring_tab())
-    tf = {k: fpair * args.nsim / (kern[k]["ms"] / max(1, kern[k]["launches"]) / 1e3) / 1e12 for k in LEGENDRE}
-    print(json.dumps({
+    runs = {}
+    for pr in precs:
+        sht.set_precision(pr)
+        runs[pr] = timed_steps(sht, lib, step, args.steps, args.warmup)
+    info = (ctypes.c_longlong * 36)()
+    _lib.check(lib.fsht_info(sht._h, info, 36), sht._h)
+    ladder = slice_ladder(list(info), args.niter, spin=2)
+    fpair = 4 * legendre_flops_per_map(args.nside, lmax, sht.ring_tab())  # F1 and F2 against Q and U
+    passes = {"legendre_analysis": args.niter + 1, "legendre_synthesis": args.niter + 1}
+    fp64_work = {k: fpair * args.nsim * passes[k] for k in LEGENDRE}
+    int8_work = {"legendre_analysis": fpair * args.nsim * sum(slice_products(k) for k in ladder["analysis"]),
+                 "legendre_synthesis": fpair * args.nsim * (sum(slice_products(k) for k in ladder["residual_synthesis"])
+                                                            + slice_products(ladder["eb2qu"]))}
+    rates = {pr: args.nsim * args.steps / (r["ms_total"] / 1e3) for pr, r in runs.items()}
+    summ = {}
+    for pr, r in runs.items():
+        kern = r["kern"]
+        if pr == "mixed":
+            roof = legendre_roofline(kern, args.steps, int8_work, "TOP/s", PEAK_INT8_MFMA_TOPS, passes)[1]
+            roof["fp64_equivalent_tflops"] = legendre_roofline(kern, args.steps, fp64_work, "TFLOP/s",
+                                                               PEAK_FP64_MFMA_TFLOPS, passes)[1]["achieved"]
+        else:
+            roof = legendre_roofline(kern, args.steps, fp64_work, "TFLOP/s", PEAK_FP64_MFMA_TFLOPS, passes)[1]
+        summ[pr] = {"qu_pairs_per_s": rates[pr], "ms_per_step": r["ms_total"] / args.steps,
+                    "latency_ms_per_step": {"p50": pct(r["lat"], 50), "p99": pct(r["lat"], 99)},
+                    "kernels_ms_per_step": {k: v["ms"] / args.steps for k, v in kern.items()},
+                    "roofline": roof, "eb_alm_rel_err_vs_input_by_niter": err_in[pr]}
+    head = "mixed" if "mixed" in runs else "fp64"
+    out = {
         "metric": "Q/U pairs/s qu2eb+eb2qu (spin 2; not the benchmark value)",
-        "value": args.nsim * args.steps / (r["ms_total"] / 1e3), "unit": "QU pairs/s",
-        "ms_per_step": r["ms_total"] / args.steps, "steps": args.steps, "warmup": args.warmup, "dtype": "f64",
-        "precision": "fp64 (mixed precision covers the temperature transforms only)",
+        "value": rates[head], "unit": "QU pairs/s", "variant": head,
+        "ms_per_step": runs[head]["ms_total"] / args.steps, "steps": args.steps, "warmup": args.warmup,
+        "dtype": "i8/f64" if head == "mixed" else "f64",
         "config": {"workload": f"qu2eb(niter={args.niter})+eb2qu nside={args.nside} lmax={lmax} nsim={args.nsim} "
                                "Q/U from seeded E/B (C_l^BB = 0.05 C_l^EE)",
-                   "nside": args.nside, "lmax": lmax, "nsim": args.nsim, "niter": args.niter},
-        "latency_ms_per_step": {"p50": pct(r["lat"], 50), "p99": pct(r["lat"], 99)},
-        "legendre_tflops": tf,
-        "eb_alm_rel_err_vs_input_by_niter": errs,
-        "kernels_ms_per_step": {k: v["ms"] / args.steps for k, v in kern.items()},
-    }))
+                   "nside": args.nside, "lmax": lmax, "nsim": args.nsim, "niter": args.niter,
+                   "slices_per_pass": ladder},
+    }
+    out.update({pr: summ[pr] for pr in runs})
+    if len(runs) == 2:
+        out["mixed_vs_fp64"] = rates["mixed"] / rates["fp64"]
+        out["mixed_eb_rel_err_vs_fp64_by_niter"] = err_vs
+        out["mixed_eb2qu_map_rel_err_vs_fp64"] = eb2qu_err
+        out["mixed_err_target"] = 1e-10
+    print(json.dumps(out))
 
 
 def host_round_trip(sht, maps_dev, args):
@@ -611,19 +682,18 @@ def errors_by_niter(sht, lib, maps, niter):
     return errs, map_err
 
 
-def legendre_roofline(kern, steps, work, bound_unit, peak):
+def legendre_roofline(kern, steps, work, bound_unit, peak, passes):
     """Roofline of the dominant Legendre class (the larger total time): the
-    algorithmic work of one pass (`work[cls]` per step / passes per step) over
-    the class's average pass duration.  A pass is one library call of the
-    class; its launches are all counted in its duration."""
+    algorithmic work of one pass (`work[cls]` per step / `passes[cls]` per
+    step) over the class's average pass duration (its time per step over its
+    passes per step; a pass may span several recorded launches)."""
     dom = max(LEGENDRE, key=lambda k: kern[k]["ms"])
-    passes = kern[dom]["launches"] / steps
-    avg_ms = kern[dom]["ms"] / max(1, kern[dom]["launches"])
-    per_pass = work[dom] / passes
+    avg_ms = kern[dom]["ms"] / steps / passes[dom]
+    per_pass = work[dom] / passes[dom]
     achieved = per_pass / (avg_ms / 1e3) / 1e12
     return dom, {"bound": "mfma", "kernel": dom, "achieved": achieved, "peak": peak, "unit": bound_unit,
                  "frac": achieved / peak, "work_per_launch": per_pass, "avg_launch_ms": avg_ms,
-                 "launches_per_step": passes}
+                 "launches_per_step": passes[dom]}
 
 
 def gpu_c0(precision):
@@ -743,8 +813,8 @@ def main():
         sht.alm2t(alm_out, maps_out)
 
     sht.set_precision("mixed")  # builds the int8 tables before any timing
-    info = (ctypes.c_longlong * 15)()
-    _lib.check(lib.fsht_info(sht._h, info, 15), sht._h)
+    info = (ctypes.c_longlong * 36)()
+    _lib.check(lib.fsht_info(sht._h, info, 36), sht._h)
     ladder = slice_ladder(list(info), args.niter)
     runs = {}
     for prec in ("fp64", "mixed"):
@@ -767,15 +837,16 @@ def main():
     int8_work = {"legendre_analysis": fpm * args.nsim * sum(slice_products(k) for k in ladder["analysis"]),
                  "legendre_synthesis": fpm * args.nsim * (sum(slice_products(k) for k in ladder["residual_synthesis"])
                                                           + slice_products(ladder["alm2t"]))}
+    passes = {"legendre_analysis": args.niter + 1, "legendre_synthesis": args.niter + 1}
     roofs = {}
     for prec, r in runs.items():
         if prec == "mixed":
-            dom, roof = legendre_roofline(r["kern"], args.steps, int8_work, "TOP/s", PEAK_INT8_MFMA_TOPS)
-            fp = legendre_roofline(r["kern"], args.steps, fp64_work, "TFLOP/s", PEAK_FP64_MFMA_TFLOPS)[1]
+            dom, roof = legendre_roofline(r["kern"], args.steps, int8_work, "TOP/s", PEAK_INT8_MFMA_TOPS, passes)
+            fp = legendre_roofline(r["kern"], args.steps, fp64_work, "TFLOP/s", PEAK_FP64_MFMA_TFLOPS, passes)[1]
             roof["fp64_equivalent_tflops"] = fp["achieved"]
             roof["traffic"] = load_traffic(dom, "_mixed", args.nside, args.nsim)
         else:
-            dom, roof = legendre_roofline(r["kern"], args.steps, fp64_work, "TFLOP/s", PEAK_FP64_MFMA_TFLOPS)
+            dom, roof = legendre_roofline(r["kern"], args.steps, fp64_work, "TFLOP/s", PEAK_FP64_MFMA_TFLOPS, passes)
             roof["traffic"] = load_traffic(dom, "", args.nside, args.nsim)
         roofs[prec] = roof
 

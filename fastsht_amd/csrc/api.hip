@@ -211,11 +211,15 @@ int fsht_set_batch(fsht_handle* H, int nsim) {
 int fsht_info(const fsht_handle* H, long long* out, int n) {
   if (!H || !out) return fail(nullptr, FSHT_ERR_ARG, "NULL argument");
   const Handle* h = &H->h;
-  long long v[16] = {h->nside,    h->lmax,     h->nsim,     h->niter,        h->pol,
+  long long v[36] = {h->nside,    h->lmax,     h->nsim,     h->niter,        h->pol,
                      h->npix,     h->nsimp,    h->G_size,   h->precision,    h->mix_bytes,
                      h->mix_k_lo, h->mix_k_hi, h->mix_damp_min, h->mix_k_out, h->mix_damp_min_a,
                      h->nsim_cap};
-  for (int i = 0; i < n && i < 16; ++i) out[i] = v[i];
+  for (int sp = 0; sp < 2; ++sp) {
+    for (int k = 0; k < 9; ++k) v[16 + 10 * sp + k] = h->mix_lad[sp][k / 3][k % 3];
+    v[25 + 10 * sp] = h->mix_out[sp];
+  }
+  for (int i = 0; i < n && i < 36; ++i) out[i] = v[i];
   return FSHT_OK;
 }
 
@@ -257,6 +261,18 @@ int fsht_set_precision(fsht_handle* H, int precision) {
   h->mix_damp_min = 2;    // residual syntheses
   h->mix_damp_min_a = 1;  // analyses (after the first, they act on residuals)
   h->mix_k_out = 6;
+  // the same rule as a table per (pass kind, damping).  Spin 2 adds one
+  // slice to the residual synthesis no later iteration damps: with five
+  // there, qu2eb(niter >= 1) sits at 1.0-1.5e-10 against the FP64 result at
+  // nside 64-1024 (lmax = 2 nside - 1); with six, at <= 1.5e-11
+  for (int d = 0; d < 3; ++d) {
+    h->mix_lad[0][0][d] = d == 0 ? h->mix_k_out : (d >= h->mix_damp_min ? h->mix_k_lo : h->mix_k_hi);
+    h->mix_lad[0][1][d] = d >= h->mix_damp_min ? h->mix_k_lo : h->mix_k_hi;
+    h->mix_lad[0][2][d] = d >= h->mix_damp_min_a ? h->mix_k_lo : h->mix_k_hi;
+    for (int k = 0; k < 3; ++k) h->mix_lad[1][k][d] = h->mix_lad[0][k][d];
+  }
+  h->mix_lad[1][1][0] = std::min(6, h->mix_lad[0][1][0] + 1);
+  h->mix_out[0] = h->mix_out[1] = h->mix_k_out;
   h->precision = precision;
   return FSHT_OK;
 }
@@ -493,38 +509,78 @@ static int check_common(fsht_handle* H, int loc) {
 // iterations damp that pass's error: the last analysis and the last two
 // residual syntheses carry more.  An unrefined analysis (niter = 0) carries
 // the full slice count.
-static int analysis_iter_mixed(Handle* h, double* a0, int niter, int method) {
+// The mixed-precision state a transform of the current spin needs (built on
+// first use: the spin-0 P tables of a spin-2 capable handle, the spin-2
+// F1/F2 exponents and work lists).
+static int ensure_mixed(Handle* h) {
+  const hipError_t e = fsht::mixed_ensure(h, h->spin);
+  if (e == hipErrorOutOfMemory)
+    return fail(h, FSHT_ERR_NOMEM, "mixed precision: the int8 Legendre tables and sliced operands do not fit");
+  if (e != hipSuccess) return fail(h, FSHT_ERR_HIP, std::string("mixed precision set-up: ") + hipGetErrorString(e));
+  return FSHT_OK;
+}
+
+// Mixed precision applies to this transform: the handle runs mixed, and an
+// allocation-free call (fsht_fix_eb_noalloc) only uses mixed-precision state
+// that is already set up (the rest runs in FP64, as the call allocates nothing).
+static bool use_mixed(const Handle* h) {
+  return h->precision == FSHT_PREC_MIXED && (!h->noalloc || (h->spin ? h->mix2_ready : h->mix0_ready));
+}
+
+// One mixed-precision Legendre pass (synthesis = 0: alm <- (beta alm +) the
+// analysis of the operand in FQ; 1: the m-major buffer G <- synthesis of the
+// operand in AQ), range of orders by range; spin-2 ranges generate their
+// F1/F2 tables first.
+static int mixed_pass(Handle* h, int synthesis, double* a0, double* a1, double beta, int K, int exps) {
+  if (!synthesis && exps) HIPCHK(h, fsht::launch_aqe_reset(h));
+  const int nr = fsht::mixed_nrange(h);
+  for (int c = 0; c < nr; ++c) {
+    if (h->spin) HIPCHK(h, timed(h, fsht::KC_TABLES, [&] { return fsht::launch_ftab(h, c, synthesis, K); }));
+    if (synthesis)
+      HIPCHK(h, timed(h, fsht::KC_SYNTHESIS, [&] { return fsht::launch_synthesis_q(h, c, h->ws_G, K); }));
+    else
+      HIPCHK(h, timed(h, fsht::KC_ANALYSIS, [&] { return fsht::launch_analysis_q(h, c, a0, a1, beta, K, exps); }));
+  }
+  return FSHT_OK;
+}
+
+static int analysis_iter_mixed(Handle* h, double* a0, double* a1, int niter, int method) {
+  int rc = ensure_mixed(h);
+  if (rc) return rc;
   // error of analysis k is damped by the niter - k later iterations, that of
   // synthesis k (the residual of iteration k) by niter - k - 1
-  auto kslices = [&](int damping) { return damping >= h->mix_damp_min ? h->mix_k_lo : h->mix_k_hi; };
-  auto kslices_a = [&](int damping) { return damping >= h->mix_damp_min_a ? h->mix_k_lo : h->mix_k_hi; };
+  const int sp = h->spin ? 1 : 0;
+  auto lad = [&](int kind, int damping) { return h->mix_lad[sp][kind][damping < 2 ? damping : 2]; };
   HIPCHK(h, fsht::launch_fexp_reset(h));
   HIPCHK(h, timed(h, fsht::KC_GATHER, [&] { return fsht::launch_gather(h, h->ws_specT, h->ws_G, h->d_FQe); }));
   // a single unrefined analysis carries the full slice count
-  int ka = niter == 0 ? h->mix_k_out : kslices(niter);  // the full maps: as damped as a synthesis
+  int ka = niter == 0 ? h->mix_out[sp] : lad(0, niter);
   HIPCHK(h, timed(h, fsht::KC_SLICE, [&] { return fsht::launch_quant_F(h, h->ws_G, ka); }));
   // each analysis but the last also records alm's column exponents for the
   // next residual synthesis
-  HIPCHK(h, timed(h, fsht::KC_ANALYSIS, [&] { return fsht::launch_analysis_q(h, a0, 0.0, ka, niter > 0); }));
+  if ((rc = mixed_pass(h, 0, a0, a1, 0.0, ka, niter > 0))) return rc;
   if (niter > 0 && !h->ws_F)
     HIPCHK(h, hipMalloc((void**)&h->ws_F, (size_t)(h->G_size / (h->pol ? 1 : 2)) * sizeof(double)));
   for (int it = 0; it < niter; ++it) {
-    const int ks = kslices(niter - it - 1);
-    HIPCHK(h, timed(h, fsht::KC_SLICE, [&] { return fsht::launch_quant_A(h, a0, ks, 1); }));
-    HIPCHK(h, timed(h, fsht::KC_SYNTHESIS, [&] { return fsht::launch_synthesis_q(h, h->ws_G, ks); }));
+    const int ks = lad(1, niter - it - 1);
+    HIPCHK(h, timed(h, fsht::KC_SLICE, [&] { return fsht::launch_quant_A(h, a0, a1, ks, 1); }));
+    if ((rc = mixed_pass(h, 1, nullptr, nullptr, 0.0, ks, 0))) return rc;
     HIPCHK(h, fsht::launch_fexp_reset(h));
     HIPCHK(h, timed(h, fsht::KC_FOLD, [&] {
       return fsht::launch_gather_prefix(h, h->ws_G, h->ws_specT, h->ws_F, method == 0 ? 1 : 0, h->d_FQe);
     }));
-    ka = kslices_a(niter - it - 1);
+    ka = lad(2, niter - it - 1);
     HIPCHK(h, timed(h, fsht::KC_SLICE, [&] { return fsht::launch_quant_F(h, h->ws_F, ka); }));
-    HIPCHK(h, timed(h, fsht::KC_ANALYSIS, [&] { return fsht::launch_analysis_q(h, a0, 1.0, ka, it + 1 < niter); }));
+    if ((rc = mixed_pass(h, 0, a0, a1, 1.0, ka, it + 1 < niter))) return rc;
   }
   return FSHT_OK;
 }
 
 static int analysis_iter(Handle* h, double* a0, double* a1, int niter, int want0, int want1, int method = 0) {
-  if (h->precision == FSHT_PREC_MIXED && h->spin == 0) return analysis_iter_mixed(h, a0, niter, method);
+  // mixed precision: temperature, and spin 2 when both E and B are wanted
+  // (the F2 contractions add to what the F1 ones wrote)
+  if (use_mixed(h) && (h->spin == 0 || (want0 && want1)))
+    return analysis_iter_mixed(h, a0, h->spin ? a1 : nullptr, niter, method);
   HIPCHK(h, timed(h, fsht::KC_GATHER, [&] { return fsht::launch_gather(h, h->ws_specT, h->ws_G); }));
   HIPCHK(h, timed(h, fsht::KC_ANALYSIS, [&] { return fsht::launch_analysis(h, h->ws_G, a0, a1, 0.0, want0, want1); }));
   if (niter > 0 && !h->ws_F)
@@ -600,9 +656,10 @@ int fsht_alm2t(fsht_handle* H, const double* alms, double* maps, int loc) {
     a = h->ws_alm[0];
     h->have_dev_alm = 0;
   }
-  if (h->precision == FSHT_PREC_MIXED) {
-    HIPCHK(h, timed(h, fsht::KC_SLICE, [&] { return fsht::launch_quant_A(h, a, h->mix_k_out, 0); }));
-    HIPCHK(h, timed(h, fsht::KC_SYNTHESIS, [&] { return fsht::launch_synthesis_q(h, h->ws_G, h->mix_k_out); }));
+  if (use_mixed(h)) {
+    if ((rc = ensure_mixed(h))) return rc;
+    HIPCHK(h, timed(h, fsht::KC_SLICE, [&] { return fsht::launch_quant_A(h, a, nullptr, h->mix_out[0], 0); }));
+    if ((rc = mixed_pass(h, 1, nullptr, nullptr, 0.0, h->mix_out[0], 0))) return rc;
   } else {
     HIPCHK(h, timed(h, fsht::KC_SYNTHESIS, [&] { return fsht::launch_synthesis(h, a, nullptr, h->ws_G); }));
   }
@@ -670,7 +727,13 @@ int fsht_eb2qu(fsht_handle* H, const double* E, const double* B, double* Q, doub
     b = h->ws_alm[1];
     h->have_dev_alm = 0;
   }
-  HIPCHK(h, timed(h, fsht::KC_SYNTHESIS, [&] { return fsht::launch_synthesis(h, e, b, h->ws_G); }));
+  if (use_mixed(h)) {
+    if ((rc = ensure_mixed(h))) return rc;
+    HIPCHK(h, timed(h, fsht::KC_SLICE, [&] { return fsht::launch_quant_A(h, e, b, h->mix_out[1], 0); }));
+    if ((rc = mixed_pass(h, 1, nullptr, nullptr, 0.0, h->mix_out[1], 0))) return rc;
+  } else {
+    HIPCHK(h, timed(h, fsht::KC_SYNTHESIS, [&] { return fsht::launch_synthesis(h, e, b, h->ws_G); }));
+  }
   HIPCHK(h, timed(h, fsht::KC_FOLD, [&] { return fsht::launch_fold(h, h->ws_G, nullptr, h->ws_specT, 0); }));
   if ((rc = inverse_spectra(h, Q, U, loc))) return rc;
   if (!loc) HIPCHK(h, hipStreamSynchronize(h->stream));
@@ -822,7 +885,9 @@ int fsht_fix_eb_noalloc(fsht_handle* H, double* Q, double* U, const double* mask
   // no profiling events either: recording them would create events
   const int prof = h->profiling;
   h->profiling = 0;
+  h->noalloc = 1;
   rc = fix_eb_device(H, Q, U, mask, Bmap, ab, bt, cf, niter, dE, dB2, dZ, dQ2, dU2);
+  h->noalloc = 0;
   h->profiling = prof;
   return rc;
 }

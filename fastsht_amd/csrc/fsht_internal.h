@@ -121,15 +121,24 @@ hipError_t launch_fold(Handle* h, const double* G, const double* specT_t, double
 // m-groups spread over the XCDs by the estimated cost of each item
 hipError_t upload_items(const std::vector<LegItem>& v, const std::vector<double>& cost, LegItem** d, int* n);
 // mixed precision (k_mixed.hip)
-hipError_t mixed_setup(Handle* h);        // P tables + work lists (allocates; set-up only)
+hipError_t mixed_setup(Handle* h);        // shared state, and the P tables of a temperature-only handle
+hipError_t mixed_ensure(Handle* h, int spin);  // the state a spin-0 or spin-2 transform needs (set-up on first use)
 void mixed_free(Handle* h);
 hipError_t launch_fexp_reset(Handle* h);                          // FQe <- kMixZeroE
 hipError_t launch_quant_F(Handle* h, const double* F, int K);      // m-major FP64 operand -> FQ (FQe set)
-// packed alm -> AQ; have_exp: AQe already holds alm's column exponents (set by
-// the analysis that wrote alm with exps = 1), else they are computed here
-hipError_t launch_quant_A(Handle* h, const double* alm, int K, int have_exp);
-hipError_t launch_analysis_q(Handle* h, double* alm, double beta, int K, int exps);
-hipError_t launch_synthesis_q(Handle* h, double* G, int K);
+// packed alm (alm1: the second component, spin 2) -> AQ; have_exp: AQe
+// already holds the columns' exponents (set by the analysis that wrote the
+// alm with exps = 1), else they are computed here
+hipError_t launch_quant_A(Handle* h, const double* alm0, const double* alm1, int K, int have_exp);
+// Slice-GEMM passes run over ranges of orders: one range for spin 0; for
+// spin 2 each range's F1/F2 tables are generated (launch_ftab) just before
+// its contractions.  An analysis that records its result's column exponents
+// (exps = 1) needs launch_aqe_reset first.
+int mixed_nrange(const Handle* h);
+hipError_t launch_ftab(Handle* h, int range, int synthesis, int K);
+hipError_t launch_aqe_reset(Handle* h);
+hipError_t launch_analysis_q(Handle* h, int range, double* alm0, double* alm1, double beta, int K, int exps);
+hipError_t launch_synthesis_q(Handle* h, int range, double* G, int K);
 // map-domain helpers (k_maps.hip)
 hipError_t launch_mask_maps(double* maps, const double* mask, long long npix, int nmaps, hipStream_t s);
 hipError_t launch_zero(double* p, long long n, hipStream_t s);
@@ -216,15 +225,23 @@ struct Handle {
   // current call's spin (0 or 2)
   int spin = 0;
 
-  // mixed precision (temperature transforms): int8-slice P tables, quantized
-  // operands, work lists and the slice counts per pass
+  // mixed precision: int8-slice Legendre tables, sliced operands, work lists
+  // and the slice counts per pass
   int precision = 0;  // 0: FP64, 1: mixed
-  int mix_ready = 0;
+  int noalloc = 0;    // inside an allocation-free call: no state is set up on first use
+  int mix_ready = 0;   // offsets, sliced operands and table work items set up
+  int mix0_ready = 0;  // spin 0: P tables and work lists
+  int mix2_ready = 0;  // spin 2: F1/F2 exponents, m ranges, chunk buffers and work lists
   std::vector<MixOff> mix_off;  // per (m, class)
   MixOff* d_mix_off = nullptr;
   signed char *d_TA = nullptr, *d_TS = nullptr, *d_FQ = nullptr, *d_AQ = nullptr;
+  int fq_alias = 0;  // d_FQ lives in ws_spec
   int *d_TAe = nullptr, *d_TSe = nullptr, *d_FQe = nullptr, *d_AQe = nullptr;
   int *d_firstA = nullptr, *d_firstS = nullptr;
+  int2* d_pitems = nullptr;        // (m, 256-row block) items of the table kernels, m ascending
+  std::vector<int> pitem_off;      // first item of each m (lmax + 2 entries)
+  long long mix_ta_total = 0, mix_tae = 0, mix_tse = 0;  // table bytes (six slices) and exponent counts
+  int mix_fa = 0, mix_fs = 0;      // entries of the first-nonzero-step tables
   // slice-GEMM work lists: [0] full blocks of 4 column tiles (4 row tiles
   // each), [1] the remaining column tiles one by one (8 row tiles each), [2]
   // the remaining column tiles as one padded block of 4 (for six slices,
@@ -232,6 +249,17 @@ struct Handle {
   LegItem* d_anaq_items[3] = {nullptr, nullptr, nullptr};
   LegItem* d_synq_items[3] = {nullptr, nullptr, nullptr};
   int n_anaq_items[3] = {0, 0, 0}, n_synq_items[3] = {0, 0, 0};
+  // spin 2: F1 and F2 exponents / first nonzero steps (resident), the tables
+  // of one m range in one operand order (generated per pass), the ranges and
+  // the work lists per (function, range, kind)
+  int *d_TAe2[2] = {nullptr, nullptr}, *d_TSe2[2] = {nullptr, nullptr};
+  int *d_firstA2[2] = {nullptr, nullptr}, *d_firstS2[2] = {nullptr, nullptr};
+  signed char* d_T2[2] = {nullptr, nullptr};
+  long long t2_cap = 0;
+  std::vector<int> mix2_chunk;  // range c holds the orders [mix2_chunk[c], mix2_chunk[c + 1])
+  std::vector<LegItem*> d_anaq2, d_synq2;  // index ((f * nchunk) + c) * 3 + kind
+  std::vector<int> n_anaq2, n_synq2;
+  int t2_lay = -1, t2_ks = 0;  // operand order and slices the chunk buffers hold (single-range plans)
   long long mix_bytes = 0;  // device memory held by the mixed-precision tables and operands
   int mix_nrb_max = 0;  // 256-row blocks of the longest order (grid of the operand slicer)
   // slices per pass: the first analysis and a residual synthesis whose error
@@ -241,6 +269,12 @@ struct Handle {
   // (set from lmax / nside: the stronger the aliasing, the weaker the
   // damping); alm2t and an unrefined t2alm use mix_k_out
   int mix_k_lo = 4, mix_k_hi = 5, mix_damp_min = 2, mix_k_out = 6, mix_damp_min_a = 1;
+  // the slices of every pass per spin (0: temperature, 1: spin 2): [kind][d]
+  // with kind 0 the first analysis, 1 a residual synthesis, 2 a residual
+  // analysis, and d = min(2, later Jacobi iterations that damp the pass's
+  // error); mix_out: alm2t / eb2qu and an unrefined analysis
+  int mix_lad[2][3][3] = {{{6, 5, 4}, {5, 5, 4}, {5, 4, 4}}, {{6, 5, 4}, {6, 5, 4}, {5, 4, 4}}};
+  int mix_out[2] = {6, 6};
 
   // optional per-kernel-class timing with HIP events on `stream`
   int profiling = 0;
@@ -251,6 +285,20 @@ struct Handle {
   std::vector<Timed> timed;
   std::vector<hipEvent_t> event_pool;
 };
+
+// Spin-2 functions from P_l and P_{l-1} (src/get_plm_recursive.f90:111-124).
+// (f = 0: F1, 1: F2; sp = (lambda_l, normal_l); bw = cos/sin^2, two_on_s2 = 2/sin^2,
+// cw = m/sin^2)
+__device__ inline double spin_f(int f, double P, double Pm1, double2 sp, int l, int m, double x, double bw,
+                                double two_on_s2, double cw) {
+  const double lam = sp.x, nl = sp.y;
+  const double dl = (double)l;
+  if (f == 0) {
+    const double lm2 = dl - (double)m * (double)m;
+    return (Pm1 * lam * bw - P * (two_on_s2 * lm2 + dl * (dl - 1.0))) * nl;
+  }
+  return (Pm1 * lam - P * (2.0 * x * (dl - 1.0))) * cw * nl;
+}
 
 // kernel classes reported by fsht_profile_read
 // Batches smaller than the handle's capacity: the m-major buffers keep nsimp
@@ -274,6 +322,7 @@ enum KernelClass {
   KC_FOLD,
   KC_GATHER,
   KC_SLICE,  // mixed precision: operand slicing (k_quant_F / k_quant_A)
+  KC_TABLES, // mixed precision: spin-2 F1/F2 tables of a range of orders (k_ftab)
   KC_COUNT
 };
 

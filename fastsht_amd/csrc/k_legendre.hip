@@ -74,18 +74,6 @@ __device__ inline Src syn_src(int spin, int otype, int f) {
   }
 }
 
-// Spin-2 functions from P_l and P_{l-1} (src/get_plm_recursive.f90:111-124).
-__device__ inline double spin_f(int f, double P, double Pm1, double2 sp, int l, int m, double x, double bw,
-                                double two_on_s2, double cw) {
-  const double lam = sp.x, nl = sp.y;
-  const double dl = (double)l;
-  if (f == 0) {
-    const double lm2 = dl - (double)m * (double)m;
-    return (Pm1 * lam * bw - P * (two_on_s2 * lm2 + dl * (dl - 1.0))) * nl;
-  }
-  return (Pm1 * lam - P * (2.0 * x * (dl - 1.0))) * cw * nl;
-}
-
 // Per-thread recurrence state for one ring.  The recurrence is 0 until the
 // chunk that starts at l = ls, where (p1, p2) are loaded from sv.
 struct Rec {

@@ -21,6 +21,7 @@
 // N/S ring symmetry is folded as in the FP64 path: one contraction per parity
 // class over the north rings, with the folded operand N +- S.
 #include <algorithm>
+#include <cstdlib>
 #include <vector>
 
 #include "fsht_internal.h"
@@ -228,18 +229,177 @@ __global__ __launch_bounds__(256, 1) void k_ptab(const int2* __restrict__ items,
 }
 
 // ---------------------------------------------------------------------------
-// Analysis operand: the m-major folded spectrum F (FP64, layout of k_gather)
-// -> FQ slices with the per-column exponents the gather left in FQe
-// (index (2 m + class) * ncols + col).  Workgroup = (m, class, 8 ring steps,
-// 32-column tile); thread = (column, ring step): 32 rows of one column.
+// Spin-2 tables: F1 (FN = 0) or F2 (FN = 1) of src/get_plm_recursive.f90:105-133
+// for the orders of one m range, from the FP64 recurrence and start states of
+// the P tables, every value formed by spin_f from P_l and P_{l-1} (the
+// FP64 spin-2 kernels' own expression).  Workgroup = (m, block of 256 ring
+// rows); thread = ring; tiles of 64 l steps (32 values per class).
+//   MODE 0: exponents (TSe per ring row and class over every l; TAe per l
+//           over the rings) and the first nonzero steps, as k_ptab<0>.
+//   MODE 1: the KS digit slices of one operand order (LAY 0: TA, LAY 1: TS)
+//           into a buffer holding the range: the blocks of (m, class) start
+//           at byte (o.ta or o.ts) / kMixK * KS - tbase.
+// ---------------------------------------------------------------------------
+template <int FN, int MODE, int LAY, int KS>
+__global__ __launch_bounds__(256, 1) void k_ftab(const int2* __restrict__ items, const RingGeo* __restrict__ geo,
+                                                 const int* __restrict__ i0tab, const double2* __restrict__ ab,
+                                                 const double2* __restrict__ spin, const long long* __restrict__ aboff,
+                                                 const long long* __restrict__ synoff,
+                                                 const int* __restrict__ syn_st, const double2* __restrict__ syn_v,
+                                                 const MixOff* __restrict__ mo, signed char* __restrict__ T,
+                                                 long long tbase, int* __restrict__ TAe, int* __restrict__ TSe,
+                                                 int* __restrict__ firstA, int* __restrict__ firstS, int lmax,
+                                                 int nhr) {
+  constexpr bool TA_OUT = MODE == 1 && LAY == 0;
+  constexpr bool TS_OUT = MODE == 1 && LAY == 1;
+  __shared__ __align__(16) signed char stage[TA_OUT ? 2 : 1][TA_OUT ? KS : 1][32][TA_OUT ? 256 + 16 : 16];
+  __shared__ int emax[2][32], rmin[2];
+  const int2 it = items[blockIdx.x];
+  const int m = it.x, rb = it.y, tid = threadIdx.x;
+  const int i0 = i0tab[m];
+  const int R = nhr + 1 - i0;
+  const int row = rb * 256 + tid;
+  const bool valid = row < R;
+  const long long yo = synoff[m] + (valid ? row : R - 1);
+  const int st = valid ? syn_st[yo] : kNeverL;
+  const double2 sv = syn_v[yo];  // (P_ls-2, P_ls-1) at the 32-step chunk start ls <= st
+  const int ls = (st > lmax) ? kNeverL : m + ((st - m) & ~31);
+  const RingGeo g = geo[valid ? i0 + row : nhr];
+  const double x = g.z;
+  const double s2 = g.s * g.s;
+  const double bw = x / s2, t2 = 2.0 / s2, cw = (double)m / s2;
+  const double2* abm = ab + aboff[m];
+  const double2* spm = spin + aboff[m];
+  const MixOff o0 = mo[2 * m], o1 = mo[2 * m + 1];
+  const bool inrow = row < o0.nrs * 32;  // row of the padded ring tiles (both classes have nrs tiles)
+  const long long rpos = (long long)(row >> 5) * 32 + dpos(row & 31);  // within a class's TS exponents
+  int eS0 = kMixZeroE, eS1 = kMixZeroE;
+  if (TS_OUT && inrow) {
+    eS0 = TSe[o0.tse + rpos];
+    eS1 = TSe[o1.tse + rpos];
+  }
+  if (tid < 64) emax[tid >> 5][tid & 31] = kMixZeroE;
+  if (tid < 2) rmin[tid] = kNeverStep;
+  __syncthreads();
+
+  double p1 = 0.0, p2 = 0.0, mx0 = 0.0, mx1 = 0.0;
+  int myfirst0 = kNeverStep, myfirst1 = kNeverStep;
+  const int ntile = (lmax - m) / 64 + 1;
+  for (int lt = 0; lt < ntile; ++lt) {
+    double v[64];
+    const int lb = m + 64 * lt;
+#pragma unroll
+    for (int s = 0; s < 64; ++s) {
+      const int l = lb + s;
+      if ((s & 31) == 0 && l == ls) {
+        p1 = sv.y;
+        p2 = sv.x;
+      }
+      const bool in = l <= lmax;
+      const double2 c = in ? abm[l - m] : make_double2(0.0, 0.0);
+      const double2 sp = in ? spm[l - m] : make_double2(0.0, 0.0);
+      const double p = fma(c.x * x, p1, -(c.y * p2));
+      v[s] = spin_f(FN, p, p1, sp, l, m, x, bw, t2, cw);
+      p2 = p1;
+      p1 = p;
+    }
+#pragma unroll
+    for (int cls = 0; cls < 2; ++cls) {
+      const MixOff& o = cls ? o1 : o0;
+      if (lt >= o.nlt) continue;
+      double mx = 0.0;
+#pragma unroll
+      for (int j = 0; j < 32; ++j) mx = fmax(mx, fabs(v[2 * j + cls]));
+      if (MODE == 0) {
+        if (cls) mx1 = fmax(mx1, mx); else mx0 = fmax(mx0, mx);
+        if (mx > 0.0) {
+#pragma unroll
+          for (int j = 0; j < 32; ++j) atomicMax(&emax[cls][j], mix_exp(fabs(v[2 * j + cls])));
+          atomicMin(&rmin[cls], row >> 5);
+          if (inrow) {
+            if (cls) myfirst1 = min(myfirst1, lt); else myfirst0 = min(myfirst0, lt);
+          }
+        }
+      } else if (TS_OUT && inrow) {
+        // this ring's lane (row & 31) + 32 g holds values 16 g .. 16 g + 15
+        const int e = cls ? eS1 : eS0;
+        double sc[32];
+#pragma unroll
+        for (int j = 0; j < 32; ++j) sc[j] = ldexp(v[2 * j + cls], -e);
+        signed char* dst = T + (o.ts / kMixK * KS - tbase) + ((long long)(row >> 5) * o.nlt + lt) * KS * 1024 +
+                           (row & 31) * 16;
+        put16<KS>(sc, dst);
+        put16<KS>(sc + 16, dst + 32 * 16);
+      }
+    }
+    if (TA_OUT) {
+      // TA digits of the 32 values of each class with the per-l exponents, into LDS
+#pragma unroll
+      for (int cls = 0; cls < 2; ++cls) {
+        const MixOff& o = cls ? o1 : o0;
+        if (lt >= o.nlt) continue;
+        const int* ea = TAe + o.tae + (long long)lt * 32;
+#pragma unroll
+        for (int j = 0; j < 32; ++j) {
+          double r = ldexp(v[2 * j + cls], -ea[dpos(j)]);
+#pragma unroll
+          for (int s = 0; s < KS; ++s) {
+            const double t = r * 128.0;
+            const double q = rint(t);
+            r = t - q;
+            stage[cls][s][j][tid] = (signed char)(int)q;
+          }
+        }
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int cls = 0; cls < 2; ++cls) {
+      const MixOff& o = cls ? o1 : o0;
+      if (lt >= o.nlt) continue;
+      if (MODE == 0) {
+        if (tid < 32 && emax[cls][tid] != kMixZeroE) atomicMax(&TAe[o.tae + (long long)lt * 32 + dpos(tid)], emax[cls][tid]);
+        if (tid == 0 && rmin[cls] != kNeverStep) atomicMin(&firstA[o.fa + lt], rmin[cls]);
+      } else if (TA_OUT) {
+        const int nrs_here = min(8, o.nrs - rb * 8);
+        signed char* base = T + (o.ta / kMixK * KS - tbase);
+        for (int ch = tid; ch < nrs_here * KS * 64; ch += 256) {
+          const int L = ch & 63, s = (ch >> 6) % KS, rsl = (ch >> 6) / KS;
+          const v4i val = *reinterpret_cast<const v4i*>(&stage[cls][s][L & 31][rsl * 32 + 16 * (L >> 5)]);
+          *reinterpret_cast<v4i*>(base + (((long long)lt * o.nrs + rb * 8 + rsl) * KS + s) * 1024 + L * 16) = val;
+        }
+      }
+    }
+    __syncthreads();
+    if (MODE == 0) {
+      if (tid < 64) emax[tid >> 5][tid & 31] = kMixZeroE;
+      if (tid < 2) rmin[tid] = kNeverStep;
+      __syncthreads();
+    }
+  }
+  if (MODE == 0 && inrow) {
+    TSe[o0.tse + rpos] = valid ? mix_exp(mx0) : kMixZeroE;
+    TSe[o1.tse + rpos] = valid ? mix_exp(mx1) : kMixZeroE;
+    if (myfirst0 != kNeverStep) atomicMin(&firstS[o0.fs + (row >> 5)], myfirst0);
+    if (myfirst1 != kNeverStep) atomicMin(&firstS[o1.fs + (row >> 5)], myfirst1);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Analysis operand: the m-major folded spectrum F (FP64, layout of k_gather,
+// ncomp field components) -> FQ slices with the per-column exponents the
+// gather left in FQe (index (2 m + class) * ncols + col).  Workgroup = (m,
+// class, 8 ring steps, 32-column tile); thread = (column, ring step): 32 rows
+// of one column.
 // ---------------------------------------------------------------------------
 template <int K>
 __global__ __launch_bounds__(256) void k_quant_F(const MixOff* __restrict__ mo, const int* __restrict__ i0tab,
                                                  const long long* __restrict__ goff, const double* __restrict__ F,
                                                  const int* __restrict__ FQe, signed char* __restrict__ FQ, int nhr,
-                                                 int ncols, int nsim) {
+                                                 int ncomp, int nsim, int nsimp) {
+  const int ncols = 2 * ncomp * nsimp;
   const int m = blockIdx.x, cls = blockIdx.y & 1, kb = blockIdx.y >> 1, ct = blockIdx.z;
-  if (!cols_active(ct * 32, ct * 32 + 32, nsim, ncols / 2)) return;  // no map in the tile
+  if (!cols_active(ct * 32, ct * 32 + 32, nsim, nsimp)) return;  // no map in the tile
   const MixOff o = mo[2 * m + cls];
   const int cl = threadIdx.x & 31, q = threadIdx.x >> 5;
   const int rs = kb * 8 + q;
@@ -247,14 +407,14 @@ __global__ __launch_bounds__(256) void k_quant_F(const MixOff* __restrict__ mo, 
   const int R = nhr + 1 - i0tab[m];
   const int col = ct * 32 + cl;
   const int e = FQe[(long long)(2 * m + cls) * ncols + col];
-  const double* src = F + goff[m] / 2 + (long long)cls * R * ncols + col;
+  const double* src = F + goff[m] * ncomp / 2 + (long long)cls * R * ncols + col;
   double v[32];
 #pragma unroll
   for (int j = 0; j < 32; ++j) {
     const int r = rs * 32 + j;
     v[j] = r < R ? ldexp(src[(long long)r * ncols], -e) : 0.0;
   }
-  signed char* dst = FQ + o.fq + ((long long)ct * o.nrs + rs) * K * 1024 + cl * 16;
+  signed char* dst = FQ + (long long)ncomp * o.fq + ((long long)ct * o.nrs + rs) * K * 1024 + cl * 16;
   put16<K>(v, dst);
   put16<K>(v + 16, dst + 32 * 16);
 }
@@ -262,15 +422,16 @@ __global__ __launch_bounds__(256) void k_quant_F(const MixOff* __restrict__ mo, 
 // ---------------------------------------------------------------------------
 // Synthesis operand: packed alm -> AQ slices with one exponent per column
 // (AQe, index (2 m + class) * ncols + col).  Columns as in the m-major
-// buffer: col < nsimp is Re a_lm of map col, nsimp <= col < 2 nsimp is Im.
-// Workgroup = (m, class, 32-column tile).  HAVE_EXP = 0: the column maxima
-// first (setting AQe), then the digits (second read of the same alm, mostly
-// from cache); HAVE_EXP = 1: AQe was set by the analysis that wrote alm.
+// buffer: per component c (alm0, then alm1 when ncomp = 2) the block
+// [Re a_lm of the nsimp maps, Im a_lm of the nsimp maps].  Workgroup = (m,
+// class, 32-column tile).  HAVE_EXP = 0: the column maxima first (setting
+// AQe), then the digits (second read of the same alm, mostly from cache);
+// HAVE_EXP = 1: AQe was set by the analysis that wrote alm.
 // ---------------------------------------------------------------------------
 template <int K, int HAVE_EXP>
-__global__ __launch_bounds__(256) void k_quant_A(const MixOff* __restrict__ mo, const double* __restrict__ alm,
-                                                 signed char* __restrict__ AQ, int* __restrict__ AQe, int nsim,
-                                                 int nsimp, int lmax) {
+__global__ __launch_bounds__(256) void k_quant_A(const MixOff* __restrict__ mo, const double* __restrict__ alm0,
+                                                 const double* __restrict__ alm1, signed char* __restrict__ AQ,
+                                                 int* __restrict__ AQe, int ncomp, int nsim, int nsimp, int lmax) {
   __shared__ double red[8][32];
   __shared__ int ex[32];
   const int m = blockIdx.x, cls = blockIdx.y, ct = blockIdx.z;
@@ -278,12 +439,14 @@ __global__ __launch_bounds__(256) void k_quant_A(const MixOff* __restrict__ mo, 
   const MixOff o = mo[2 * m + cls];
   const int cl = threadIdx.x & 31, q = threadIdx.x >> 5;
   const int col = ct * 32 + cl;
-  const int ncols = 2 * nsimp;
-  const int reim = col >= nsimp ? 1 : 0;
-  const int sim = col - reim * nsimp;
+  const int ncols = 2 * ncomp * nsimp;
+  const int comp = col / (2 * nsimp);
+  const int rem = col - comp * 2 * nsimp;
+  const int reim = rem >= nsimp ? 1 : 0;
+  const int sim = rem - reim * nsimp;
   const bool ok = sim < nsim && !(m == 0 && reim);
   const long long L1 = lmax + 1;
-  const double* src = alm + (ok ? sim : 0) + (long long)nsim * (reim ? (L1 * (L1 - m) - m) : (L1 * m));
+  const double* src = (comp ? alm1 : alm0) + (ok ? sim : 0) + (long long)nsim * (reim ? (L1 * (L1 - m) - m) : (L1 * m));
   auto val = [&](int lst, int j) -> double {
     const int l = m + cls + 2 * (lst * 32 + j);
     return (ok && l <= lmax) ? src[(long long)nsim * l] : 0.0;
@@ -311,29 +474,18 @@ __global__ __launch_bounds__(256) void k_quant_A(const MixOff* __restrict__ mo, 
     double v[32];
 #pragma unroll
     for (int j = 0; j < 32; ++j) v[j] = ldexp(val(lst, j), -e);
-    signed char* dst = AQ + o.aq + ((long long)ct * o.nlt + lst) * K * 1024 + cl * 16;
+    signed char* dst = AQ + (long long)ncomp * o.aq + ((long long)ct * o.nlt + lst) * K * 1024 + cl * 16;
     put16<K>(v, dst);
     put16<K>(v + 16, dst + 32 * 16);
   }
 }
 
 // ---------------------------------------------------------------------------
-// The slice GEMM.  Workgroup = 8 waves (two per SIMD) on (m, class, NA row
-// tiles, NB column tiles): NB = 4 with NA = 4 (wave w owns row tile w/2 and
-// column tiles 2(w%2), 2(w%2)+1), or NB = 1 with NA = 8 (wave w owns row tile
-// w and the column tile) for the last column tiles of a batch whose tile
-// count is not a multiple of 4.  Per 32-wide step of the contraction the
-// workgroup needs (NA + NB) K blocks of 1 KB (K slices of its A and B tiles);
-// the waves copy them global -> LDS with direct-to-LDS loads
-// (global_load_lds_dwordx4: 1 KB per wave instruction, lane order kept) into a
-// ring of NS stages, NS - 1 steps ahead of the MFMAs.  Each step every wave
-// waits for its own loads of the step (vmcnt), one barrier publishes them,
-// the stage that every wave finished with is refilled, and the wave reads its
-// operands from LDS and issues K(K+1)/2 digit products per B tile into one
-// int32 accumulator per digit sum.
-//   SYN = 0 (analysis): A = TA (rows l), B = FQ (columns), out = alm
-//   SYN = 1 (synthesis): A = TS (rows ring), B = AQ (columns), out = G
+// Slice GEMM building blocks
 // ---------------------------------------------------------------------------
+#ifndef FSHT_SLICE_PP
+#define FSHT_SLICE_PP 1  // 0: the single-group slice GEMM for every slice count
+#endif
 constexpr int kLdsBudget = 147456;  // bytes of LDS the slice GEMM may use
 
 // s_waitcnt with vmcnt <= N and the other counters unconstrained (gfx9 encoding)
@@ -354,17 +506,164 @@ __device__ inline double combine(const v16i (&D)[K], int r) {
   return v;
 }
 
+// Arguments of the slice GEMM kernels.
+struct SliceArgs {
+  const LegItem* items;
+  const MixOff* mo;
+  const signed char* TP;  // left operand: the Legendre table (P, F1 or F2) in TA or TS order
+  const int* TPe;         // its row exponents
+  long long abase;        // the table's operand blocks sit at (o.ta or o.ts) / kMixK * ast - abase
+  int ast;                // slices stored per contraction step of the table
+  const signed char* OQ;  // right operand (FQ or AQ) and its column exponents
+  const int* OQe;
+  double* out0;  // analysis: alm of component 0 (T, or E); synthesis: the m-major buffer G
+  double* out1;  // analysis, spin 2: the B alm
+  const long long* goff;
+  const int* i0tab;
+  int nsim, nsimp, ncomp, lmax, nhr, nct;
+  // 0: spin 0, or F1 (an output column is its input column); 1: F2 (the
+  // input column of component c, part r feeds component 1 - c, part 1 - r)
+  int fn;
+  double pa, beta;
+  int* aqe;  // analysis: column exponents of the result for the next synthesis operand (or NULL)
+};
+
+// Column map of the spin-2 contractions (QU2EB / EB2QU, src/sht_main.f90:383-436
+// and 1079-1147 with the alpha/swap table of sht_data_init.f90:606-632).
+// With columns [Q_re, Q_im, U_re, U_im] (analysis) or [E_re, E_im, B_re,
+// B_im] (synthesis):
+//   analysis  E_re = -pa F1.Q_re + pa F2.U_im   E_im = -pa F1.Q_im - pa F2.U_re
+//             B_re = -pa F1.U_re - pa F2.Q_im   B_im = -pa F1.U_im + pa F2.Q_re
+//   synthesis Q_re = -F1.E_re + F2.B_im         Q_im = -F1.E_im - F2.B_re
+//             U_re = -F1.B_re - F2.E_im         U_im = -F1.B_im + F2.E_re
+// so F1 keeps the column with factor -1 and F2 swaps both the component and
+// Re/Im with factor +1 where they agree, -1 where they differ.
+__device__ inline void spin2_map(int fn, int comp, int reim, int& comp_o, int& reim_o, double& sgn) {
+  if (fn == 0) {
+    comp_o = comp;
+    reim_o = reim;
+    sgn = -1.0;
+  } else {
+    comp_o = 1 - comp;
+    reim_o = 1 - reim;
+    sgn = comp == reim ? 1.0 : -1.0;
+  }
+}
+
+// Epilogue of one wave's output tiles (row tile ta, column tiles tb0 ..
+// tb0 + TPW - 1): value = 2^(eA(row) + eB(col)) * sum_d 2^-7(d+2) D_d.
+//   SYN = 0: alm = pa * sgn * value + beta * alm (the column exponents of
+//            the result are folded into aqe when given)
+//   SYN = 1: G = sgn * value (+ G when beta != 0), in the fold of the table's
+//            parity (F2: the other fold)
+template <int K, int SYN, int TPW>
+__device__ inline void slice_epilogue(const v16i (&D0)[K], const v16i (&D1)[K], const SliceArgs& A, const MixOff& o,
+                                      int m, int cls, int bcls, int ta, int tb0, int L) {
+  const int ncols = 2 * A.ncomp * A.nsimp;
+  const int h = L >> 5;
+  const int* pe = A.TPe + (SYN ? o.tse : o.tae) + (long long)ta * 32 + 16 * h;  // accumulator order
+#pragma unroll
+  for (int t = 0; t < TPW; ++t) {
+    const int tb = tb0 + t;
+    if (tb >= A.nct) break;
+    const int col = tb * 32 + (L & 31);
+    const int eB = A.OQe[(long long)(2 * m + bcls) * ncols + col];
+    const int comp = col / (2 * A.nsimp);
+    const int rem = col - comp * 2 * A.nsimp;
+    const int reim = rem >= A.nsimp ? 1 : 0;
+    const int sim = rem - reim * A.nsimp;
+    int comp_o = comp, reim_o = reim;
+    double sgn = 1.0;
+    if (A.ncomp == 2) spin2_map(A.fn, comp, reim, comp_o, reim_o, sgn);
+    if (!SYN) {
+      if (sim >= A.nsim || (m == 0 && reim_o)) continue;
+      const long long L1 = A.lmax + 1;
+      // this column's entries: sim + nsim (l + L1 m) for Re, sim + nsim (l - m + L1 (L1 - m)) for Im
+      double* oc = (comp_o ? A.out1 : A.out0) + sim + (long long)A.nsim * (reim_o ? L1 * (L1 - m) - m : L1 * m);
+      // the previous values (beta != 0) are all loaded before any store, so
+      // the read-modify-write costs one memory latency, not sixteen
+      double prev[16];
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int l = m + cls + 2 * (ta * 32 + drow(r, h));
+        prev[r] = (A.beta != 0.0 && l <= A.lmax) ? oc[(long long)A.nsim * l] : 0.0;
+      }
+      const double f = A.pa * sgn;
+      double mx = 0.0;
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int l = m + cls + 2 * (ta * 32 + drow(r, h));
+        if (l > A.lmax) continue;
+        const double v = f * __builtin_ldexp(combine<K>(t ? D1 : D0, r), pe[r] + eB) + A.beta * prev[r];
+        oc[(long long)A.nsim * l] = v;
+        mx = fmax(mx, fabs(v));
+      }
+      // the synthesis operand's exponent of this output column: max over
+      // every workgroup writing it (k_quant_A<K, 1> reads it)
+      if (A.aqe) {
+        const int col_o = comp_o * 2 * A.nsimp + reim_o * A.nsimp + sim;
+        atomicMax(A.aqe + (long long)(2 * m + cls) * ncols + col_o, mix_exp(mx));
+      }
+    } else {
+      const int R = A.nhr + 1 - A.i0tab[m];
+      const int fold = A.fn ? 1 - cls : cls;
+      const int col_o = comp_o * 2 * A.nsimp + reim_o * A.nsimp + sim;
+      double* g = A.out0 + A.goff[m] * A.ncomp / 2 + (long long)fold * R * ncols + col_o;
+      double prev[16];
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int row = ta * 32 + drow(r, h);
+        prev[r] = (A.beta != 0.0 && row < R) ? g[(long long)row * ncols] : 0.0;
+      }
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int row = ta * 32 + drow(r, h);
+        if (row < R) g[(long long)row * ncols] = sgn * __builtin_ldexp(combine<K>(t ? D1 : D0, r), pe[r] + eB) + prev[r];
+      }
+    }
+  }
+}
+
+// Source of the 1 KB operand block `b` of a stage at step s0 (A tiles
+// first: NA * K blocks, then B tiles: NB * K blocks) and its per-step stride.
+template <int K, int SYN, int NA, int NB>
+__device__ inline void slice_src(const SliceArgs& A, const MixOff& o, const MixOff& ob, const LegItem& it, int nsteps,
+                                 int ntA, int s0, int b, const signed char*& src, int& step) {
+  if (b < NA * K) {
+    const int tile = min(it.blk * NA + b / K, ntA - 1);
+    const signed char* baseA = A.TP + (SYN ? o.ts : o.ta) / kMixK * A.ast - A.abase;
+    src = baseA + (((long long)tile * nsteps + s0) * A.ast + b % K) * 1024;
+    step = A.ast * 1024;
+  } else {
+    const int bb = b - NA * K;
+    const int tile = min(it.colblk * NB + bb / K, A.nct - 1);
+    const signed char* baseB = A.OQ + (long long)A.ncomp * (SYN ? ob.aq : ob.fq);
+    src = baseB + (((long long)tile * nsteps + s0) * K + bb % K) * 1024;
+    step = K * 1024;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// The slice GEMM.  Workgroup = 8 waves (two per SIMD) on (m, class, NA row
+// tiles, NB column tiles): NB = 4 with NA = 4 (wave w owns row tile w/2 and
+// column tiles 2(w%2), 2(w%2)+1), or NB = 1 with NA = 8 (wave w owns row tile
+// w and the column tile) for the last column tiles of a batch whose tile
+// count is not a multiple of 4.  Per 32-wide step of the contraction the
+// workgroup needs (NA + NB) K blocks of 1 KB (K slices of its A and B tiles);
+// the waves copy them global -> LDS with direct-to-LDS loads
+// (global_load_lds_dwordx4: 1 KB per wave instruction, lane order kept) into a
+// ring of NS stages, NS - 1 steps ahead of the MFMAs.  Each step every wave
+// waits for its own loads of the step (vmcnt), one barrier publishes them,
+// the stage that every wave finished with is refilled, and the wave reads its
+// operands from LDS and issues K(K+1)/2 digit products per B tile into one
+// int32 accumulator per digit sum.
+//   SYN = 0 (analysis): A = TA (rows l), B = FQ (columns), out = alm
+//   SYN = 1 (synthesis): A = TS (rows ring), B = AQ (columns), out = G
+// For F2 (A.fn = 1) the analysis reads the operand of the other fold: F2 of
+// parity class c pairs with the N -/+ S fold of class 1 - c.
+// ---------------------------------------------------------------------------
 template <int K, int SYN, int NB>
-__global__ __launch_bounds__(512, 1) void k_slice_gemm(const LegItem* __restrict__ items,
-                                                       const MixOff* __restrict__ mo,
-                                                       const signed char* __restrict__ TP,
-                                                       const int* __restrict__ TPe,
-                                                       const signed char* __restrict__ OQ,
-                                                       const int* __restrict__ OQe, double* __restrict__ out,
-                                                       const long long* __restrict__ goff,
-                                                       const int* __restrict__ i0tab, int nsim, int nsimp, int lmax,
-                                                       int nhr, int nct, double pa, double beta,
-                                                       int* __restrict__ aqe) {
+__global__ __launch_bounds__(512, 1) void k_slice_gemm(const SliceArgs A) {
   static_assert(NB == 4 || NB == 1, "column tiles per workgroup");
   constexpr int NW = 8;                                 // waves
   constexpr int NA = NB == 4 ? 4 : 8;                   // row tiles
@@ -374,23 +673,21 @@ __global__ __launch_bounds__(512, 1) void k_slice_gemm(const LegItem* __restrict
   constexpr int NS = kLdsBudget / (NBLK * 1024) < 4 ? kLdsBudget / (NBLK * 1024) : 4;  // stages
   static_assert(NS >= 3, "pipeline needs three stages");
   __shared__ __align__(16) signed char st[NS][NBLK * 1024];
-  const LegItem it = items[blockIdx.x];
+  const LegItem it = A.items[blockIdx.x];
   if (it.m < 0) return;  // whole workgroup
-  if (!cols_active(it.colblk * NB * 32, (it.colblk + 1) * NB * 32, nsim, nsimp)) return;  // no map in the block
+  if (!cols_active(it.colblk * NB * 32, (it.colblk + 1) * NB * 32, A.nsim, A.nsimp)) return;  // no map in the block
   const int m = it.m, cls = it.par;
+  const int bcls = (!SYN && A.fn) ? 1 - cls : cls;  // class (fold) of the right operand
   const int w = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
   const int L = threadIdx.x & 63;
-  const MixOff o = mo[2 * m + cls];
+  const MixOff o = A.mo[2 * m + cls];
+  const MixOff ob = A.mo[2 * m + bcls];
   const int nsteps = SYN ? o.nlt : o.nrs;
   const int ntA = SYN ? o.nrs : o.nlt;
-  const int ncols = 2 * nsimp;
-  const signed char* baseA = TP + (SYN ? o.ts : o.ta);
-  const signed char* baseB = OQ + (SYN ? o.aq : o.fq);
 
   const int s0 = it.r0;
   // this wave's PER source streams: wave-uniform pointers at step s0 that
-  // advance by one step's stride per issue (A tiles keep kMixK slices per
-  // step, B tiles K), and their blocks within a stage
+  // advance by one step's stride per issue, and their blocks within a stage
   const signed char* gsrc[PER];
   int gstep[PER], lofs[PER];
 #pragma unroll
@@ -398,16 +695,7 @@ __global__ __launch_bounds__(512, 1) void k_slice_gemm(const LegItem* __restrict
     int b = w + NW * k;
     if (b >= NBLK) b -= NW;  // this wave's previous block again (same data, same place)
     lofs[k] = b * 1024;
-    if (b < NA * K) {
-      const int tile = min(it.blk * NA + b / K, ntA - 1);
-      gsrc[k] = baseA + (((long long)tile * nsteps + s0) * kMixK + b % K) * 1024;
-      gstep[k] = kMixK * 1024;
-    } else {
-      const int bb = b - NA * K;
-      const int tile = min(it.colblk * NB + bb / K, nct - 1);
-      gsrc[k] = baseB + (((long long)tile * nsteps + s0) * K + bb % K) * 1024;
-      gstep[k] = K * 1024;
-    }
+    slice_src<K, SYN, NA, NB>(A, o, ob, it, nsteps, ntA, s0, b, gsrc[k], gstep[k]);
   }
   int fill = 0;  // stage the next issue fills (steps s0, s0 + 1, ... go to stages 0, 1, ..., NS - 1, 0, ...)
   auto issue = [&]() {
@@ -424,7 +712,7 @@ __global__ __launch_bounds__(512, 1) void k_slice_gemm(const LegItem* __restrict
   const int wb = NB == 4 ? (w & 1) * 2 : 0;      // its first column tile within the workgroup
   const int ta = it.blk * NA + wa;
   const int tb0 = it.colblk * NB + wb;
-  const bool act = ta < ntA && tb0 < nct;
+  const bool act = ta < ntA && tb0 < A.nct;
   v16i D0[K], D1[K];
 #pragma unroll
   for (int d = 0; d < K; ++d) D0[d] = D1[d] = v16i{};
@@ -466,53 +754,146 @@ __global__ __launch_bounds__(512, 1) void k_slice_gemm(const LegItem* __restrict
     }
   }
   if (!act) return;
+  slice_epilogue<K, SYN, TPW>(D0, D1, A, o, m, cls, bcls, ta, tb0, L);
+}
 
-  // epilogue: out = 2^(eA(row) + eB(col)) * sum_d 2^-7(d+2) D_d
-  const int h = L >> 5;
-  const int* pe = TPe + (SYN ? o.tse : o.tae) + (long long)ta * 32 + 16 * h;  // accumulator order
+// ---------------------------------------------------------------------------
+// The slice GEMM with two wave groups in alternation ("ping-pong").  Same
+// work items, tiles and operand streams as k_slice_gemm, but the 8 waves
+// form two groups of 4 (waves 0-3 and 4-7, one wave of each group per SIMD)
+// whose phases alternate: while one group issues its step's MFMAs from
+// registers, the other reads its next operands from LDS, so the matrix pipe
+// of every SIMD is fed by one group or the other and the LDS reads no longer
+// sit between a barrier and the first MFMA.  Group 0 issues every
+// direct-to-LDS load.
+// ---------------------------------------------------------------------------
+// s_waitcnt lgkmcnt(0) with the other counters unconstrained (gfx9 encoding)
+__device__ inline void wait_lgkm0() { __builtin_amdgcn_s_waitcnt(0xF | (0x3 << 14) | (0x7 << 4)); }
+// keeps the scheduler from moving any instruction across this point
+__device__ inline void sched_fence() { __builtin_amdgcn_sched_barrier(0); }
+
+template <int K, int SYN, int NB>
+__global__ __launch_bounds__(512, 1) void k_slice_gemm_pp(const SliceArgs A) {
+  static_assert(NB == 4 || NB == 1, "column tiles per workgroup");
+  constexpr int NG = 4;                                 // waves per group
+  constexpr int NA = NB == 4 ? 4 : 8;                   // row tiles
+  constexpr int TPW = NB == 4 ? 2 : 1;                  // column tiles per wave
+  constexpr int NBLK = (NA + NB) * K;                   // 1 KB blocks per step
+  constexpr int PER = (NBLK + NG - 1) / NG;             // loads per group-0 wave per step (the last may repeat)
+  constexpr int NS = kLdsBudget / (NBLK * 1024) < 4 ? kLdsBudget / (NBLK * 1024) : 4;  // stages
+  static_assert(NS >= 3, "pipeline needs three stages");
+  static_assert((NS - 1) * PER < 64, "vmcnt range");
+  __shared__ __align__(16) signed char st[NS][NBLK * 1024];
+  const LegItem it = A.items[blockIdx.x];
+  if (it.m < 0) return;  // whole workgroup
+  if (!cols_active(it.colblk * NB * 32, (it.colblk + 1) * NB * 32, A.nsim, A.nsimp)) return;  // no map in the block
+  const int m = it.m, cls = it.par;
+  const int bcls = (!SYN && A.fn) ? 1 - cls : cls;  // class (fold) of the right operand
+  const int w = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
+  const int grp = w >> 2;
+  const int L = threadIdx.x & 63;
+  const MixOff o = A.mo[2 * m + cls];
+  const MixOff ob = A.mo[2 * m + bcls];
+  const int nsteps = SYN ? o.nlt : o.nrs;
+  const int ntA = SYN ? o.nrs : o.nlt;
+  const int s0 = it.r0;
+
+  // group 0's source streams (wave-uniform pointers at step s0, advanced by
+  // one step's stride per issue) and their blocks within a stage
+  const signed char* gsrc[PER];
+  int gstep[PER], lofs[PER];
 #pragma unroll
-  for (int t = 0; t < TPW; ++t) {
-    const int tb = tb0 + t;
-    if (tb >= nct) break;
-    const int col = tb * 32 + (L & 31);
-    const int eB = OQe[(long long)(2 * m + cls) * ncols + col];
-    if (!SYN) {
-      const int reim = col >= nsimp ? 1 : 0;
-      const int sim = col - reim * nsimp;
-      if (sim >= nsim || (m == 0 && reim)) continue;
-      const long long L1 = lmax + 1;
-      // this column's entries: sim + nsim (l + L1 m) for Re, sim + nsim (l - m + L1 (L1 - m)) for Im
-      double* oc = out + sim + (long long)nsim * (reim ? L1 * (L1 - m) - m : L1 * m);
-      // the previous values (beta != 0) are all loaded before any store, so
-      // the read-modify-write costs one memory latency, not sixteen
-      double prev[16];
-#pragma unroll
-      for (int r = 0; r < 16; ++r) {
-        const int l = m + cls + 2 * (ta * 32 + drow(r, h));
-        prev[r] = (beta != 0.0 && l <= lmax) ? oc[(long long)nsim * l] : 0.0;
-      }
-      double mx = 0.0;
-#pragma unroll
-      for (int r = 0; r < 16; ++r) {
-        const int l = m + cls + 2 * (ta * 32 + drow(r, h));
-        if (l > lmax) continue;
-        const double v = pa * __builtin_ldexp(combine<K>(t ? D1 : D0, r), pe[r] + eB) + beta * prev[r];
-        oc[(long long)nsim * l] = v;
-        mx = fmax(mx, fabs(v));
-      }
-      // the synthesis operand's exponent of this column: max over every
-      // workgroup writing it (k_quant_A<K, 1> reads it)
-      if (aqe) atomicMax(aqe + (long long)(2 * m + cls) * ncols + col, mix_exp(mx));
-    } else {
-      const int R = nhr + 1 - i0tab[m];
-      const long long gbase = goff[m] / 2 + (long long)cls * R * ncols;
-#pragma unroll
-      for (int r = 0; r < 16; ++r) {
-        const int row = ta * 32 + drow(r, h);
-        if (row < R) out[gbase + (long long)row * ncols + col] = __builtin_ldexp(combine<K>(t ? D1 : D0, r), pe[r] + eB);
-      }
-    }
+  for (int k = 0; k < PER; ++k) {
+    int b = (w & 3) + NG * k;
+    if (b >= NBLK) b -= NG;  // this wave's previous block again (same data, same place)
+    lofs[k] = b * 1024;
+    slice_src<K, SYN, NA, NB>(A, o, ob, it, nsteps, ntA, s0, b, gsrc[k], gstep[k]);
   }
+  auto issue = [&](int stage) {
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+      __builtin_amdgcn_global_load_lds((const void*)(gsrc[k] + L * 16), (lds_ptr)(st[stage] + lofs[k]), 16, 0, 0);
+      gsrc[k] += gstep[k];
+    }
+  };
+  // group 0: wait until the loads of step `s` are done, given that the
+  // loads of the steps up to `last` were issued (at most NS - 2 after s)
+  auto wait_step = [&](int s, int last) {
+    const int ahead = min(NS - 2, last - s);
+    if (NS > 3 && ahead >= 2) wait_vm<(NS > 3 ? 2 : 1) * PER>();
+    else if (ahead >= 1) wait_vm<PER>();
+    else wait_vm<0>();
+  };
+
+  const int wa = NB == 4 ? (w >> 1) : w;         // this wave's row tile within the workgroup
+  const int wb = NB == 4 ? (w & 1) * 2 : 0;      // its first column tile within the workgroup
+  const int ta = it.blk * NA + wa;
+  const int tb0 = it.colblk * NB + wb;
+  const bool act = ta < ntA && tb0 < A.nct;
+  v16i D0[K], D1[K];
+#pragma unroll
+  for (int d = 0; d < K; ++d) D0[d] = D1[d] = v16i{};
+  v4i ra[K], rb[TPW][K];
+  auto load_regs = [&](int stage) {
+    const signed char* sb = st[stage] + L * 16;
+#pragma unroll
+    for (int sl = 0; sl < K; ++sl) ra[sl] = *reinterpret_cast<const v4i*>(sb + (wa * K + sl) * 1024);
+#pragma unroll
+    for (int t = 0; t < TPW; ++t)
+#pragma unroll
+      for (int sl = 0; sl < K; ++sl) rb[t][sl] = *reinterpret_cast<const v4i*>(sb + (NA * K + (wb + t) * K + sl) * 1024);
+  };
+  auto mma = [&]() {
+#pragma unroll
+    for (int j = 0; j < K; ++j)
+#pragma unroll
+      for (int i = 0; i < K - j; ++i) {
+        D0[i + j] = mfma_i8(ra[i], rb[0][j], D0[i + j]);
+        if (TPW == 2) D1[i + j] = mfma_i8(ra[i], rb[TPW - 1][j], D1[i + j]);
+      }
+  };
+
+  // Both groups run the same loop, MEM(t) then MMA(t) per step, group 1
+  // one phase behind group 0 (it starts with an idle phase and skips the
+  // barrier after its last MMA phase, so both pass the same barriers).
+  //   MEM(t): read stage t into registers; group 0 refills stage t - 1
+  //           (read by both groups by now) with step t - 1 + NS.
+  //   MMA(t): the step's MFMAs; group 0 then waits for its loads of t + 1.
+  // Group 0's MEM(t) runs beside group 1's MMA(t - 1), its MMA(t) beside
+  // group 1's MEM(t).  Stage t is published by the barrier that ends group
+  // 0's MMA(t - 1) (the prologue barrier for t = s0); group 1 completes its
+  // LDS reads (lgkmcnt 0) before each barrier, so no refill overtakes them.
+  // An item without steps passes no barrier at all (s0 is uniform).
+  if (s0 < nsteps) {
+    if (grp == 0) {
+#pragma unroll
+      for (int k = 0; k < NS; ++k)
+        if (s0 + k < nsteps) issue(k);
+      wait_step(s0, min(s0 + NS - 1, nsteps - 1));
+    }
+    sched_fence();
+    raw_barrier();
+    if (grp == 1) raw_barrier();
+    sched_fence();
+  }
+  int cur = 0, prev = NS - 1;  // stages of steps t and t - 1
+  for (int t = s0; t < nsteps; ++t) {
+    if (act) load_regs(cur);
+    if (grp == 0 && t > s0 && t - 1 + NS < nsteps) issue(prev);
+    wait_lgkm0();
+    sched_fence();
+    raw_barrier();
+    sched_fence();
+    if (act) mma();
+    if (grp == 0 && t + 1 < nsteps) wait_step(t + 1, min(t + NS - 1, nsteps - 1));
+    sched_fence();
+    if (grp == 0 || t + 1 < nsteps) raw_barrier();
+    sched_fence();
+    prev = cur;
+    cur = cur + 1 == NS ? 0 : cur + 1;
+  }
+  if (!act) return;
+  slice_epilogue<K, SYN, TPW>(D0, D1, A, o, m, cls, bcls, ta, tb0, L);
 }
 
 // ---------------------------------------------------------------------------
@@ -520,22 +901,43 @@ __global__ __launch_bounds__(512, 1) void k_slice_gemm(const LegItem* __restrict
 // ---------------------------------------------------------------------------
 static int cdiv(long long a, long long b) { return (int)((a + b - 1) / b); }
 
+static void free_lists(std::vector<LegItem*>& v) {
+  for (LegItem* p : v)
+    if (p) (void)hipFree(p);
+  v.clear();
+}
+
 void mixed_free(Handle* h) {
-  void* ptrs[] = {h->d_mix_off, h->d_TA, h->d_TS, h->d_FQ, h->d_AQ, h->d_TAe, h->d_TSe, h->d_FQe, h->d_AQe,
-                  h->d_firstA,  h->d_firstS};
+  void* ptrs[] = {h->d_mix_off, h->d_TA,     h->d_TS,     h->d_AQ,     h->d_TAe,    h->d_TSe,     h->d_FQe,
+                  h->d_AQe,     h->d_firstA, h->d_firstS, h->d_pitems, h->d_T2[0], h->d_T2[1], h->d_TAe2[0],
+                  h->d_TAe2[1], h->d_TSe2[0], h->d_TSe2[1], h->d_firstA2[0], h->d_firstA2[1], h->d_firstS2[0],
+                  h->d_firstS2[1]};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
+  if (h->d_FQ && !h->fq_alias) (void)hipFree(h->d_FQ);
   for (int k = 0; k < 3; ++k) {
     if (h->d_anaq_items[k]) (void)hipFree(h->d_anaq_items[k]);
     if (h->d_synq_items[k]) (void)hipFree(h->d_synq_items[k]);
     h->d_anaq_items[k] = h->d_synq_items[k] = nullptr;
     h->n_anaq_items[k] = h->n_synq_items[k] = 0;
   }
+  free_lists(h->d_anaq2);
+  free_lists(h->d_synq2);
+  h->n_anaq2.clear();
+  h->n_synq2.clear();
   h->d_mix_off = nullptr;
   h->d_TA = h->d_TS = h->d_FQ = h->d_AQ = nullptr;
   h->d_TAe = h->d_TSe = h->d_FQe = h->d_AQe = nullptr;
   h->d_firstA = h->d_firstS = nullptr;
-  h->mix_ready = 0;
+  h->d_pitems = nullptr;
+  for (int f = 0; f < 2; ++f) {
+    h->d_T2[f] = nullptr;
+    h->d_TAe2[f] = h->d_TSe2[f] = nullptr;
+    h->d_firstA2[f] = h->d_firstS2[f] = nullptr;
+  }
+  h->fq_alias = 0;
+  h->mix_ready = h->mix0_ready = h->mix2_ready = 0;
+  h->t2_lay = -1;
   h->mix_bytes = 0;
 }
 
@@ -546,19 +948,81 @@ static hipError_t fill_int(Handle* h, int* p, long long n, int v) {
   return hipGetLastError();
 }
 
-hipError_t mixed_setup(Handle* h) {
+// Slice-GEMM work lists of the orders [m0, m1) for a batch of nct 32-column
+// tiles: [0] full blocks of 4 column tiles (4 row tiles each), [1] the
+// remaining column tiles one by one (8 row tiles each), [2] the remaining
+// column tiles as one padded block of 4 (six slices).  Every (m, class, row
+// tiles, column tiles) writes its outputs, starting at the first step with a
+// nonzero table digit of its row tiles.
+static hipError_t build_q_lists(Handle* h, int m0, int m1, int nct, const std::vector<int>& firstA,
+                                const std::vector<int>& firstS, LegItem** ana_out, int* nana, LegItem** syn_out,
+                                int* nsyn) {
+  const int nfull = nct / 4, nrem = nct % 4;
+  for (int k = 0; k < 3; ++k) {
+    ana_out[k] = syn_out[k] = nullptr;
+    nana[k] = nsyn[k] = 0;
+  }
+  for (int k = 0; k < 3; ++k) {
+    if (k > 0 && nrem == 0) break;
+    const int NA = k == 1 ? 8 : 4;
+    std::vector<LegItem> ana, syn;
+    std::vector<double> ana_cost, syn_cost;
+    // column blocks of this list (colblk counts blocks of NB tiles) and their tile counts
+    std::vector<int> cbs, cnt;
+    if (k == 0)
+      for (int cb = 0; cb < nfull; ++cb) cbs.push_back(cb), cnt.push_back(4);
+    else if (k == 1)
+      for (int r = 0; r < nrem; ++r) cbs.push_back(4 * nfull + r), cnt.push_back(1);
+    else
+      cbs.push_back(nfull), cnt.push_back(4);
+    for (int m = m0; m < m1; ++m)
+      for (int c = 0; c < 2; ++c) {
+        const MixOff& o = h->mix_off[2 * m + c];
+        for (int b = 0; NA * b < o.nlt; ++b) {
+          int r0 = o.nrs;
+          for (int t = NA * b; t < std::min(o.nlt, NA * b + NA); ++t) r0 = std::min(r0, firstA[o.fa + t]);
+          for (size_t q = 0; q < cbs.size(); ++q) {
+            ana.push_back(LegItem{m, c, b, cbs[q], r0});
+            ana_cost.push_back(1.0 + (double)(o.nrs - r0) * NA * cnt[q]);
+          }
+        }
+        for (int b = 0; NA * b < o.nrs; ++b) {
+          int l0 = o.nlt;
+          for (int t = NA * b; t < std::min(o.nrs, NA * b + NA); ++t) l0 = std::min(l0, firstS[o.fs + t]);
+          for (size_t q = 0; q < cbs.size(); ++q) {
+            syn.push_back(LegItem{m, c, b, cbs[q], l0});
+            syn_cost.push_back(1.0 + (double)(o.nlt - l0) * NA * cnt[q]);
+          }
+        }
+      }
+    hipError_t e = upload_items(ana, ana_cost, &ana_out[k], &nana[k]);
+    if (e == hipSuccess) e = upload_items(syn, syn_cost, &syn_out[k], &nsyn[k]);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
+// Offsets, sliced operands (room for the handle's field components) and the
+// (m, ring block) items of the table kernels: shared by spin 0 and spin 2.
+// The analysis operand FQ lives in the map-major spectrum buffer when it
+// fits there: that buffer only holds data between a transform's ring FFTs
+// and its transposes, while FQ only lives between a gather and the analysis.
+static hipError_t mixed_setup_common(Handle* h) {
   if (h->mix_ready) return hipSuccess;
   const int lmax = h->lmax, nm = lmax + 1;
-  const int nct = h->nsimp / 16;  // 32-column tiles of the 2*nsimp spin-0 columns
-  const int ncols = 2 * h->nsimp;
+  const int ncw = h->ncomp_ws;
+  const int nct = h->nsimp / 16;  // 32-column tiles of the 2*nsimp columns of one component
+  const int ncols = 2 * ncw * h->nsimp;
   h->mix_off.assign(2 * (size_t)nm, MixOff{});
   long long ta = 0, fq = 0, aq = 0, tae = 0, tse = 0;
   int fa = 0, fs = 0, nrb_max = 0;
   std::vector<int2> pitems;
+  h->pitem_off.assign((size_t)nm + 1, 0);
   for (int m = 0; m <= lmax; ++m) {
     const int R = h->nhr + 1 - h->i0[m];
     const int nrs = cdiv(R, 32), nrb = cdiv(nrs, 8);
     nrb_max = std::max(nrb_max, nrb);
+    h->pitem_off[m] = (int)pitems.size();
     for (int rb = 0; rb < nrb; ++rb) pitems.push_back(make_int2(m, rb));
     for (int c = 0; c < 2; ++c) {
       const int nval = (lmax >= m + c) ? (lmax - m - c) / 2 + 1 : 0;
@@ -583,114 +1047,205 @@ hipError_t mixed_setup(Handle* h) {
       fs += nrs;
     }
   }
+  h->pitem_off[nm] = (int)pitems.size();
   h->mix_nrb_max = nrb_max;
+  h->mix_ta_total = ta;
+  h->mix_tae = tae;
+  h->mix_tse = tse;
+  h->mix_fa = fa;
+  h->mix_fs = fs;
   const long long nexp = 2LL * nm * ncols;
+  const long long fq_bytes = (long long)ncw * fq;
+  hipError_t e = hipSuccess;
+  auto al = [&](void** p, long long bytes) {
+    if (e == hipSuccess) e = hipMalloc(p, (size_t)std::max(bytes, 256LL));
+  };
+  if (fq_bytes <= (long long)ncw * h->nsim_cap * h->npix * 8) {
+    h->d_FQ = reinterpret_cast<signed char*>(h->ws_spec);
+    h->fq_alias = 1;
+  } else {
+    al((void**)&h->d_FQ, fq_bytes);
+  }
+  al((void**)&h->d_AQ, (long long)ncw * aq);
+  al((void**)&h->d_FQe, nexp * 4);
+  al((void**)&h->d_AQe, nexp * 4);
+  al((void**)&h->d_mix_off, (long long)h->mix_off.size() * (long long)sizeof(MixOff));
+  al((void**)&h->d_pitems, (long long)pitems.size() * (long long)sizeof(int2));
+  if (e == hipSuccess)
+    e = hipMemcpy(h->d_mix_off, h->mix_off.data(), h->mix_off.size() * sizeof(MixOff), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(h->d_pitems, pitems.data(), pitems.size() * sizeof(int2), hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    mixed_free(h);
+    return e;
+  }
+  h->mix_bytes = (h->fq_alias ? 0 : fq_bytes) + (long long)ncw * aq + 8 * nexp;
+  h->mix_ready = 1;
+  return hipSuccess;
+}
+
+// Spin 0: the P tables in both operand orders (six slices), resident.
+static hipError_t mixed_setup_spin0(Handle* h) {
+  if (h->mix0_ready) return hipSuccess;
+  const int lmax = h->lmax;
+  const long long ta = h->mix_ta_total, tae = h->mix_tae, tse = h->mix_tse;
+  const int fa = h->mix_fa, fs = h->mix_fs;
   hipError_t e = hipSuccess;
   auto al = [&](void** p, long long bytes) {
     if (e == hipSuccess) e = hipMalloc(p, (size_t)std::max(bytes, 256LL));
   };
   al((void**)&h->d_TA, ta);
   al((void**)&h->d_TS, ta);
-  al((void**)&h->d_FQ, fq);
-  al((void**)&h->d_AQ, aq);
   al((void**)&h->d_TAe, tae * 4);
   al((void**)&h->d_TSe, tse * 4);
-  al((void**)&h->d_FQe, nexp * 4);
-  al((void**)&h->d_AQe, nexp * 4);
   al((void**)&h->d_firstA, (long long)fa * 4);
   al((void**)&h->d_firstS, (long long)fs * 4);
-  al((void**)&h->d_mix_off, (long long)h->mix_off.size() * (long long)sizeof(MixOff));
-  int2* d_pitems = nullptr;
-  al((void**)&d_pitems, (long long)pitems.size() * (long long)sizeof(int2));
-  if (e != hipSuccess) {
-    (void)hipFree(d_pitems);
-    mixed_free(h);
-    return e;
-  }
-  h->mix_bytes = 2 * ta + fq + aq + 4 * (tae + tse + 2 * nexp + fa + fs);
-  e = hipMemcpy(h->d_mix_off, h->mix_off.data(), h->mix_off.size() * sizeof(MixOff), hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = hipMemcpy(d_pitems, pitems.data(), pitems.size() * sizeof(int2), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = fill_int(h, h->d_firstA, fa, kNeverStep);
   if (e == hipSuccess) e = fill_int(h, h->d_firstS, fs, kNeverStep);
   if (e == hipSuccess) e = fill_int(h, h->d_TAe, tae, kMixZeroE);
-  const dim3 pgrid((unsigned)pitems.size());
+  const dim3 pgrid((unsigned)h->pitem_off[lmax + 1]);
   if (e == hipSuccess) {
-    hipLaunchKernelGGL(k_ptab<0>, pgrid, dim3(256), 0, h->stream, d_pitems, h->d_geo, h->d_i0, h->d_ab, h->d_aboff,
+    hipLaunchKernelGGL(k_ptab<0>, pgrid, dim3(256), 0, h->stream, h->d_pitems, h->d_geo, h->d_i0, h->d_ab, h->d_aboff,
                        h->d_synoff, h->d_syn_st, h->d_syn_v, h->d_mix_off, h->d_TA, h->d_TAe, h->d_TS, h->d_TSe,
                        h->d_firstA, h->d_firstS, lmax, h->nhr);
     e = hipGetLastError();
   }
   if (e == hipSuccess) {
-    hipLaunchKernelGGL(k_ptab<1>, pgrid, dim3(256), 0, h->stream, d_pitems, h->d_geo, h->d_i0, h->d_ab, h->d_aboff,
+    hipLaunchKernelGGL(k_ptab<1>, pgrid, dim3(256), 0, h->stream, h->d_pitems, h->d_geo, h->d_i0, h->d_ab, h->d_aboff,
                        h->d_synoff, h->d_syn_st, h->d_syn_v, h->d_mix_off, h->d_TA, h->d_TAe, h->d_TS, h->d_TSe,
                        h->d_firstA, h->d_firstS, lmax, h->nhr);
     e = hipGetLastError();
   }
   if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
-  (void)hipFree(d_pitems);
   std::vector<int> firstA(fa), firstS(fs);
   if (e == hipSuccess) e = hipMemcpy(firstA.data(), h->d_firstA, (size_t)fa * 4, hipMemcpyDeviceToHost);
   if (e == hipSuccess) e = hipMemcpy(firstS.data(), h->d_firstS, (size_t)fs * 4, hipMemcpyDeviceToHost);
-  if (e != hipSuccess) {
-    mixed_free(h);
-    return e;
-  }
-  // work lists: every (m, class, row tiles, column tiles) writes its
-  // outputs, starting at the first step with a nonzero P digit of its row tiles
-  const int nfull = nct / 4, nrem = nct % 4;
-  for (int k = 0; k < 3; ++k) {
-    if (k > 0 && nrem == 0) break;
-    const int NA = k == 1 ? 8 : 4;
-    std::vector<LegItem> ana, syn;
-    std::vector<double> ana_cost, syn_cost;
-    // column blocks of this list (colblk counts blocks of NB tiles) and their tile counts
-    std::vector<int> cbs, cnt;
-    if (k == 0)
-      for (int cb = 0; cb < nfull; ++cb) cbs.push_back(cb), cnt.push_back(4);
-    else if (k == 1)
-      for (int r = 0; r < nrem; ++r) cbs.push_back(4 * nfull + r), cnt.push_back(1);
-    else
-      cbs.push_back(nfull), cnt.push_back(4);
-    for (int m = 0; m <= lmax; ++m)
-      for (int c = 0; c < 2; ++c) {
-        const MixOff& o = h->mix_off[2 * m + c];
-        for (int b = 0; NA * b < o.nlt; ++b) {
-          int r0 = o.nrs;
-          for (int t = NA * b; t < std::min(o.nlt, NA * b + NA); ++t) r0 = std::min(r0, firstA[o.fa + t]);
-          for (size_t q = 0; q < cbs.size(); ++q) {
-            ana.push_back(LegItem{m, c, b, cbs[q], r0});
-            ana_cost.push_back(1.0 + (double)(o.nrs - r0) * NA * cnt[q]);
-          }
-        }
-        for (int b = 0; NA * b < o.nrs; ++b) {
-          int l0 = o.nlt;
-          for (int t = NA * b; t < std::min(o.nrs, NA * b + NA); ++t) l0 = std::min(l0, firstS[o.fs + t]);
-          for (size_t q = 0; q < cbs.size(); ++q) {
-            syn.push_back(LegItem{m, c, b, cbs[q], l0});
-            syn_cost.push_back(1.0 + (double)(o.nlt - l0) * NA * cnt[q]);
-          }
-        }
-      }
-    e = upload_items(ana, ana_cost, &h->d_anaq_items[k], &h->n_anaq_items[k]);
-    if (e == hipSuccess) e = upload_items(syn, syn_cost, &h->d_synq_items[k], &h->n_synq_items[k]);
-    if (e != hipSuccess) {
-      mixed_free(h);
-      return e;
-    }
-  }
-  h->mix_ready = 1;
+  if (e == hipSuccess)
+    e = build_q_lists(h, 0, lmax + 1, h->nsimp / 16, firstA, firstS, h->d_anaq_items, h->n_anaq_items,
+                      h->d_synq_items, h->n_synq_items);
+  if (e != hipSuccess) return e;  // the caller frees the mixed-precision state
+  h->mix_bytes += 2 * ta + 4 * (tae + tse + fa + fs);
+  h->mix0_ready = 1;
   return hipSuccess;
 }
 
-hipError_t launch_fexp_reset(Handle* h) { return fill_int(h, h->d_FQe, 2LL * (h->lmax + 1) * 2 * h->nsimp, kMixZeroE); }
+// Spin 2: the F1/F2 exponents and first nonzero steps (resident), the plan of
+// m ranges whose tables are generated per pass (at most kT2Cap bytes per
+// function at six slices; FSHT_T2_CAP_MB lowers it, e.g. to exercise many
+// ranges on small grids), the chunk buffers and the work lists per
+// (function, range).
+constexpr long long kT2Cap = 6LL << 30;
+static long long t2_cap_limit() {
+  const char* v = std::getenv("FSHT_T2_CAP_MB");
+  const long long mb = v ? std::atoll(v) : 0;
+  return mb > 0 ? std::min(kT2Cap, mb << 20) : kT2Cap;
+}
+
+#define FSHT_FTAB_ARGS(T, TB, FN)                                                                                    \
+  h->d_geo, h->d_i0, h->d_ab, h->d_spin, h->d_aboff, h->d_synoff, h->d_syn_st, h->d_syn_v, h->d_mix_off, T, TB,    \
+      h->d_TAe2[FN], h->d_TSe2[FN], h->d_firstA2[FN], h->d_firstS2[FN], h->lmax, h->nhr
+
+static hipError_t mixed_setup_spin2(Handle* h) {
+  if (h->mix2_ready) return hipSuccess;
+  const int lmax = h->lmax;
+  const long long tae = h->mix_tae, tse = h->mix_tse;
+  const int fa = h->mix_fa, fs = h->mix_fs;
+  hipError_t e = hipSuccess;
+  auto al = [&](void** p, long long bytes) {
+    if (e == hipSuccess) e = hipMalloc(p, (size_t)std::max(bytes, 256LL));
+  };
+  for (int f = 0; f < 2; ++f) {
+    al((void**)&h->d_TAe2[f], tae * 4);
+    al((void**)&h->d_TSe2[f], tse * 4);
+    al((void**)&h->d_firstA2[f], (long long)fa * 4);
+    al((void**)&h->d_firstS2[f], (long long)fs * 4);
+    if (e == hipSuccess) e = fill_int(h, h->d_firstA2[f], fa, kNeverStep);
+    if (e == hipSuccess) e = fill_int(h, h->d_firstS2[f], fs, kNeverStep);
+    if (e == hipSuccess) e = fill_int(h, h->d_TAe2[f], tae, kMixZeroE);
+  }
+  const dim3 pgrid((unsigned)h->pitem_off[lmax + 1]);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL((k_ftab<0, 0, 0, kMixK>), pgrid, dim3(256), 0, h->stream, h->d_pitems,
+                       FSHT_FTAB_ARGS(nullptr, 0LL, 0));
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL((k_ftab<1, 0, 0, kMixK>), pgrid, dim3(256), 0, h->stream, h->d_pitems,
+                       FSHT_FTAB_ARGS(nullptr, 0LL, 1));
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+  std::vector<int> firstA[2], firstS[2];
+  for (int f = 0; f < 2 && e == hipSuccess; ++f) {
+    firstA[f].resize(fa);
+    firstS[f].resize(fs);
+    e = hipMemcpy(firstA[f].data(), h->d_firstA2[f], (size_t)fa * 4, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(firstS[f].data(), h->d_firstS2[f], (size_t)fs * 4, hipMemcpyDeviceToHost);
+  }
+  if (e != hipSuccess) return e;
+  // m ranges: whole orders, each range's tables (one function, one operand
+  // order, six slices) within kT2Cap
+  h->mix2_chunk.assign(1, 0);
+  long long cap = 0;
+  const long long cap_limit = t2_cap_limit();
+  for (int m = 0; m <= lmax;) {
+    int m1 = m;
+    const long long start = h->mix_off[2 * m].ta;
+    while (m1 <= lmax && (m1 == m || (m1 + 1 <= lmax ? h->mix_off[2 * (m1 + 1)].ta : h->mix_ta_total) - start <= cap_limit))
+      ++m1;
+    const long long end = m1 <= lmax ? h->mix_off[2 * m1].ta : h->mix_ta_total;
+    cap = std::max(cap, end - start);
+    h->mix2_chunk.push_back(m1);
+    m = m1;
+  }
+  const int nchunk = (int)h->mix2_chunk.size() - 1;
+  h->t2_cap = cap;
+  al((void**)&h->d_T2[0], cap);
+  al((void**)&h->d_T2[1], cap);
+  if (e != hipSuccess) return e;
+  const int nct2 = h->nsimp / 8;  // 32-column tiles of the 4*nsimp spin-2 columns
+  h->d_anaq2.assign((size_t)2 * nchunk * 3, nullptr);
+  h->d_synq2.assign((size_t)2 * nchunk * 3, nullptr);
+  h->n_anaq2.assign((size_t)2 * nchunk * 3, 0);
+  h->n_synq2.assign((size_t)2 * nchunk * 3, 0);
+  for (int f = 0; f < 2 && e == hipSuccess; ++f)
+    for (int c = 0; c < nchunk && e == hipSuccess; ++c) {
+      const size_t q = ((size_t)f * nchunk + c) * 3;
+      e = build_q_lists(h, h->mix2_chunk[c], h->mix2_chunk[c + 1], nct2, firstA[f], firstS[f], &h->d_anaq2[q],
+                        &h->n_anaq2[q], &h->d_synq2[q], &h->n_synq2[q]);
+    }
+  if (e != hipSuccess) return e;
+  h->mix_bytes += 2 * cap + 8 * (tae + tse + fa + fs);
+  h->t2_lay = -1;
+  h->mix2_ready = 1;
+  return hipSuccess;
+}
+
+hipError_t mixed_setup(Handle* h) {
+  hipError_t e = mixed_setup_common(h);
+  if (e == hipSuccess && !h->pol) e = mixed_setup_spin0(h);
+  if (e != hipSuccess) mixed_free(h);
+  return e;
+}
+
+hipError_t mixed_ensure(Handle* h, int spin) {
+  hipError_t e = mixed_setup_common(h);
+  if (e == hipSuccess) e = spin ? mixed_setup_spin2(h) : mixed_setup_spin0(h);
+  if (e != hipSuccess) mixed_free(h);
+  return e;
+}
+
+static int cur_ncomp(const Handle* h) { return h->spin ? 2 : 1; }
+
+hipError_t launch_fexp_reset(Handle* h) {
+  return fill_int(h, h->d_FQe, 2LL * (h->lmax + 1) * 2 * cur_ncomp(h) * h->nsimp, kMixZeroE);
+}
 
 hipError_t launch_quant_F(Handle* h, const double* F, int K) {
-  const int nct = h->nsimp / 16;
-  dim3 grid(h->lmax + 1, 2 * h->mix_nrb_max, nct);
-  const int ncols = 2 * h->nsimp;
+  const int ncomp = cur_ncomp(h);
+  dim3 grid(h->lmax + 1, 2 * h->mix_nrb_max, ncomp * h->nsimp / 16);
 #define FSHT_QF(KV)                                                                                              \
   hipLaunchKernelGGL(k_quant_F<KV>, grid, dim3(256), 0, h->stream, h->d_mix_off, h->d_i0, h->d_goff, F, h->d_FQe, \
-                     h->d_FQ, h->nhr, ncols, h->nsim)
+                     h->d_FQ, h->nhr, ncomp, h->nsim, h->nsimp)
   switch (K) {
     case 4: FSHT_QF(4); break;
     case 5: FSHT_QF(5); break;
@@ -701,12 +1256,12 @@ hipError_t launch_quant_F(Handle* h, const double* F, int K) {
   return hipGetLastError();
 }
 
-hipError_t launch_quant_A(Handle* h, const double* alm, int K, int have_exp) {
-  const int nct = h->nsimp / 16;
-  dim3 grid(h->lmax + 1, 2, nct);
-#define FSHT_QA(KV, E)                                                                                              \
-  hipLaunchKernelGGL((k_quant_A<KV, E>), grid, dim3(256), 0, h->stream, h->d_mix_off, alm, h->d_AQ, h->d_AQe, h->nsim, \
-                     h->nsimp, h->lmax)
+hipError_t launch_quant_A(Handle* h, const double* alm0, const double* alm1, int K, int have_exp) {
+  const int ncomp = cur_ncomp(h);
+  dim3 grid(h->lmax + 1, 2, ncomp * h->nsimp / 16);
+#define FSHT_QA(KV, E)                                                                                               \
+  hipLaunchKernelGGL((k_quant_A<KV, E>), grid, dim3(256), 0, h->stream, h->d_mix_off, alm0, alm1, h->d_AQ, h->d_AQe, \
+                     ncomp, h->nsim, h->nsimp, h->lmax)
   switch (K * 2 + (have_exp ? 1 : 0)) {
     case 8: FSHT_QA(4, 0); break;
     case 9: FSHT_QA(4, 1); break;
@@ -720,23 +1275,30 @@ hipError_t launch_quant_A(Handle* h, const double* alm, int K, int have_exp) {
   return hipGetLastError();
 }
 
+// One slice-GEMM pass over a set of work lists (full 4-tile column blocks,
+// then the remaining column tiles: one by one, or for six slices one padded
+// block).
 template <int SYN>
-static hipError_t launch_gemm(Handle* h, double* out, double beta, int K, int* aqe) {
-  const int nct = h->nsimp / 16;
-  const signed char* TP = SYN ? h->d_TS : h->d_TA;
-  const int* TPe = SYN ? h->d_TSe : h->d_TAe;
-  const signed char* OQ = SYN ? h->d_AQ : h->d_FQ;
-  const int* OQe = SYN ? h->d_AQe : h->d_FQe;
-  // full column blocks, then the remaining tiles (one by one; six slices: one padded block)
+static hipError_t launch_gemm(Handle* h, SliceArgs A, int K, LegItem* const* items, const int* nitems) {
   const int lists[2] = {0, K <= 5 ? 1 : 2};
   for (int q = 0; q < 2; ++q) {
     const int k = lists[q];
-    const int n = SYN ? h->n_synq_items[k] : h->n_anaq_items[k];
+    const int n = nitems[k];
     if (n == 0) continue;
-    const LegItem* items = SYN ? h->d_synq_items[k] : h->d_anaq_items[k];
-#define FSHT_GQ(KV, NBV)                                                                                          \
-  hipLaunchKernelGGL((k_slice_gemm<KV, SYN, NBV>), dim3(n), dim3(512), 0, h->stream, items, h->d_mix_off, TP, TPe, \
-                     OQ, OQe, out, h->d_goff, h->d_i0, h->nsim, h->nsimp, h->lmax, h->nhr, nct, h->pa, beta, aqe)
+    A.items = items[k];
+#define FSHT_GQ_K(KERN, KV, NBV) hipLaunchKernelGGL((KERN<KV, SYN, NBV>), dim3(n), dim3(512), 0, h->stream, A)
+#if FSHT_SLICE_PP
+// four and five slices: the two-group kernel; six slices (whose operand
+// registers would not fit beside the accumulators twice over): one group
+#define FSHT_GQ(KV, NBV)                       \
+  if constexpr (KV <= 5) {                     \
+    FSHT_GQ_K(k_slice_gemm_pp, KV, NBV);       \
+  } else {                                     \
+    FSHT_GQ_K(k_slice_gemm, KV, NBV);          \
+  }
+#else
+#define FSHT_GQ(KV, NBV) FSHT_GQ_K(k_slice_gemm, KV, NBV)
+#endif
     if (k == 1) {
       switch (K) {
         case 4: FSHT_GQ(4, 1); break;
@@ -752,18 +1314,126 @@ static hipError_t launch_gemm(Handle* h, double* out, double beta, int K, int* a
       }
     }
 #undef FSHT_GQ
+#undef FSHT_GQ_K
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
   }
   return hipSuccess;
 }
 
-hipError_t launch_analysis_q(Handle* h, double* alm, double beta, int K, int exps) {
-  if (!exps) return launch_gemm<0>(h, alm, beta, K, nullptr);
-  // alm column exponents for the next synthesis operand
-  hipError_t e = fill_int(h, h->d_AQe, 2LL * (h->lmax + 1) * 2 * h->nsimp, kMixZeroE);
-  return e != hipSuccess ? e : launch_gemm<0>(h, alm, beta, K, h->d_AQe);
+static SliceArgs base_args(Handle* h, int SYN, int ncomp) {
+  SliceArgs A{};
+  A.mo = h->d_mix_off;
+  A.OQ = SYN ? h->d_AQ : h->d_FQ;
+  A.OQe = SYN ? h->d_AQe : h->d_FQe;
+  A.goff = h->d_goff;
+  A.i0tab = h->d_i0;
+  A.nsim = h->nsim;
+  A.nsimp = h->nsimp;
+  A.ncomp = ncomp;
+  A.lmax = h->lmax;
+  A.nhr = h->nhr;
+  A.nct = ncomp * h->nsimp / 16;
+  A.pa = h->pa;
+  return A;
 }
-hipError_t launch_synthesis_q(Handle* h, double* G, int K) { return launch_gemm<1>(h, G, 0.0, K, nullptr); }
+
+int mixed_nrange(const Handle* h) { return h->spin ? (int)h->mix2_chunk.size() - 1 : 1; }
+
+// Spin 2: F1 and F2 slices of the orders of range c, in one operand order
+// (synthesis = 0: TA, 1: TS), into the chunk buffers (kept when one range
+// covers every order).
+static long long range_base(const Handle* h, int c, int K) { return h->mix_off[2 * h->mix2_chunk[c]].ta / kMixK * K; }
+
+hipError_t launch_ftab(Handle* h, int c, int lay, int K) {
+  const int nchunk = (int)h->mix2_chunk.size() - 1;
+  if (nchunk == 1 && h->t2_lay == lay && h->t2_ks == K) return hipSuccess;
+  const int m0 = h->mix2_chunk[c];
+  const long long tbase = range_base(h, c, K);
+  const int i0 = h->pitem_off[m0], i1 = h->pitem_off[h->mix2_chunk[c + 1]];
+  if (i1 <= i0) return hipSuccess;
+  const int2* items = h->d_pitems + i0;
+  const dim3 grid((unsigned)(i1 - i0));
+#define FSHT_FT(FN, LAY, KS) \
+  hipLaunchKernelGGL((k_ftab<FN, 1, LAY, KS>), grid, dim3(256), 0, h->stream, items, FSHT_FTAB_ARGS(h->d_T2[FN], tbase, FN))
+#define FSHT_FT2(LAY, KS) \
+  FSHT_FT(0, LAY, KS);    \
+  FSHT_FT(1, LAY, KS)
+  switch (lay * 8 + K) {
+    case 4: FSHT_FT2(0, 4); break;
+    case 5: FSHT_FT2(0, 5); break;
+    case 6: FSHT_FT2(0, 6); break;
+    case 12: FSHT_FT2(1, 4); break;
+    case 13: FSHT_FT2(1, 5); break;
+    case 14: FSHT_FT2(1, 6); break;
+    default: return hipErrorInvalidValue;
+  }
+#undef FSHT_FT2
+#undef FSHT_FT
+  h->t2_lay = lay;
+  h->t2_ks = K;
+  return hipGetLastError();
+}
+
+hipError_t launch_aqe_reset(Handle* h) {
+  return fill_int(h, h->d_AQe, 2LL * (h->lmax + 1) * 2 * cur_ncomp(h) * h->nsimp, kMixZeroE);
+}
+
+hipError_t launch_analysis_q(Handle* h, int c, double* alm0, double* alm1, double beta, int K, int exps) {
+  if (h->spin == 0) {
+    SliceArgs A = base_args(h, 0, 1);
+    A.TP = h->d_TA;
+    A.TPe = h->d_TAe;
+    A.ast = kMixK;
+    A.out0 = alm0;
+    A.beta = beta;
+    A.aqe = exps ? h->d_AQe : nullptr;
+    return launch_gemm<0>(h, A, K, h->d_anaq_items, h->n_anaq_items);
+  }
+  // spin 2: F1, then F2 (which adds to F1's result and records the exponents)
+  const int nchunk = (int)h->mix2_chunk.size() - 1;
+  hipError_t e = hipSuccess;
+  for (int f = 0; f < 2 && e == hipSuccess; ++f) {
+    SliceArgs A = base_args(h, 0, 2);
+    A.TP = h->d_T2[f];
+    A.TPe = h->d_TAe2[f];
+    A.ast = K;
+    A.abase = range_base(h, c, K);
+    A.out0 = alm0;
+    A.out1 = alm1;
+    A.fn = f;
+    A.beta = f ? 1.0 : beta;
+    A.aqe = (f && exps) ? h->d_AQe : nullptr;
+    const size_t q = ((size_t)f * nchunk + c) * 3;
+    e = launch_gemm<0>(h, A, K, &h->d_anaq2[q], &h->n_anaq2[q]);
+  }
+  return e;
+}
+
+hipError_t launch_synthesis_q(Handle* h, int c, double* G, int K) {
+  if (h->spin == 0) {
+    SliceArgs A = base_args(h, 1, 1);
+    A.TP = h->d_TS;
+    A.TPe = h->d_TSe;
+    A.ast = kMixK;
+    A.out0 = G;
+    return launch_gemm<1>(h, A, K, h->d_synq_items, h->n_synq_items);
+  }
+  const int nchunk = (int)h->mix2_chunk.size() - 1;
+  hipError_t e = hipSuccess;
+  for (int f = 0; f < 2 && e == hipSuccess; ++f) {
+    SliceArgs A = base_args(h, 1, 2);
+    A.TP = h->d_T2[f];
+    A.TPe = h->d_TSe2[f];
+    A.ast = K;
+    A.abase = range_base(h, c, K);
+    A.out0 = G;
+    A.fn = f;
+    A.beta = f ? 1.0 : 0.0;
+    const size_t q = ((size_t)f * nchunk + c) * 3;
+    e = launch_gemm<1>(h, A, K, &h->d_synq2[q], &h->n_synq2[q]);
+  }
+  return e;
+}
 
 }  // namespace fsht

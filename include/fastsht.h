@@ -81,7 +81,11 @@ int fsht_set_batch(fsht_handle* h, int nsim);
  * times (analysis k: niter - k, synthesis k: niter - k - 1), and a later
  * analysis damped at least damp_min_a times, use k_lo slices per operand, the
  * others k_hi; alm2t and t2alm with niter = 0 use k_out; then the capacity
- * (the nsim the handle was created with). */
+ * (the nsim the handle was created with); then, for spin 0 (out[16..26)) and
+ * spin 2 (out[26..36)), the slices of every pass as a table: out[b + 3 k + d]
+ * for pass kind k (0: the first analysis, 1: a residual synthesis, 2: a
+ * residual analysis) whose error d = min(2, damping) later iterations damp,
+ * and out[b + 9] for alm2t / eb2qu and an unrefined analysis. */
 int fsht_info(const fsht_handle* h, long long* out, int n);
 
 #define FSHT_PREC_FP64 0  /* every Legendre contraction in FP64 (the default) */
@@ -180,8 +184,9 @@ int fsht_host_free(void* p);
  * fsht_profile(h, 1) starts recording (and clears earlier records); classes:
  * 0 ring FFT forward, 1 ring FFT inverse, 2 transposes, 3 Legendre analysis,
  * 4 Legendre synthesis, 5 alias fold / Jacobi residual, 6 spectrum gather
- * into the m-major analysis operand, 7 operand slicing (mixed precision).
- * fsht_profile_read fills out[2c] = launches, out[2c+1] = total ms (c < 8). */
+ * into the m-major analysis operand, 7 operand slicing (mixed precision),
+ * 8 spin-2 Legendre table generation (mixed precision).  fsht_profile_read
+ * fills out[2c] = launches, out[2c+1] = total ms (c < 9). */
 int fsht_profile(fsht_handle* h, int enable);
 int fsht_profile_read(fsht_handle* h, double* out, int n);
 

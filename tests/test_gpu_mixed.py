@@ -77,3 +77,93 @@ def test_mixed_vs_fp64_gpu_nside256(torch_cuda):
     err = maxrel(mmx.cpu().numpy(), m64.cpu().numpy())
     print(f"  alm2t: {err:.2e}", flush=True)
     assert err < TOL
+
+
+# ---------------------------------------------------------------------------
+# Spin 2: qu2eb(niter) and eb2qu with int8-slice F1/F2 contractions (tables
+# generated per pass, one range of orders at a time) against the oracle's
+# QU2EB / qu2eb_iter_old / EB2QU, and against the FP64 GPU path.
+# ---------------------------------------------------------------------------
+# lmax = 2 nside - 1 and the stronger aliasing of 3 nside - 1 and 4 nside (other ladders)
+SPIN2_CASES = [(4, 8, 3), (8, 20, 5), (16, 47, 7), (32, 63, 9), (32, 95, 5), (32, 128, 4), (64, 127, 5)]
+
+
+def _qu_maps(R, lmax, nsim, seed):
+    """Q/U of seeded E/B alm (C_l^BB = 0.05 C_l^EE) plus N(0, 1e-3) noise."""
+    E = O.random_alm(lmax, nsim, seed=seed)
+    B = O.random_alm(lmax, nsim, seed=seed + 1) * np.sqrt(0.05)
+    Q, U = R.eb2qu(E, B)
+    rng = np.random.default_rng(seed + 2)
+    Q = np.asfortranarray(Q + rng.standard_normal(Q.shape) * np.sqrt(1e-3))
+    U = np.asfortranarray(U + rng.standard_normal(U.shape) * np.sqrt(1e-3))
+    return Q, U
+
+
+def _spin2_case(ns, lmax, nsim):
+    from fastsht_amd import SHT
+    R = O.OracleSHT(ns, lmax, nsim)
+    Q, U = _qu_maps(R, lmax, nsim, seed=ns + lmax)
+    for niter in (0, 1, 2, 3):
+        S = SHT(ns, lmax, nsim, niter=niter, pol=True, precision="mixed")
+        E, B = S.qu2eb(Q, U)
+        Eo, Bo = R.qu2eb(Q, U, niter=niter)
+        err = max(maxrel(E, Eo), maxrel(B, Bo))
+        print(f"  qu2eb niter={niter}: {err:.2e}", flush=True)
+        assert err < TOL, (niter, err)
+    Ea, Ba = O.random_alm(lmax, nsim, seed=5), O.random_alm(lmax, nsim, seed=6)
+    Qg, Ug = S.eb2qu(Ea, Ba)
+    Qo, Uo = R.eb2qu(Ea, Ba)
+    err = max(maxrel(Qg, Qo), maxrel(Ug, Uo))
+    print(f"  eb2qu: {err:.2e}", flush=True)
+    assert err < TOL
+    # the same handle's temperature transforms (spin-0 tables built on first use)
+    err = maxrel(S.t2alm(Q), R.t2alm(Q, niter=3))
+    print(f"  t2alm niter=3 on the spin-2 handle: {err:.2e}", flush=True)
+    assert err < TOL
+
+
+@pytest.mark.parametrize("ns,lmax,nsim", SPIN2_CASES)
+def test_mixed_spin2_vs_oracle(torch_cuda, ns, lmax, nsim):
+    print(f"case mixed spin2 nside={ns} lmax={lmax} nsim={nsim}", flush=True)
+    _spin2_case(ns, lmax, nsim)
+
+
+def test_mixed_spin2_many_ranges(torch_cuda, monkeypatch):
+    """Tables generated in ranges of orders of at most 1 MB each (10 ranges
+    at nside 64 / lmax 127): the same results as one range."""
+    monkeypatch.setenv("FSHT_T2_CAP_MB", "1")
+    print("case mixed spin2 ranges nside=64 lmax=127 nsim=5 cap=1MB", flush=True)
+    _spin2_case(64, 127, 5)
+
+
+def test_mixed_spin2_vs_fp64_gpu_nside256(torch_cuda):
+    """nside 256, lmax 511, 40 Q/U pairs (six 32-column tiles: one full block
+    and two single tiles): qu2eb niter 3 and eb2qu against the FP64 GPU path."""
+    from fastsht_amd import SHT
+    torch = torch_cuda
+    ns, lmax, nsim = 256, 511, 40
+    print(f"case mixed spin2 vs fp64 nside={ns} lmax={lmax} nsim={nsim}", flush=True)
+    F = SHT(ns, lmax, nsim, niter=3, pol=True)
+    M = SHT(ns, lmax, nsim, niter=3, pol=True, precision="mixed")
+
+    def dev(a):
+        t = torch.from_numpy(a).cuda()
+        return t.permute(2, 1, 0).contiguous().permute(2, 1, 0)
+
+    E0 = dev(O.random_alm(lmax, nsim, seed=21))
+    B0 = dev(O.random_alm(lmax, nsim, seed=22)) * 0.2236
+    Q, U = F.eb2qu(E0, B0)
+    g = torch.Generator("cuda").manual_seed(7)
+    Q += 0.0316 * torch.randn(Q.shape, dtype=torch.float64, device="cuda", generator=g)
+    U += 0.0316 * torch.randn(U.shape, dtype=torch.float64, device="cuda", generator=g)
+    E64, B64 = F.qu2eb(Q, U)
+    Emx, Bmx = M.qu2eb(Q, U)
+    torch.cuda.synchronize()
+    err = max(maxrel(Emx.cpu().numpy(), E64.cpu().numpy()), maxrel(Bmx.cpu().numpy(), B64.cpu().numpy()))
+    print(f"  qu2eb niter=3: {err:.2e}", flush=True)
+    assert err < TOL
+    Q64, U64 = F.eb2qu(E64, B64)
+    Qmx, Umx = M.eb2qu(E64, B64)
+    err = max(maxrel(Qmx.cpu().numpy(), Q64.cpu().numpy()), maxrel(Umx.cpu().numpy(), U64.cpu().numpy()))
+    print(f"  eb2qu: {err:.2e}", flush=True)
+    assert err < TOL
