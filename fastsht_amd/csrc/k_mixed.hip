@@ -42,6 +42,9 @@ __device__ inline v16i mfma_i8(v4i a, v4i b, v16i c) { return __builtin_amdgcn_m
 __device__ inline int drow(int r, int h) { return 8 * (r >> 2) + 4 * h + (r & 3); }
 __device__ inline int dpos(int i) { return 16 * ((i >> 2) & 1) + 4 * (i >> 3) + (i & 3); }
 
+// keeps the scheduler from moving any instruction across this point
+__device__ inline void sched_fence() { __builtin_amdgcn_sched_barrier(0); }
+
 constexpr int kNeverStep = 0x3f3f3f3f;  // "no nonzero step" (fill value of the first-step tables)
 constexpr int kNeverL = 0x3fffffff;
 
@@ -229,30 +232,27 @@ __global__ __launch_bounds__(256, 1) void k_ptab(const int2* __restrict__ items,
 }
 
 // ---------------------------------------------------------------------------
-// Spin-2 tables: F1 (FN = 0) or F2 (FN = 1) of src/get_plm_recursive.f90:105-133
-// for the orders of one m range, from the FP64 recurrence and start states of
-// the P tables, every value formed by spin_f from P_l and P_{l-1} (the
-// FP64 spin-2 kernels' own expression).  Workgroup = (m, block of 256 ring
-// rows); thread = ring; tiles of 64 l steps (32 values per class).
-//   MODE 0: exponents (TSe per ring row and class over every l; TAe per l
-//           over the rings) and the first nonzero steps, as k_ptab<0>.
-//   MODE 1: the KS digit slices of one operand order (LAY 0: TA, LAY 1: TS)
-//           into a buffer holding the range: the blocks of (m, class) start
-//           at byte (o.ta or o.ts) / kMixK * KS - tbase.
+// Spin-2 tables: F1 (FN = 0) or F2 (FN = 1) of src/get_plm_recursive.f90:105-133,
+// from the FP64 recurrence and start states of the P tables, every value
+// formed by spin_f from P_l and P_{l-1} (the FP64 spin-2 kernels' own
+// expression).  Workgroup = (m, block of 256 ring rows); thread = ring.
+// k_ftab_exp (once per handle): the exponents (TSe per ring row and class
+// over every l; TAe per l over the rings) and the first nonzero steps, as
+// k_ptab<0>, in tiles of 64 l steps (32 values per class).
 // ---------------------------------------------------------------------------
-template <int FN, int MODE, int LAY, int KS>
-__global__ __launch_bounds__(256, 1) void k_ftab(const int2* __restrict__ items, const RingGeo* __restrict__ geo,
+template <int FN>
+__global__ __launch_bounds__(256, 1) void k_ftab_exp(const int2* __restrict__ items, const RingGeo* __restrict__ geo,
                                                  const int* __restrict__ i0tab, const double2* __restrict__ ab,
                                                  const double2* __restrict__ spin, const long long* __restrict__ aboff,
                                                  const long long* __restrict__ synoff,
                                                  const int* __restrict__ syn_st, const double2* __restrict__ syn_v,
-                                                 const MixOff* __restrict__ mo, signed char* __restrict__ T,
-                                                 long long tbase, int* __restrict__ TAe, int* __restrict__ TSe,
+                                                 const MixOff* __restrict__ mo, int* __restrict__ TAe, int* __restrict__ TSe,
                                                  int* __restrict__ firstA, int* __restrict__ firstS, int lmax,
                                                  int nhr) {
-  constexpr bool TA_OUT = MODE == 1 && LAY == 0;
-  constexpr bool TS_OUT = MODE == 1 && LAY == 1;
-  __shared__ __align__(16) signed char stage[TA_OUT ? 2 : 1][TA_OUT ? KS : 1][32][TA_OUT ? 256 + 16 : 16];
+  constexpr int MODE = 0;
+  constexpr bool TA_OUT = false, TS_OUT = false;
+  constexpr int KS = 1;
+  __shared__ __align__(16) signed char stage[1][1][32][16];
   __shared__ int emax[2][32], rmin[2];
   const int2 it = items[blockIdx.x];
   const int m = it.x, rb = it.y, tid = threadIdx.x;
@@ -273,11 +273,6 @@ __global__ __launch_bounds__(256, 1) void k_ftab(const int2* __restrict__ items,
   const MixOff o0 = mo[2 * m], o1 = mo[2 * m + 1];
   const bool inrow = row < o0.nrs * 32;  // row of the padded ring tiles (both classes have nrs tiles)
   const long long rpos = (long long)(row >> 5) * 32 + dpos(row & 31);  // within a class's TS exponents
-  int eS0 = kMixZeroE, eS1 = kMixZeroE;
-  if (TS_OUT && inrow) {
-    eS0 = TSe[o0.tse + rpos];
-    eS1 = TSe[o1.tse + rpos];
-  }
   if (tid < 64) emax[tid >> 5][tid & 31] = kMixZeroE;
   if (tid < 2) rmin[tid] = kNeverStep;
   __syncthreads();
@@ -320,36 +315,6 @@ __global__ __launch_bounds__(256, 1) void k_ftab(const int2* __restrict__ items,
             if (cls) myfirst1 = min(myfirst1, lt); else myfirst0 = min(myfirst0, lt);
           }
         }
-      } else if (TS_OUT && inrow) {
-        // this ring's lane (row & 31) + 32 g holds values 16 g .. 16 g + 15
-        const int e = cls ? eS1 : eS0;
-        double sc[32];
-#pragma unroll
-        for (int j = 0; j < 32; ++j) sc[j] = ldexp(v[2 * j + cls], -e);
-        signed char* dst = T + (o.ts / kMixK * KS - tbase) + ((long long)(row >> 5) * o.nlt + lt) * KS * 1024 +
-                           (row & 31) * 16;
-        put16<KS>(sc, dst);
-        put16<KS>(sc + 16, dst + 32 * 16);
-      }
-    }
-    if (TA_OUT) {
-      // TA digits of the 32 values of each class with the per-l exponents, into LDS
-#pragma unroll
-      for (int cls = 0; cls < 2; ++cls) {
-        const MixOff& o = cls ? o1 : o0;
-        if (lt >= o.nlt) continue;
-        const int* ea = TAe + o.tae + (long long)lt * 32;
-#pragma unroll
-        for (int j = 0; j < 32; ++j) {
-          double r = ldexp(v[2 * j + cls], -ea[dpos(j)]);
-#pragma unroll
-          for (int s = 0; s < KS; ++s) {
-            const double t = r * 128.0;
-            const double q = rint(t);
-            r = t - q;
-            stage[cls][s][j][tid] = (signed char)(int)q;
-          }
-        }
       }
     }
     __syncthreads();
@@ -360,14 +325,6 @@ __global__ __launch_bounds__(256, 1) void k_ftab(const int2* __restrict__ items,
       if (MODE == 0) {
         if (tid < 32 && emax[cls][tid] != kMixZeroE) atomicMax(&TAe[o.tae + (long long)lt * 32 + dpos(tid)], emax[cls][tid]);
         if (tid == 0 && rmin[cls] != kNeverStep) atomicMin(&firstA[o.fa + lt], rmin[cls]);
-      } else if (TA_OUT) {
-        const int nrs_here = min(8, o.nrs - rb * 8);
-        signed char* base = T + (o.ta / kMixK * KS - tbase);
-        for (int ch = tid; ch < nrs_here * KS * 64; ch += 256) {
-          const int L = ch & 63, s = (ch >> 6) % KS, rsl = (ch >> 6) / KS;
-          const v4i val = *reinterpret_cast<const v4i*>(&stage[cls][s][L & 31][rsl * 32 + 16 * (L >> 5)]);
-          *reinterpret_cast<v4i*>(base + (((long long)lt * o.nrs + rb * 8 + rsl) * KS + s) * 1024 + L * 16) = val;
-        }
       }
     }
     __syncthreads();
@@ -382,6 +339,142 @@ __global__ __launch_bounds__(256, 1) void k_ftab(const int2* __restrict__ items,
     TSe[o1.tse + rpos] = valid ? mix_exp(mx1) : kMixZeroE;
     if (myfirst0 != kNeverStep) atomicMin(&firstS[o0.fs + (row >> 5)], myfirst0);
     if (myfirst1 != kNeverStep) atomicMin(&firstS[o1.fs + (row >> 5)], myfirst1);
+  }
+}
+
+// k_ftab_dig (per pass): the KS digit slices of one operand order (LAY 0:
+// TA, LAY 1: TS) for the orders of one range, into a buffer holding the
+// range (the blocks of (m, class) start at byte (o.ta or o.ts) / kMixK * KS -
+// tbase).  Half tiles of 32 l steps (16 values per class), 8 steps at a time
+// (scheduling fences keep the coefficient loads of one group of 8 live, not
+// of the whole tile): TS digits are packed into the ring's 16-byte lanes in
+// registers and stored per half tile; TA digits go through LDS and leave as
+// 16-byte lanes of 16 rings each.
+template <int FN, int LAY, int KS>
+__global__ __launch_bounds__(256, 2) void k_ftab_dig(const int2* __restrict__ items, const RingGeo* __restrict__ geo,
+                                                     const int* __restrict__ i0tab, const double2* __restrict__ ab,
+                                                     const double2* __restrict__ spin,
+                                                     const long long* __restrict__ aboff,
+                                                     const long long* __restrict__ synoff,
+                                                     const int* __restrict__ syn_st,
+                                                     const double2* __restrict__ syn_v, const MixOff* __restrict__ mo,
+                                                     signed char* __restrict__ T, long long tbase,
+                                                     const int* __restrict__ TAe, const int* __restrict__ TSe,
+                                                     int lmax, int nhr) {
+  constexpr bool TA = LAY == 0;
+  __shared__ __align__(16) signed char stage[TA ? 2 : 1][TA ? KS : 1][16][TA ? 256 + 16 : 16];
+  __shared__ int ea_s[2][32];
+  const int2 it = items[blockIdx.x];
+  const int m = it.x, rb = it.y, tid = threadIdx.x;
+  const int i0 = i0tab[m];
+  const int R = nhr + 1 - i0;
+  const int row = rb * 256 + tid;
+  const bool valid = row < R;
+  const long long yo = synoff[m] + (valid ? row : R - 1);
+  const int st = valid ? syn_st[yo] : kNeverL;
+  const double2 sv = syn_v[yo];  // (P_ls-2, P_ls-1) at the 32-step chunk start ls <= st
+  const int ls = (st > lmax) ? kNeverL : m + ((st - m) & ~31);
+  const RingGeo g = geo[valid ? i0 + row : nhr];
+  const double x = g.z;
+  const double s2 = g.s * g.s;
+  const double bw = x / s2, t2 = 2.0 / s2, cw = (double)m / s2;
+  const double2* abm = ab + aboff[m];
+  const double2* spm = spin + aboff[m];
+  const MixOff o0 = mo[2 * m], o1 = mo[2 * m + 1];
+  const bool inrow = row < o0.nrs * 32;  // row of the padded ring tiles (both classes have nrs tiles)
+  const long long rpos = (long long)(row >> 5) * 32 + dpos(row & 31);
+  int eS0 = kMixZeroE, eS1 = kMixZeroE;
+  if (!TA && inrow) {
+    eS0 = TSe[o0.tse + rpos];
+    eS1 = TSe[o1.tse + rpos];
+  }
+  double p1 = 0.0, p2 = 0.0;
+  const int ntile = (lmax - m) / 64 + 1;
+  for (int lt = 0; lt < ntile; ++lt) {
+    const int lb = m + 64 * lt;
+    if (TA) {
+      if (tid < 64) {
+        const int cls = tid >> 5, j = tid & 31;
+        const MixOff& o = cls ? o1 : o0;
+        ea_s[cls][j] = lt < o.nlt ? TAe[o.tae + (long long)lt * 32 + dpos(j)] : 0;
+      }
+      __syncthreads();
+    }
+#pragma unroll
+    for (int hh = 0; hh < 2; ++hh) {
+      unsigned w[2][KS][4];
+#pragma unroll
+      for (int c = 0; c < 2; ++c)
+#pragma unroll
+        for (int k = 0; k < KS; ++k)
+#pragma unroll
+          for (int q = 0; q < 4; ++q) w[c][k][q] = 0u;
+#pragma unroll
+      for (int c8 = 0; c8 < 4; ++c8) {
+#pragma unroll
+        for (int s8 = 0; s8 < 8; ++s8) {
+          const int s = 32 * hh + 8 * c8 + s8;  // step within the tile
+          const int l = lb + s;
+          if (s8 == 0 && c8 == 0 && l == ls) {
+            p1 = sv.y;
+            p2 = sv.x;
+          }
+          const bool in = l <= lmax;
+          const double2 cf = in ? abm[l - m] : make_double2(0.0, 0.0);
+          const double2 sp = in ? spm[l - m] : make_double2(0.0, 0.0);
+          const double p = fma(cf.x * x, p1, -(cf.y * p2));
+          const double f = spin_f(FN, p, p1, sp, l, m, x, bw, t2, cw);
+          p2 = p1;
+          p1 = p;
+          const int cls = s & 1, jh = (s >> 1) & 15;  // class, value within the half tile
+          double r = ldexp(f, -(TA ? ea_s[cls][(s >> 1)] : (cls ? eS1 : eS0)));
+#pragma unroll
+          for (int k = 0; k < KS; ++k) {
+            const double t = r * 128.0;
+            const double q = rint(t);
+            r = t - q;
+            const unsigned byte = (unsigned)(int)q & 255u;
+            if (TA) stage[cls][k][jh][tid] = (signed char)byte;
+            else w[cls][k][jh >> 2] |= byte << (8 * (jh & 3));
+          }
+        }
+        sched_fence();
+      }
+      if (!TA) {
+        // this ring's lane (row & 31) + 32 hh holds the half tile's 16 values
+#pragma unroll
+        for (int cls = 0; cls < 2; ++cls) {
+          const MixOff& o = cls ? o1 : o0;
+          if (lt >= o.nlt || !inrow) continue;
+          signed char* dst = T + (o.ts / kMixK * KS - tbase) + ((long long)(row >> 5) * o.nlt + lt) * KS * 1024 +
+                             ((row & 31) + 32 * hh) * 16;
+#pragma unroll
+          for (int k = 0; k < KS; ++k)
+            *reinterpret_cast<v4i*>(dst + k * 1024) =
+                v4i{(int)w[cls][k][0], (int)w[cls][k][1], (int)w[cls][k][2], (int)w[cls][k][3]};
+        }
+      } else {
+        __syncthreads();
+        // lane lrow + 32 g of block (l tile, ring step, slice) holds the 16
+        // rings 16 g .. 16 g + 15 of the step at l value lrow; this half tile
+        // fills the lanes with lrow in [16 hh, 16 hh + 16)
+#pragma unroll
+        for (int cls = 0; cls < 2; ++cls) {
+          const MixOff& o = cls ? o1 : o0;
+          if (lt >= o.nlt) continue;
+          const int nrs_here = min(8, o.nrs - rb * 8);
+          signed char* base = T + (o.ta / kMixK * KS - tbase);
+          for (int ch = tid; ch < nrs_here * KS * 32; ch += 256) {
+            const int Lh = ch & 31, k = (ch >> 5) % KS, rsl = (ch >> 5) / KS;
+            const int jr = Lh & 15, gg = Lh >> 4;
+            const v4i val = *reinterpret_cast<const v4i*>(&stage[cls][k][jr][rsl * 32 + 16 * gg]);
+            *reinterpret_cast<v4i*>(base + (((long long)lt * o.nrs + rb * 8 + rsl) * KS + k) * 1024 +
+                                    (16 * hh + jr + 32 * gg) * 16) = val;
+          }
+        }
+        __syncthreads();
+      }
+    }
   }
 }
 
@@ -769,8 +862,6 @@ __global__ __launch_bounds__(512, 1) void k_slice_gemm(const SliceArgs A) {
 // ---------------------------------------------------------------------------
 // s_waitcnt lgkmcnt(0) with the other counters unconstrained (gfx9 encoding)
 __device__ inline void wait_lgkm0() { __builtin_amdgcn_s_waitcnt(0xF | (0x3 << 14) | (0x7 << 4)); }
-// keeps the scheduler from moving any instruction across this point
-__device__ inline void sched_fence() { __builtin_amdgcn_sched_barrier(0); }
 
 template <int K, int SYN, int NB>
 __global__ __launch_bounds__(512, 1) void k_slice_gemm_pp(const SliceArgs A) {
@@ -1140,9 +1231,8 @@ static long long t2_cap_limit() {
   return mb > 0 ? std::min(kT2Cap, mb << 20) : kT2Cap;
 }
 
-#define FSHT_FTAB_ARGS(T, TB, FN)                                                                                    \
-  h->d_geo, h->d_i0, h->d_ab, h->d_spin, h->d_aboff, h->d_synoff, h->d_syn_st, h->d_syn_v, h->d_mix_off, T, TB,    \
-      h->d_TAe2[FN], h->d_TSe2[FN], h->d_firstA2[FN], h->d_firstS2[FN], h->lmax, h->nhr
+#define FSHT_FTAB_COMMON \
+  h->d_geo, h->d_i0, h->d_ab, h->d_spin, h->d_aboff, h->d_synoff, h->d_syn_st, h->d_syn_v, h->d_mix_off
 
 static hipError_t mixed_setup_spin2(Handle* h) {
   if (h->mix2_ready) return hipSuccess;
@@ -1164,13 +1254,13 @@ static hipError_t mixed_setup_spin2(Handle* h) {
   }
   const dim3 pgrid((unsigned)h->pitem_off[lmax + 1]);
   if (e == hipSuccess) {
-    hipLaunchKernelGGL((k_ftab<0, 0, 0, kMixK>), pgrid, dim3(256), 0, h->stream, h->d_pitems,
-                       FSHT_FTAB_ARGS(nullptr, 0LL, 0));
+    hipLaunchKernelGGL(k_ftab_exp<0>, pgrid, dim3(256), 0, h->stream, h->d_pitems, FSHT_FTAB_COMMON, h->d_TAe2[0],
+                       h->d_TSe2[0], h->d_firstA2[0], h->d_firstS2[0], h->lmax, h->nhr);
     e = hipGetLastError();
   }
   if (e == hipSuccess) {
-    hipLaunchKernelGGL((k_ftab<1, 0, 0, kMixK>), pgrid, dim3(256), 0, h->stream, h->d_pitems,
-                       FSHT_FTAB_ARGS(nullptr, 0LL, 1));
+    hipLaunchKernelGGL(k_ftab_exp<1>, pgrid, dim3(256), 0, h->stream, h->d_pitems, FSHT_FTAB_COMMON, h->d_TAe2[1],
+                       h->d_TSe2[1], h->d_firstA2[1], h->d_firstS2[1], h->lmax, h->nhr);
     e = hipGetLastError();
   }
   if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
@@ -1355,7 +1445,8 @@ hipError_t launch_ftab(Handle* h, int c, int lay, int K) {
   const int2* items = h->d_pitems + i0;
   const dim3 grid((unsigned)(i1 - i0));
 #define FSHT_FT(FN, LAY, KS) \
-  hipLaunchKernelGGL((k_ftab<FN, 1, LAY, KS>), grid, dim3(256), 0, h->stream, items, FSHT_FTAB_ARGS(h->d_T2[FN], tbase, FN))
+  hipLaunchKernelGGL((k_ftab_dig<FN, LAY, KS>), grid, dim3(256), 0, h->stream, items, FSHT_FTAB_COMMON, h->d_T2[FN], \
+                     tbase, h->d_TAe2[FN], h->d_TSe2[FN], h->lmax, h->nhr)
 #define FSHT_FT2(LAY, KS) \
   FSHT_FT(0, LAY, KS);    \
   FSHT_FT(1, LAY, KS)
