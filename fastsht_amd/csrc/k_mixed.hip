@@ -1437,7 +1437,13 @@ static long long range_base(const Handle* h, int c, int K) { return h->mix_off[2
 
 hipError_t launch_ftab(Handle* h, int c, int lay, int K) {
   const int nchunk = (int)h->mix2_chunk.size() - 1;
-  if (nchunk == 1 && h->t2_lay == lay && h->t2_ks == K) return hipSuccess;
+  // a captured stream always generates (a graph replays what was recorded,
+  // whatever the buffers hold then) and leaves the buffers' content unknown
+  hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+  hipError_t ce = hipStreamIsCapturing(h->stream, &cap);
+  if (ce != hipSuccess) return ce;
+  const bool capturing = cap != hipStreamCaptureStatusNone;
+  if (!capturing && nchunk == 1 && h->t2_lay == lay && h->t2_ks == K) return hipSuccess;
   const int m0 = h->mix2_chunk[c];
   const long long tbase = range_base(h, c, K);
   const int i0 = h->pitem_off[m0], i1 = h->pitem_off[h->mix2_chunk[c + 1]];
@@ -1461,7 +1467,7 @@ hipError_t launch_ftab(Handle* h, int c, int lay, int K) {
   }
 #undef FSHT_FT2
 #undef FSHT_FT
-  h->t2_lay = lay;
+  h->t2_lay = capturing ? -1 : lay;
   h->t2_ks = K;
   return hipGetLastError();
 }

@@ -2,7 +2,7 @@
 
 A job is one transform of `nsim` maps of one shape (nside, lmax): forward
 (t2alm refined with the group's `niter`, plus qu2eb for a polarised job) or
-backward (alm2t, plus eb2qu), in FP64 or mixed precision (temperature only),
+backward (alm2t, plus eb2qu), in FP64 or mixed precision,
 on device tensors in the layouts of the `SHT` object: maps (npix, nsim) and
 alm (nsim, lmax+1, lmax+1), Fortran order.
 

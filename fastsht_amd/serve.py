@@ -2,7 +2,7 @@
 
 A request is one transform of `nsim` maps of one shape: forward (t2alm,
 refined with `niter` Jacobi iterations, plus qu2eb for polarised requests) or
-backward (alm2t, plus eb2qu), in FP64 or mixed precision (temperature only),
+backward (alm2t, plus eb2qu), in FP64 or mixed precision,
 with its inputs and outputs as device tensors in the layouts of the `SHT`
 object: maps (npix, n) and alm (n, lmax+1, lmax+1), Fortran order.
 
@@ -95,14 +95,23 @@ class Request:
 
 def estimate_lane_bytes(nside, lmax, cap, mixed=False):
     """Device bytes of a lane (an upper estimate: spin-2 workspace, batch
-    buffers of three components, tables, and the int8 Legendre tables and
-    operands of mixed precision).  A lane is charged the larger of this
-    estimate and its measured footprint, so admission and accounting agree."""
+    buffers of three components, tables, and for mixed precision the int8 P
+    tables in both operand orders, the sliced operands of two components and
+    the two buffers the spin-2 F1/F2 tables of a range of orders are
+    generated into, at most 6 GiB each).  A lane is charged the larger of this
+    estimate and its measured footprint, so admission and accounting agree;
+    the mixed-precision state of a lane is built when its first mixed
+    transform of each spin runs, and charged when the lane turns mixed."""
     npix, L, nsp = 12 * nside ** 2, lmax + 1, (cap + 15) // 16 * 16
     ws = 8 * npix * (2 * cap + 8 * nsp)
     io = 8 * 3 * cap * (npix + L * L)
     tables = 64 * L * L + 8 * npix + 0.3e9
-    mix = (12 * 0.5 * L * L * 2 * nside + 6 * 2 * nsp * L * (2 * nside + L / 2)) if mixed else 0
+    mix = 0
+    if mixed:
+        ptab = 12 * 0.5 * L * L * 2 * nside                # spin 0: six slices, two operand orders
+        operands = 2 * 6 * 2 * nsp * L * (2 * nside + L / 2)  # FQ and AQ of two components
+        ftab = 2 * min(6 * 2 ** 30, ptab / 2)                # spin 2: F1 and F2 buffers of one range
+        mix = ptab + operands + ftab
     return ws + io + tables + mix
 
 

@@ -81,9 +81,9 @@ class SHT:
     """
 
     def __init__(self, nside, lmax, nsim, niter=0, pol=False, all_buff=False, *, precision="fp64"):
-        """precision="mixed" runs the temperature transforms (t2alm, alm2t) on
-        int8-slice MFMA Legendre contractions refined in FP64 (about 1e-10 of
-        the FP64 result); spin-2 transforms stay FP64."""
+        """precision="mixed" runs the Legendre contractions of t2alm / alm2t
+        and of qu2eb / eb2qu on int8-slice MFMA, refined in FP64 (within
+        1e-10 of the FP64 result); qu2e and qu2b stay FP64."""
         if precision not in ("fp64", "mixed"):
             raise ValueError("precision must be 'fp64' or 'mixed'")
         self.precision = precision
@@ -106,8 +106,10 @@ class SHT:
             self.set_precision("mixed")
 
     def set_precision(self, precision):
-        """Switch the temperature transforms between "fp64" and "mixed" (the
-        first switch to "mixed" builds the handle's int8 Legendre tables)."""
+        """Switch the transforms between "fp64" and "mixed" (the first switch
+        to "mixed" builds the int8 Legendre tables of a temperature-only
+        handle; a spin-2 capable handle builds its temperature tables and its
+        spin-2 state at its first mixed transform of each spin)."""
         if precision not in ("fp64", "mixed"):
             raise ValueError("precision must be 'fp64' or 'mixed'")
         code = _lib.FSHT_PREC_MIXED if precision == "mixed" else _lib.FSHT_PREC_FP64

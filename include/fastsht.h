@@ -89,18 +89,23 @@ int fsht_set_batch(fsht_handle* h, int nsim);
 int fsht_info(const fsht_handle* h, long long* out, int n);
 
 #define FSHT_PREC_FP64 0  /* every Legendre contraction in FP64 (the default) */
-#define FSHT_PREC_MIXED 1 /* temperature transforms on int8-slice MFMA, refined in FP64 */
+#define FSHT_PREC_MIXED 1 /* Legendre contractions on int8-slice MFMA, refined in FP64 */
 
-/* Precision of the temperature transforms (fsht_t2alm, fsht_t2alm_iter,
- * fsht_alm2t) on this handle.  FSHT_PREC_MIXED evaluates each Legendre
- * contraction as exact int8 products of 7-bit fixed-point digits (4 to 6 per
- * operand, v_mfma_i32_32x32x32_i8) and keeps the ring FFTs, the Jacobi
- * residual and the alm accumulation in FP64; t2alm(niter) then matches the
- * FP64 result to about 1e-10 (max |diff| / max |alm|) and alm2t to about
- * 1e-11.  No counterpart in the reference, whose GEMMs (src/sht_core.f90:30-168)
- * are FP64 only.  The first switch to FSHT_PREC_MIXED builds the sliced
- * Legendre tables (about 12 bytes per (m, l, ring) term plus sliced-operand
- * workspace) and returns when they are ready; spin-2 transforms stay FP64. */
+/* Precision of the Legendre contractions of this handle's transforms
+ * (fsht_t2alm, fsht_t2alm_iter, fsht_alm2t, fsht_qu2eb, fsht_eb2qu; fsht_qu2e
+ * and fsht_qu2b stay FP64).  FSHT_PREC_MIXED evaluates each contraction as
+ * exact int8 products of 7-bit fixed-point digits (4 to 6 per operand,
+ * v_mfma_i32_32x32x32_i8) and keeps the ring FFTs, the Jacobi residual and
+ * the alm accumulation in FP64; t2alm(niter) and qu2eb(niter) then match the
+ * FP64 result to about 1e-10 (max |diff| / max |alm|), alm2t and eb2qu to
+ * about 1e-11.  No counterpart in the reference, whose GEMMs
+ * (src/sht_core.f90:30-168) are FP64 only.  Switching a temperature-only
+ * handle to FSHT_PREC_MIXED builds its sliced Legendre tables (about 12 bytes
+ * per (m, l, ring) term plus sliced-operand workspace) before returning; a
+ * spin-2 capable handle builds its temperature tables at its first mixed
+ * temperature transform, and at its first mixed spin-2 transform the F1/F2
+ * exponents and two buffers (at most 6 GiB each) that each pass generates
+ * the F1/F2 tables of a range of orders into. */
 int fsht_set_precision(fsht_handle* h, int precision);
 
 /* The ring-skip table used (first kept north ring per m, m = 0..lmax), as read

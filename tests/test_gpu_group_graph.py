@@ -14,11 +14,13 @@ pytestmark = pytest.mark.gpu
 TOL = 1e-14
 
 # (nside, lmax, nsim, pol, direction, precision); two jobs share each of the
-# shapes (16, 32) and (32, 95)
+# shapes (16, 32), (32, 95), (32, 64) and (16, 47), the last two with a
+# polarised mixed-precision job (spin-2 tables generated inside the graph)
 JOBS = [(16, 32, 5, False, "forward", "fp64"), (32, 95, 17, True, "forward", "fp64"),
         (16, 47, 33, True, "backward", "fp64"), (32, 64, 40, False, "forward", "mixed"),
         (16, 32, 12, True, "forward", "fp64"), (64, 128, 3, False, "backward", "mixed"),
-        (32, 95, 9, False, "backward", "fp64")]
+        (32, 95, 9, False, "backward", "fp64"), (32, 64, 7, True, "forward", "mixed"),
+        (16, 47, 6, True, "backward", "mixed")]
 NITER = 2
 
 
