@@ -275,6 +275,16 @@ def pct(lat, p):
     return lat[min(len(lat) - 1, max(0, int(math.ceil(p / 100.0 * len(lat))) - 1))]
 
 
+def load_executed(kernel, prec, nside, nsim):
+    """Executed-MFMA fraction of a Legendre class (MFMA work actually issued
+    per second over the peak, from the rocprofv3 counters kept under
+    profiles/ for this shape), or None."""
+    tf = os.path.join(ROOT, "profiles", f"mfma_executed_nside{nside}_nsim{nsim}.json")
+    if os.path.exists(tf):
+        return json.load(open(tf)).get(prec, {}).get(kernel)
+    return None
+
+
 def load_traffic(kernel, tag, nside, nsim):
     tf = os.path.join(ROOT, "profiles", f"traffic_{kernel}{tag}_nside{nside}_nsim{nsim}.json")
     if os.path.exists(tf):
@@ -845,9 +855,11 @@ def main():
             fp = legendre_roofline(r["kern"], args.steps, fp64_work, "TFLOP/s", PEAK_FP64_MFMA_TFLOPS, passes)[1]
             roof["fp64_equivalent_tflops"] = fp["achieved"]
             roof["traffic"] = load_traffic(dom, "_mixed", args.nside, args.nsim)
+            roof["executed_frac"] = load_executed(dom, "mixed", args.nside, args.nsim)
         else:
             dom, roof = legendre_roofline(r["kern"], args.steps, fp64_work, "TFLOP/s", PEAK_FP64_MFMA_TFLOPS, passes)
             roof["traffic"] = load_traffic(dom, "", args.nside, args.nsim)
+            roof["executed_frac"] = load_executed(dom, "fp64", args.nside, args.nsim)
         roofs[prec] = roof
 
     def summary(prec):
