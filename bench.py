@@ -366,8 +366,8 @@ def pol_round_trip(args, lmax):
     for pr in precs:
         sht.set_precision(pr)
         runs[pr] = timed_steps(sht, lib, step, args.steps, args.warmup)
-    info = (ctypes.c_longlong * 36)()
-    _lib.check(lib.fsht_info(sht._h, info, 36), sht._h)
+    info = (ctypes.c_longlong * 39)()
+    _lib.check(lib.fsht_info(sht._h, info, 39), sht._h)
     ladder = slice_ladder(list(info), args.niter, spin=2)
     fpair = 4 * legendre_flops_per_map(args.nside, lmax, sht.ring_tab())  # F1 and F2 against Q and U
     passes = {"legendre_analysis": args.niter + 1, "legendre_synthesis": args.niter + 1}
@@ -823,8 +823,8 @@ def main():
         sht.alm2t(alm_out, maps_out)
 
     sht.set_precision("mixed")  # builds the int8 tables before any timing
-    info = (ctypes.c_longlong * 36)()
-    _lib.check(lib.fsht_info(sht._h, info, 36), sht._h)
+    info = (ctypes.c_longlong * 39)()
+    _lib.check(lib.fsht_info(sht._h, info, 39), sht._h)
     ladder = slice_ladder(list(info), args.niter)
     runs = {}
     for prec in ("fp64", "mixed"):

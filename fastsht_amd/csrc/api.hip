@@ -211,7 +211,7 @@ int fsht_set_batch(fsht_handle* H, int nsim) {
 int fsht_info(const fsht_handle* H, long long* out, int n) {
   if (!H || !out) return fail(nullptr, FSHT_ERR_ARG, "NULL argument");
   const Handle* h = &H->h;
-  long long v[36] = {h->nside,    h->lmax,     h->nsim,     h->niter,        h->pol,
+  long long v[39] = {h->nside,    h->lmax,     h->nsim,     h->niter,        h->pol,
                      h->npix,     h->nsimp,    h->G_size,   h->precision,    h->mix_bytes,
                      h->mix_k_lo, h->mix_k_hi, h->mix_damp_min, h->mix_k_out, h->mix_damp_min_a,
                      h->nsim_cap};
@@ -219,7 +219,8 @@ int fsht_info(const fsht_handle* H, long long* out, int n) {
     for (int k = 0; k < 9; ++k) v[16 + 10 * sp + k] = h->mix_lad[sp][k / 3][k % 3];
     v[25 + 10 * sp] = h->mix_out[sp];
   }
-  for (int i = 0; i < n && i < 36; ++i) out[i] = v[i];
+  fsht::mixed_planned_bytes(h, v + 36);
+  for (int i = 0; i < n && i < 39; ++i) out[i] = v[i];
   return FSHT_OK;
 }
 

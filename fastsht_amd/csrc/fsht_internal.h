@@ -123,6 +123,10 @@ hipError_t upload_items(const std::vector<LegItem>& v, const std::vector<double>
 // mixed precision (k_mixed.hip)
 hipError_t mixed_setup(Handle* h);        // shared state, and the P tables of a temperature-only handle
 hipError_t mixed_ensure(Handle* h, int spin);  // the state a spin-0 or spin-2 transform needs (set-up on first use)
+// Device bytes the mixed-precision state would take, planned on the host:
+// out[0] the shared sliced operands, out[1] the spin-0 tables, out[2] the
+// spin-2 exponents and range buffers.
+void mixed_planned_bytes(const Handle* h, long long out[3]);
 void mixed_free(Handle* h);
 hipError_t launch_fexp_reset(Handle* h);                          // FQe <- kMixZeroE
 hipError_t launch_quant_F(Handle* h, const double* F, int K);      // m-major FP64 operand -> FQ (FQe set)

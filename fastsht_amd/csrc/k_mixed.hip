@@ -1093,65 +1093,87 @@ static hipError_t build_q_lists(Handle* h, int m0, int m1, int nct, const std::v
   return hipSuccess;
 }
 
-// Offsets, sliced operands (room for the handle's field components) and the
-// (m, ring block) items of the table kernels: shared by spin 0 and spin 2.
-// The analysis operand FQ lives in the map-major spectrum buffer when it
-// fits there: that buffer only holds data between a transform's ring FFTs
-// and its transposes, while FQ only lives between a gather and the analysis.
-static hipError_t mixed_setup_common(Handle* h) {
-  if (h->mix_ready) return hipSuccess;
-  const int lmax = h->lmax, nm = lmax + 1;
-  const int ncw = h->ncomp_ws;
-  const int nct = h->nsimp / 16;  // 32-column tiles of the 2*nsimp columns of one component
-  const int ncols = 2 * ncw * h->nsimp;
-  h->mix_off.assign(2 * (size_t)nm, MixOff{});
+// Host plan of the mixed-precision state (no device memory): per (m, class)
+// the offsets into the tables and the sliced operands (room for the handle's
+// field components), the (m, ring block) items of the table kernels and the
+// totals.  Shared by spin 0 and spin 2.
+struct MixPlan {
+  std::vector<MixOff> off;
+  std::vector<int> pitem_off;
+  std::vector<int2> pitems;
   long long ta = 0, fq = 0, aq = 0, tae = 0, tse = 0;
   int fa = 0, fs = 0, nrb_max = 0;
-  std::vector<int2> pitems;
-  h->pitem_off.assign((size_t)nm + 1, 0);
+};
+
+static void plan_offsets(const Handle* h, MixPlan& P) {
+  const int lmax = h->lmax, nm = lmax + 1;
+  const int nct = h->nsimp / 16;  // 32-column tiles of the 2*nsimp columns of one component
+  const int ncols = 2 * h->ncomp_ws * h->nsimp;
+  P.off.assign(2 * (size_t)nm, MixOff{});
+  P.pitem_off.assign((size_t)nm + 1, 0);
   for (int m = 0; m <= lmax; ++m) {
     const int R = h->nhr + 1 - h->i0[m];
     const int nrs = cdiv(R, 32), nrb = cdiv(nrs, 8);
-    nrb_max = std::max(nrb_max, nrb);
-    h->pitem_off[m] = (int)pitems.size();
-    for (int rb = 0; rb < nrb; ++rb) pitems.push_back(make_int2(m, rb));
+    P.nrb_max = std::max(P.nrb_max, nrb);
+    P.pitem_off[m] = (int)P.pitems.size();
+    for (int rb = 0; rb < nrb; ++rb) P.pitems.push_back(make_int2(m, rb));
     for (int c = 0; c < 2; ++c) {
       const int nval = (lmax >= m + c) ? (lmax - m - c) / 2 + 1 : 0;
       const int nlt = cdiv(nval, 32);
-      MixOff& o = h->mix_off[2 * m + c];
+      MixOff& o = P.off[2 * m + c];
       o.nrs = nrs;
       o.nlt = nlt;
-      o.ta = o.ts = ta;
-      ta += (long long)nlt * nrs * kMixK * 1024;
-      o.fq = fq;
-      fq += (long long)nct * nrs * kMixK * 1024;
-      o.aq = aq;
-      aq += (long long)nct * nlt * kMixK * 1024;
-      o.tae = tae;
-      tae += (long long)nlt * 32;
-      o.tse = tse;
-      tse += (long long)nrs * 32;
+      o.ta = o.ts = P.ta;
+      P.ta += (long long)nlt * nrs * kMixK * 1024;
+      o.fq = P.fq;
+      P.fq += (long long)nct * nrs * kMixK * 1024;
+      o.aq = P.aq;
+      P.aq += (long long)nct * nlt * kMixK * 1024;
+      o.tae = P.tae;
+      P.tae += (long long)nlt * 32;
+      o.tse = P.tse;
+      P.tse += (long long)nrs * 32;
       o.fqe = o.aqe = (long long)(2 * m + c) * ncols;
-      o.fa = fa;
-      fa += nlt;
-      o.fs = fs;
-      fs += nrs;
+      o.fa = P.fa;
+      P.fa += nlt;
+      o.fs = P.fs;
+      P.fs += nrs;
     }
   }
-  h->pitem_off[nm] = (int)pitems.size();
-  h->mix_nrb_max = nrb_max;
-  h->mix_ta_total = ta;
-  h->mix_tae = tae;
-  h->mix_tse = tse;
-  h->mix_fa = fa;
-  h->mix_fs = fs;
+  P.pitem_off[nm] = (int)P.pitems.size();
+}
+
+// The analysis operand FQ lives in the map-major spectrum buffer when it
+// fits there: that buffer only holds data between a transform's ring FFTs
+// and its transposes, while FQ only lives between a gather and the analysis.
+static bool fq_fits_spec(const Handle* h, long long fq_bytes) {
+  return fq_bytes <= (long long)h->ncomp_ws * h->nsim_cap * h->npix * 8;
+}
+
+static hipError_t mixed_setup_common(Handle* h) {
+  if (h->mix_ready) return hipSuccess;
+  const int nm = h->lmax + 1;
+  const int ncw = h->ncomp_ws;
+  const int ncols = 2 * ncw * h->nsimp;
+  MixPlan P;
+  plan_offsets(h, P);
+  h->mix_off = std::move(P.off);
+  h->pitem_off = std::move(P.pitem_off);
+  const std::vector<int2>& pitems = P.pitems;
+  const long long fq = P.fq, aq = P.aq;
+  h->mix_nrb_max = P.nrb_max;
+  h->mix_ta_total = P.ta;
+  h->mix_tae = P.tae;
+  h->mix_tse = P.tse;
+  h->mix_fa = P.fa;
+  h->mix_fs = P.fs;
   const long long nexp = 2LL * nm * ncols;
   const long long fq_bytes = (long long)ncw * fq;
   hipError_t e = hipSuccess;
   auto al = [&](void** p, long long bytes) {
     if (e == hipSuccess) e = hipMalloc(p, (size_t)std::max(bytes, 256LL));
   };
-  if (fq_bytes <= (long long)ncw * h->nsim_cap * h->npix * 8) {
+  if (fq_fits_spec(h, fq_bytes)) {
     h->d_FQ = reinterpret_cast<signed char*>(h->ws_spec);
     h->fq_alias = 1;
   } else {
@@ -1220,15 +1242,40 @@ static hipError_t mixed_setup_spin0(Handle* h) {
 }
 
 // Spin 2: the F1/F2 exponents and first nonzero steps (resident), the plan of
-// m ranges whose tables are generated per pass (at most kT2Cap bytes per
-// function at six slices; FSHT_T2_CAP_MB lowers it, e.g. to exercise many
-// ranges on small grids), the chunk buffers and the work lists per
-// (function, range).
+// m ranges whose tables are generated per pass, the chunk buffers and the
+// work lists per (function, range).  A range's tables (one function, six
+// slices) take at most min(kT2Cap, max(kT2Min, 1/16 of the handle's FP64
+// workspace)) bytes: a handle sized for few maps (a serving lane) keeps small
+// buffers; FSHT_T2_CAP_MB lowers the limit further (e.g. to exercise many
+// ranges on small grids).
 constexpr long long kT2Cap = 6LL << 30;
-static long long t2_cap_limit() {
+constexpr long long kT2Min = 256LL << 20;
+static long long t2_cap_limit(const Handle* h) {
+  const long long ws = 8LL * (2LL * h->nsim_cap * h->npix + 2LL * h->npix * h->nsimp + 2LL * h->G_size);
+  long long cap = std::min(kT2Cap, std::max(kT2Min, ws / 16));
   const char* v = std::getenv("FSHT_T2_CAP_MB");
   const long long mb = v ? std::atoll(v) : 0;
-  return mb > 0 ? std::min(kT2Cap, mb << 20) : kT2Cap;
+  return mb > 0 ? std::min(cap, mb << 20) : cap;
+}
+
+// m ranges of the spin-2 tables: whole orders, each range's tables (one
+// function, one operand order, six slices) within cap_limit bytes unless a
+// single order exceeds it.  bounds receives the first order of every range
+// and lmax + 1; returns the bytes of the largest range.
+static long long plan_t2_ranges(const std::vector<MixOff>& off, long long ta_total, int lmax, long long cap_limit,
+                                std::vector<int>& bounds) {
+  bounds.assign(1, 0);
+  long long cap = 0;
+  for (int m = 0; m <= lmax;) {
+    int m1 = m;
+    const long long start = off[2 * m].ta;
+    while (m1 <= lmax && (m1 == m || (m1 + 1 <= lmax ? off[2 * (m1 + 1)].ta : ta_total) - start <= cap_limit)) ++m1;
+    const long long end = m1 <= lmax ? off[2 * m1].ta : ta_total;
+    cap = std::max(cap, end - start);
+    bounds.push_back(m1);
+    m = m1;
+  }
+  return cap;
 }
 
 #define FSHT_FTAB_COMMON \
@@ -1272,21 +1319,7 @@ static hipError_t mixed_setup_spin2(Handle* h) {
     if (e == hipSuccess) e = hipMemcpy(firstS[f].data(), h->d_firstS2[f], (size_t)fs * 4, hipMemcpyDeviceToHost);
   }
   if (e != hipSuccess) return e;
-  // m ranges: whole orders, each range's tables (one function, one operand
-  // order, six slices) within kT2Cap
-  h->mix2_chunk.assign(1, 0);
-  long long cap = 0;
-  const long long cap_limit = t2_cap_limit();
-  for (int m = 0; m <= lmax;) {
-    int m1 = m;
-    const long long start = h->mix_off[2 * m].ta;
-    while (m1 <= lmax && (m1 == m || (m1 + 1 <= lmax ? h->mix_off[2 * (m1 + 1)].ta : h->mix_ta_total) - start <= cap_limit))
-      ++m1;
-    const long long end = m1 <= lmax ? h->mix_off[2 * m1].ta : h->mix_ta_total;
-    cap = std::max(cap, end - start);
-    h->mix2_chunk.push_back(m1);
-    m = m1;
-  }
+  const long long cap = plan_t2_ranges(h->mix_off, h->mix_ta_total, lmax, t2_cap_limit(h), h->mix2_chunk);
   const int nchunk = (int)h->mix2_chunk.size() - 1;
   h->t2_cap = cap;
   al((void**)&h->d_T2[0], cap);
@@ -1308,6 +1341,18 @@ static hipError_t mixed_setup_spin2(Handle* h) {
   h->t2_lay = -1;
   h->mix2_ready = 1;
   return hipSuccess;
+}
+
+void mixed_planned_bytes(const Handle* h, long long out[3]) {
+  MixPlan P;
+  plan_offsets(h, P);
+  const int ncw = h->ncomp_ws;
+  const long long nexp = 2LL * (h->lmax + 1) * 2 * ncw * h->nsimp;
+  const long long fq_bytes = (long long)ncw * P.fq;
+  out[0] = (fq_fits_spec(h, fq_bytes) ? 0 : fq_bytes) + (long long)ncw * P.aq + 8 * nexp;
+  out[1] = 2 * P.ta + 4 * (P.tae + P.tse + P.fa + P.fs);
+  std::vector<int> bounds;
+  out[2] = 2 * plan_t2_ranges(P.off, P.ta, h->lmax, t2_cap_limit(h), bounds) + 8 * (P.tae + P.tse + P.fa + P.fs);
 }
 
 hipError_t mixed_setup(Handle* h) {

@@ -35,6 +35,7 @@ another kind of column block (about 1e-16 relative).
 from __future__ import annotations
 
 import ctypes
+import math
 import itertools
 import time
 from collections import deque
@@ -93,25 +94,33 @@ class Request:
         return 8 * n * (12 * self.nside ** 2 + L * L)
 
 
-def estimate_lane_bytes(nside, lmax, cap, mixed=False):
+def estimate_lane_bytes(nside, lmax, cap, mixed=False, mixed_pol=False):
     """Device bytes of a lane (an upper estimate: spin-2 workspace, batch
-    buffers of three components, tables, and for mixed precision the int8 P
-    tables in both operand orders, the sliced operands of two components and
-    the two buffers the spin-2 F1/F2 tables of a range of orders are
-    generated into, at most 6 GiB each).  A lane is charged the larger of this
-    estimate and its measured footprint, so admission and accounting agree;
-    the mixed-precision state of a lane is built when its first mixed
-    transform of each spin runs, and charged when the lane turns mixed."""
+    buffers of three components, tables; with `mixed`, the int8 P tables in
+    both operand orders and the sliced operands of two components; with
+    `mixed_pol` also the two buffers the spin-2 F1/F2 tables of a range of
+    orders are generated into, at most 6 GiB each).  Admission of a request
+    that needs a new lane reserves this estimate; the lane, once created, is
+    charged what it holds: the library's allocation at creation (measured)
+    and its batch buffers, and, when its first mixed (and first polarised
+    mixed) request is admitted, the bytes the library plans for that state
+    (fsht_info), which it builds at its first mixed transform of each spin."""
     npix, L, nsp = 12 * nside ** 2, lmax + 1, (cap + 15) // 16 * 16
     ws = 8 * npix * (2 * cap + 8 * nsp)
     io = 8 * 3 * cap * (npix + L * L)
     tables = 64 * L * L + 8 * npix + 0.3e9
     mix = 0
-    if mixed:
-        ptab = 12 * 0.5 * L * L * 2 * nside                # spin 0: six slices, two operand orders
-        operands = 2 * 6 * 2 * nsp * L * (2 * nside + L / 2)  # FQ and AQ of two components
-        ftab = 2 * min(6 * 2 ** 30, ptab / 2)                # spin 2: F1 and F2 buffers of one range
-        mix = ptab + operands + ftab
+    ptab = 12 * 0.5 * L * L * 2 * nside  # spin 0: six slices, two operand orders
+    if mixed or mixed_pol:
+        # sliced operands of two components: AQ, and FQ unless it fits in the
+        # map-major spectrum buffer (2 cap maps), where the library keeps it
+        aq = 2 * 2 * (nsp // 16) * 6144 * L * (L / 64 + 1)
+        fq = 2 * 2 * (nsp // 16) * 6144 * L * math.ceil(2 * nside / 32)
+        mix += ptab + aq + (0 if fq <= 16 * cap * npix else fq)
+    if mixed_pol:
+        # spin 2: F1 and F2 buffers of one range of orders, each at most 6 GiB,
+        # 1/16 of the FP64 workspace and the function's whole table
+        mix += 2 * min(6 * 2 ** 30, max(2 ** 28, ws / 16), ptab / 2)
     return ws + io + tables + mix
 
 
@@ -131,9 +140,9 @@ class Lane:
         with torch.cuda.stream(self.stream):  # batch buffers: T, then Q/U or E/B
             self.maps = torch.empty((3, self.npix * cap), dtype=torch.float64, device="cuda")
             self.alms = torch.empty((3, cap * self.L * self.L), dtype=torch.float64, device="cuda")
-        measured = lib_bytes + 8 * (self.maps.numel() + self.alms.numel())
-        self.footprint = max(measured, estimate_lane_bytes(nside, lmax, cap))
+        self.footprint = lib_bytes + 8 * (self.maps.numel() + self.alms.numel())
         self.mixed_ready = False
+        self.mixed_pol_ready = False  # charged for the spin-2 mixed-precision buffers
         self.pending = deque()  # requests (or their remaining maps) in arrival order
         self.inflight = None    # (event, chunks, precision)
         self.last_used = time.perf_counter()
@@ -145,16 +154,27 @@ class Lane:
         L = self.L
         return self.alms[c, :n * L * L].view(L, L, n).permute(2, 1, 0)
 
-    def enable_mixed(self):
-        torch = _torch()
-        torch.cuda.synchronize()
-        free0 = torch.cuda.mem_get_info()[0]
+    def mixed_bytes(self, mixed, pol):
+        """Device bytes of the lane's mixed-precision state as the library
+        plans it (fsht_info): the sliced operands and temperature tables with
+        `mixed`, and also the spin-2 exponents and range buffers with `pol`."""
+        info = (ctypes.c_longlong * 39)()
+        _lib.check(self.lib.fsht_info(self.sht._h, info, 39), self.sht._h)
+        return (info[36] + info[37] if mixed or pol else 0) + (info[38] if pol else 0)
+
+    def _switch_mixed(self):
         self.sht.set_precision("mixed")
-        torch.cuda.synchronize()
-        est = (estimate_lane_bytes(self.nside, self.lmax, self.cap, True)
-               - estimate_lane_bytes(self.nside, self.lmax, self.cap))
-        self.footprint += max(free0 - torch.cuda.mem_get_info()[0], est)
-        self.mixed_ready = True
+
+    def enable_mixed(self, pol=False):
+        """Switch the lane's handle to mixed precision (once) and charge the
+        lane for the state its requests will build: the temperature tables and
+        sliced operands, and with `pol` the spin-2 buffers (the library builds
+        each part at the first transform that needs it)."""
+        if not self.mixed_ready:
+            self._switch_mixed()
+        pol = pol or self.mixed_pol_ready
+        self.footprint += self.mixed_bytes(True, pol) - self.mixed_bytes(self.mixed_ready, self.mixed_pol_ready)
+        self.mixed_ready, self.mixed_pol_ready = True, pol
 
     def take_batch(self):
         """The next batch: the oldest pending requests of the oldest request's
@@ -305,11 +325,12 @@ class Server:
             lane = self.lanes.get(key)
             need = r.nbytes()
             mixed = r.precision_used == "mixed"
+            mpol = mixed and r.pol
             if lane is None:
-                need += estimate_lane_bytes(r.nside, r.lmax, self.cap_for(*key), mixed)
-            elif mixed and not lane.mixed_ready:
-                need += estimate_lane_bytes(r.nside, r.lmax, lane.cap, True) - estimate_lane_bytes(
-                    r.nside, r.lmax, lane.cap)
+                need += estimate_lane_bytes(r.nside, r.lmax, self.cap_for(*key), mixed, mpol)
+            elif (mixed and not lane.mixed_ready) or (mpol and not lane.mixed_pol_ready):
+                need += (lane.mixed_bytes(mixed or lane.mixed_ready, mpol or lane.mixed_pol_ready)
+                         - lane.mixed_bytes(lane.mixed_ready, lane.mixed_pol_ready))
             if self.lane_bytes() + self.held + need > self.hbm_budget and not self._evict(need, keep=key):
                 if not any(l.pending or l.inflight is not None for l in self.lanes.values()) and self.held == 0:
                     raise RuntimeError(f"request {r.rid} needs {need / GB:.1f} GB, more than the "
@@ -319,8 +340,8 @@ class Server:
             if lane is None:
                 lane = self.lane_factory(self, r.nside, r.lmax, self.cap_for(*key))
                 self.lanes[key] = lane
-            if r.precision_used == "mixed" and not lane.mixed_ready:
-                lane.enable_mixed()
+            if r.precision_used == "mixed":
+                lane.enable_mixed(pol=r.pol)
             self.held += r.nbytes()
             lane.admit(r)
 

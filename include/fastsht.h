@@ -85,7 +85,12 @@ int fsht_set_batch(fsht_handle* h, int nsim);
  * spin 2 (out[26..36)), the slices of every pass as a table: out[b + 3 k + d]
  * for pass kind k (0: the first analysis, 1: a residual synthesis, 2: a
  * residual analysis) whose error d = min(2, damping) later iterations damp,
- * and out[b + 9] for alm2t / eb2qu and an unrefined analysis. */
+ * and out[b + 9] for alm2t / eb2qu and an unrefined analysis; then the device
+ * bytes the mixed-precision state takes once built, planned on the host
+ * whether or not it is built yet: out[36] the sliced operands (set up by
+ * fsht_set_precision), out[37] the temperature tables, out[38] the spin-2
+ * exponents and the two range buffers (lets a caller budget a handle's
+ * memory before switching it to mixed precision). */
 int fsht_info(const fsht_handle* h, long long* out, int n);
 
 #define FSHT_PREC_FP64 0  /* every Legendre contraction in FP64 (the default) */
@@ -104,8 +109,9 @@ int fsht_info(const fsht_handle* h, long long* out, int n);
  * per (m, l, ring) term plus sliced-operand workspace) before returning; a
  * spin-2 capable handle builds its temperature tables at its first mixed
  * temperature transform, and at its first mixed spin-2 transform the F1/F2
- * exponents and two buffers (at most 6 GiB each) that each pass generates
- * the F1/F2 tables of a range of orders into. */
+ * exponents and two buffers that each pass generates the F1/F2 tables of a
+ * range of orders into (each at most 6 GiB and 1/16 of the handle's FP64
+ * workspace, at least 256 MiB or one order's tables). */
 int fsht_set_precision(fsht_handle* h, int precision);
 
 /* The ring-skip table used (first kept north ring per m, m = 0..lmax), as read

@@ -136,6 +136,42 @@ def test_mixed_spin2_many_ranges(torch_cuda, monkeypatch):
     _spin2_case(64, 127, 5)
 
 
+def test_planned_mixed_bytes_match_allocation(torch_cuda):
+    """fsht_info's planned mixed-precision bytes (out[36..39)) equal the bytes
+    the library then holds (out[9]) after each part is built: the sliced
+    operands at fsht_set_precision, the temperature tables at the first mixed
+    t2alm, the spin-2 buffers at the first mixed qu2eb (also with ranges
+    capped at 1 MB)."""
+    import ctypes
+
+    from fastsht_amd import SHT, _lib
+    lib = _lib.load()
+
+    def info(S):
+        v = (ctypes.c_longlong * 39)()
+        _lib.check(lib.fsht_info(S._h, v, 39), S._h)
+        return list(v)
+
+    for ns, lmax, nsim, cap in ((32, 64, 20, None), (64, 127, 5, "1"), (128, 383, 40, None)):
+        print(f"case planned bytes nside={ns} lmax={lmax} nsim={nsim} range cap={cap}", flush=True)
+        with pytest.MonkeyPatch.context() as mp:
+            if cap:
+                mp.setenv("FSHT_T2_CAP_MB", cap)
+            S = SHT(ns, lmax, nsim, niter=1, pol=True)
+            v = info(S)
+            planned = v[36:39]
+            assert v[9] == 0 and min(planned) > 0
+            S.set_precision("mixed")
+            assert info(S)[9] == planned[0]
+            R = O.OracleSHT(ns, lmax, nsim)
+            Q, U = _qu_maps(R, lmax, nsim, seed=3)
+            S.t2alm(Q)
+            assert info(S)[9] == planned[0] + planned[1]
+            S.qu2eb(Q, U)
+            v = info(S)
+            assert v[9] == sum(planned) and v[36:39] == planned
+
+
 def test_mixed_spin2_vs_fp64_gpu_nside256(torch_cuda):
     """nside 256, lmax 511, 40 Q/U pairs (six 32-column tiles: one full block
     and two single tiles): qu2eb niter 3 and eb2qu against the FP64 GPU path."""

@@ -19,6 +19,7 @@ class FakeLane(Lane):
         self.nside, self.lmax, self.cap = nside, lmax, cap
         self.footprint = estimate_lane_bytes(nside, lmax, cap)
         self.mixed_ready = False
+        self.mixed_pol_ready = False
         self.pending = deque()
         self.inflight = None
         self.last_used = 0.0
@@ -26,8 +27,12 @@ class FakeLane(Lane):
         self.closed = False
         FakeLane.made.append(self)
 
-    def enable_mixed(self):
-        self.mixed_ready = True
+    def mixed_bytes(self, mixed, pol):
+        base = estimate_lane_bytes(self.nside, self.lmax, self.cap)
+        return estimate_lane_bytes(self.nside, self.lmax, self.cap, mixed, pol) - base
+
+    def _switch_mixed(self):
+        pass
 
     def admit(self, r):
         r.outputs = ()
@@ -107,3 +112,23 @@ def test_request_above_budget_raises():
     srv.submit(req(16, 3))
     with pytest.raises(RuntimeError, match="budget"):
         srv.run_until_idle()
+
+
+def test_spin2_buffers_charged_on_first_polarised_mixed_request():
+    """A lane is charged the spin-2 chunk buffers only when its first
+    polarised mixed request is admitted, once."""
+    srv = server(hbm_budget=1e13, capacity=lambda ns, lm: 16)
+    t = srv.submit(req(32, 3, precision="mixed"))
+    srv.run_until_idle()
+    lane = [l for l in FakeLane.made if l.nside == 32][-1]
+    assert lane.mixed_ready and not lane.mixed_pol_ready and t.done
+    q = srv.submit(req(32, 2, pol=True, precision="mixed"))
+    srv.run_until_idle()
+    assert lane.mixed_pol_ready and q.done
+    base = estimate_lane_bytes(32, 64, 16, True)
+    full = estimate_lane_bytes(32, 64, 16, True, True)
+    assert full > base > estimate_lane_bytes(32, 64, 16)
+    assert lane.footprint == pytest.approx(full)
+    srv.submit(req(32, 2, pol=True, precision="mixed"))
+    srv.run_until_idle()
+    assert lane.footprint == pytest.approx(full)  # charged once
