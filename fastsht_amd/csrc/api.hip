@@ -78,7 +78,7 @@ static void free_all(Handle* h) {
   void* ptrs[] = {h->d_geo, h->d_i0, h->d_ab, h->d_spin, h->d_aboff, h->d_tw, h->d_slot, h->d_seedoff,
                   h->d_seed_st, h->d_seed_v, h->d_seed_p, h->d_synoff, h->d_syn_st, h->d_syn_v, h->d_goff, h->d_ana_items[0][0], h->d_ana_items[0][1],
                   h->d_ana_items[1][0], h->d_ana_items[1][1], h->d_syn_items[0], h->d_syn_items[1], h->d_fft_jobs, h->d_blue, h->d_chirp, h->d_filt,
-                  h->d_twid, h->ws_spec, h->ws_specT, h->ws_G, h->ws_F, h->ws_alm[0], h->ws_alm[1]};
+                  h->d_twid, h->ws_spec, h->ws_F0, h->ws_G, h->ws_F, h->ws_alm[0], h->ws_alm[1]};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (int b = 0; b < 2; ++b) {
@@ -146,11 +146,10 @@ int fsht_create(int nside, int lmax, int nsim, int niter, int pol, fsht_handle**
   h->ncomp_ws = ncomp;
   fsht::synth_offsets(h, 2);
   const long long nspec = (long long)ncomp * nsim * h->npix;
-  const long long nspecT = h->npix * (long long)ncomp * h->nsimp;
   const long long nG = h->G_size / (ncomp == 2 ? 1 : 2);
   auto alloc = [&](double** p, long long n) -> hipError_t { return hipMalloc((void**)p, (size_t)n * sizeof(double)); };
   e = alloc(&h->ws_spec, nspec);
-  if (e == hipSuccess) e = alloc(&h->ws_specT, nspecT);
+  if (e == hipSuccess) e = alloc(&h->ws_F0, nG);
   if (e == hipSuccess) e = alloc(&h->ws_G, nG);
   if (e == hipSuccess && niter > 0) e = alloc(&h->ws_F, nG);
   if (e != hipSuccess) {
@@ -446,7 +445,7 @@ static int d2h_alm(Handle* h, double* host, const double* dev) {
                      [](size_t, size_t) { return FSHT_OK; });
 }
 
-// forward ring FFT of ncomp map sets + transpose into ws_specT.  Host maps
+// forward ring FFT of ncomp map sets into ws_spec (map-major).  Host maps
 // are uploaded chunk by chunk, each chunk's FFTs starting once it is on the GPU.
 static int forward_spectra(Handle* h, const double* m0, const double* m1, int loc) {
   const long long cnt = (long long)h->nsim * h->npix;
@@ -466,15 +465,13 @@ static int forward_spectra(Handle* h, const double* m0, const double* m1, int lo
       if (rc) return rc;
     }
   }
-  HIPCHK(h, timed(h, fsht::KC_TRANSPOSE, [&] { return fsht::launch_transpose_to_T(h, h->ws_spec, h->ws_specT); }));
   return FSHT_OK;
 }
 
-// spectrum in ws_specT -> maps.  Host maps are downloaded chunk by chunk
-// behind the inverse FFTs of the following chunks.
+// spectra in ws_spec (map-major) -> maps.  Host maps are downloaded chunk by
+// chunk behind the inverse FFTs of the following chunks.
 static int inverse_spectra(Handle* h, double* m0, double* m1, int loc) {
   const long long cnt = (long long)h->nsim * h->npix;
-  HIPCHK(h, timed(h, fsht::KC_TRANSPOSE, [&] { return fsht::launch_transpose_from_T(h, h->ws_specT, h->ws_spec); }));
   if (loc) {
     double* out[2] = {m0, m1};
     HIPCHK(h, timed(h, fsht::KC_FFT_INV, [&] { return fsht::launch_ring_fft(h, nullptr, h->ws_spec, 1, out); }));
@@ -553,10 +550,10 @@ static int analysis_iter_mixed(Handle* h, double* a0, double* a1, int niter, int
   const int sp = h->spin ? 1 : 0;
   auto lad = [&](int kind, int damping) { return h->mix_lad[sp][kind][damping < 2 ? damping : 2]; };
   HIPCHK(h, fsht::launch_fexp_reset(h));
-  HIPCHK(h, timed(h, fsht::KC_GATHER, [&] { return fsht::launch_gather(h, h->ws_specT, h->ws_G, h->d_FQe); }));
+  HIPCHK(h, timed(h, fsht::KC_GATHER, [&] { return fsht::launch_gather(h, h->ws_spec, h->ws_F0, h->d_FQe); }));
   // a single unrefined analysis carries the full slice count
   int ka = niter == 0 ? h->mix_out[sp] : lad(0, niter);
-  HIPCHK(h, timed(h, fsht::KC_SLICE, [&] { return fsht::launch_quant_F(h, h->ws_G, ka); }));
+  HIPCHK(h, timed(h, fsht::KC_SLICE, [&] { return fsht::launch_quant_F(h, h->ws_F0, ka); }));
   // each analysis but the last also records alm's column exponents for the
   // next residual synthesis
   if ((rc = mixed_pass(h, 0, a0, a1, 0.0, ka, niter > 0))) return rc;
@@ -568,7 +565,7 @@ static int analysis_iter_mixed(Handle* h, double* a0, double* a1, int niter, int
     if ((rc = mixed_pass(h, 1, nullptr, nullptr, 0.0, ks, 0))) return rc;
     HIPCHK(h, fsht::launch_fexp_reset(h));
     HIPCHK(h, timed(h, fsht::KC_FOLD, [&] {
-      return fsht::launch_gather_prefix(h, h->ws_G, h->ws_specT, h->ws_F, method == 0 ? 1 : 0, h->d_FQe);
+      return fsht::launch_gather_prefix(h, h->ws_G, h->ws_F0, h->ws_F, method == 0 ? 1 : 0, h->d_FQe);
     }));
     ka = lad(2, niter - it - 1);
     HIPCHK(h, timed(h, fsht::KC_SLICE, [&] { return fsht::launch_quant_F(h, h->ws_F, ka); }));
@@ -582,15 +579,15 @@ static int analysis_iter(Handle* h, double* a0, double* a1, int niter, int want0
   // (the F2 contractions add to what the F1 ones wrote)
   if (use_mixed(h) && (h->spin == 0 || (want0 && want1)))
     return analysis_iter_mixed(h, a0, h->spin ? a1 : nullptr, niter, method);
-  HIPCHK(h, timed(h, fsht::KC_GATHER, [&] { return fsht::launch_gather(h, h->ws_specT, h->ws_G); }));
-  HIPCHK(h, timed(h, fsht::KC_ANALYSIS, [&] { return fsht::launch_analysis(h, h->ws_G, a0, a1, 0.0, want0, want1); }));
+  HIPCHK(h, timed(h, fsht::KC_GATHER, [&] { return fsht::launch_gather(h, h->ws_spec, h->ws_F0); }));
+  HIPCHK(h, timed(h, fsht::KC_ANALYSIS, [&] { return fsht::launch_analysis(h, h->ws_F0, a0, a1, 0.0, want0, want1); }));
   if (niter > 0 && !h->ws_F)
     HIPCHK(h, hipMalloc((void**)&h->ws_F, (size_t)(h->G_size / (h->pol ? 1 : 2)) * sizeof(double)));
   for (int it = 0; it < niter; ++it) {
     HIPCHK(h, timed(h, fsht::KC_SYNTHESIS, [&] { return fsht::launch_synthesis(h, a0, a1, h->ws_G); }));
     // residual + gather in one pass (the residual spectrum is never stored)
     HIPCHK(h, timed(h, fsht::KC_FOLD,
-                    [&] { return fsht::launch_gather_prefix(h, h->ws_G, h->ws_specT, h->ws_F, method == 0 ? 1 : 0); }));
+                    [&] { return fsht::launch_gather_prefix(h, h->ws_G, h->ws_F0, h->ws_F, method == 0 ? 1 : 0); }));
     HIPCHK(h, timed(h, fsht::KC_ANALYSIS, [&] { return fsht::launch_analysis(h, h->ws_F, a0, a1, 1.0, want0, want1); }));
   }
   return FSHT_OK;
@@ -664,7 +661,7 @@ int fsht_alm2t(fsht_handle* H, const double* alms, double* maps, int loc) {
   } else {
     HIPCHK(h, timed(h, fsht::KC_SYNTHESIS, [&] { return fsht::launch_synthesis(h, a, nullptr, h->ws_G); }));
   }
-  HIPCHK(h, timed(h, fsht::KC_FOLD, [&] { return fsht::launch_fold(h, h->ws_G, nullptr, h->ws_specT, 0); }));
+  HIPCHK(h, timed(h, fsht::KC_FOLD, [&] { return fsht::launch_fold(h, h->ws_G, h->ws_spec); }));
   if ((rc = inverse_spectra(h, maps, nullptr, loc))) return rc;
   if (!loc) HIPCHK(h, hipStreamSynchronize(h->stream));
   return FSHT_OK;
@@ -735,7 +732,7 @@ int fsht_eb2qu(fsht_handle* H, const double* E, const double* B, double* Q, doub
   } else {
     HIPCHK(h, timed(h, fsht::KC_SYNTHESIS, [&] { return fsht::launch_synthesis(h, e, b, h->ws_G); }));
   }
-  HIPCHK(h, timed(h, fsht::KC_FOLD, [&] { return fsht::launch_fold(h, h->ws_G, nullptr, h->ws_specT, 0); }));
+  HIPCHK(h, timed(h, fsht::KC_FOLD, [&] { return fsht::launch_fold(h, h->ws_G, h->ws_spec); }));
   if ((rc = inverse_spectra(h, Q, U, loc))) return rc;
   if (!loc) HIPCHK(h, hipStreamSynchronize(h->stream));
   return FSHT_OK;

@@ -103,20 +103,20 @@ hipError_t build_bluestein(Handle* h);
 // maps [map0, map0 + nmap) of the comp*nsim + sim index space (nmap <= 0: all)
 hipError_t launch_ring_fft(Handle* h, const double* const* maps_comp, double* spec, int inverse,
                            double* const* out_comp, int map0 = 0, int nmap = 0);
-hipError_t launch_transpose_to_T(Handle* h, const double* spec, double* specT);
-hipError_t launch_transpose_from_T(Handle* h, const double* specT, double* spec);
-// fexp (mixed precision, else NULL): per (m, class, column) the max over rings
-// of the exponent of |F| is folded in with atomicMax (reset to kMixZeroE first)
-hipError_t launch_gather(Handle* h, const double* specT, double* F, int* fexp = nullptr);
-// residual analysis operand straight from the synthesis buffer G: aliases m' <= m
-// (full = 0, immediate update) or all aliases (full = 1, Jacobi)
-hipError_t launch_gather_prefix(Handle* h, const double* G, const double* specT, double* F, int full,
+// map-major ring spectra -> m-major analysis operand.  fexp (mixed precision,
+// else NULL): per (m, class, column) the max over rings of the exponent of |F|
+// is folded in with atomicMax (reset to kMixZeroE first)
+hipError_t launch_gather(Handle* h, const double* spec, double* F, int* fexp = nullptr);
+// residual analysis operand F0 - n * gather(fold(G)) straight from the
+// synthesis buffer G and the first operand F0: aliases m' <= m (full = 0,
+// immediate update) or all aliases (full = 1, Jacobi)
+hipError_t launch_gather_prefix(Handle* h, const double* G, const double* F0, double* F, int full,
                                 int* fexp = nullptr);
 hipError_t launch_analysis(Handle* h, const double* F, double* alm0, double* alm1, double beta,
                            int want0, int want1);
 hipError_t launch_synthesis(Handle* h, const double* alm0, const double* alm1, double* G);
-hipError_t launch_fold(Handle* h, const double* G, const double* specT_t, double* specT_out,
-                       int residual);
+// m-major synthesis output -> map-major ring spectra
+hipError_t launch_fold(Handle* h, const double* G, double* spec);
 // XCD-ordered upload of a work list (setup.hip): items of one m stay on one XCD,
 // m-groups spread over the XCDs by the estimated cost of each item
 hipError_t upload_items(const std::vector<LegItem>& v, const std::vector<double>& cost, LegItem** d, int* n);
@@ -213,7 +213,7 @@ struct Handle {
   // workspace (device)
   int ncomp_ws = 0;            // components the workspace is sized for (1 or 2)
   double* ws_spec = nullptr;   // ncomp*nsim maps of npix (sim-major)
-  double* ws_specT = nullptr;  // npix rows of ncomp*nsimp (sim-fastest)
+  double* ws_F0 = nullptr;     // m-major analysis operand of the input maps (kept for the refinement)
   double* ws_G = nullptr;      // m-major synthesis output
   double* ws_F = nullptr;      // m-major residual analysis operand of the refinement sweeps
   long long G_size = 0;
@@ -320,7 +320,7 @@ __device__ inline bool cols_active(int c0, int c1, int nsim, int nsimp) {
 enum KernelClass {
   KC_FFT_FWD = 0,
   KC_FFT_INV,
-  KC_TRANSPOSE,
+  KC_TRANSPOSE,  // no launches (the gathers and folds read and write the FFT's layout)
   KC_ANALYSIS,
   KC_SYNTHESIS,
   KC_FOLD,

@@ -1,4 +1,4 @@
-// Ring FFTs over HEALPix RING rings and the spectrum transposes.
+// Ring FFTs over HEALPix RING rings.
 //
 // Replaces map2fft / fft2map (src/compute_fft.f90:15-280), which run MKL DFTI
 // on the host ring by ring: forward real FFT, unnormalised, PERM packing
@@ -619,76 +619,6 @@ hipError_t launch_ring_fft(Handle* h, const double* const* in_comp, double* spec
     if (err != hipSuccess) return err;
   }
   return hipSuccess;
-}
-
-// ---------------------------------------------------------------------------
-// Transposes between spec (map-major: [map][pix]) and specT (pixel-major,
-// sims fastest: [pix][comp*nsimp + sim]).  Tiles of TP pixels x TC columns
-// through LDS: TC = 16 columns make every specT row segment one 128-byte line
-// and divide the column count exactly (nsimp is a multiple of 16).
-// ---------------------------------------------------------------------------
-template <int TP, int TC>
-__global__ __launch_bounds__(256) void k_to_T(const double* __restrict__ spec, double* __restrict__ specT,
-                                              long long npix, int nsim, int nsimp, int ncomp) {
-  __shared__ double tile[TC][TP + 1];
-  const long long p0 = (long long)blockIdx.x * TP;
-  const int c0 = blockIdx.y * TC;  // column in [0, ncomp*nsimp)
-  if (!cols_active(c0, c0 + TC, nsim, nsimp)) return;  // no map in the tile
-  const int ld = ncomp * nsimp;
-  for (int e = threadIdx.x; e < TP * TC; e += 256) {
-    const int r = e / TP, x = e % TP;  // column r, pixel x
-    const int col = c0 + r;
-    const int comp = col / nsimp, sim = col - comp * nsimp;
-    const long long p = p0 + x;
-    double v = 0.0;
-    if (comp < ncomp && sim < nsim && p < npix) v = spec[((long long)comp * nsim + sim) * npix + p];
-    tile[r][x] = v;
-  }
-  __syncthreads();
-  for (int e = threadIdx.x; e < TP * TC; e += 256) {
-    const int x = e / TC, r = e % TC;  // pixel x, column r
-    const long long p = p0 + x;
-    const int col = c0 + r;
-    if (p < npix && col < ld) specT[p * ld + col] = tile[r][x];
-  }
-}
-
-template <int TP, int TC>
-__global__ __launch_bounds__(256) void k_from_T(const double* __restrict__ specT, double* __restrict__ spec,
-                                                long long npix, int nsim, int nsimp, int ncomp) {
-  __shared__ double tile[TC][TP + 1];
-  const long long p0 = (long long)blockIdx.x * TP;
-  const int c0 = blockIdx.y * TC;
-  if (!cols_active(c0, c0 + TC, nsim, nsimp)) return;  // no map in the tile
-  const int ld = ncomp * nsimp;
-  for (int e = threadIdx.x; e < TP * TC; e += 256) {
-    const int x = e / TC, r = e % TC;
-    const long long p = p0 + x;
-    const int col = c0 + r;
-    tile[r][x] = (p < npix && col < ld) ? specT[p * ld + col] : 0.0;
-  }
-  __syncthreads();
-  for (int e = threadIdx.x; e < TP * TC; e += 256) {
-    const int r = e / TP, x = e % TP;
-    const int col = c0 + r;
-    const int comp = col / nsimp, sim = col - comp * nsimp;
-    const long long p = p0 + x;
-    if (comp < ncomp && sim < nsim && p < npix) spec[((long long)comp * nsim + sim) * npix + p] = tile[r][x];
-  }
-}
-
-hipError_t launch_transpose_to_T(Handle* h, const double* spec, double* specT) {
-  const int ncomp = h->spin ? 2 : 1;
-  dim3 grid((unsigned)((h->npix + 128 - 1) / 128), (ncomp * h->nsimp + 16 - 1) / 16);
-  hipLaunchKernelGGL((k_to_T<128, 16>), grid, dim3(256), 0, h->stream, spec, specT, h->npix, h->nsim, h->nsimp, ncomp);
-  return hipGetLastError();
-}
-
-hipError_t launch_transpose_from_T(Handle* h, const double* specT, double* spec) {
-  const int ncomp = h->spin ? 2 : 1;
-  dim3 grid((unsigned)((h->npix + 128 - 1) / 128), (ncomp * h->nsimp + 16 - 1) / 16);
-  hipLaunchKernelGGL((k_from_T<128, 16>), grid, dim3(256), 0, h->stream, specT, spec, h->npix, h->nsim, h->nsimp, ncomp);
-  return hipGetLastError();
 }
 
 }  // namespace fsht

@@ -1,18 +1,21 @@
 // m-major synthesis output -> ring spectra (PERM packing), with aliasing.
 //
-// Replaces mat2fft_fast and mat2fft_fast_iter (src/fft_mapping.f90:84-190)
-// plus the zeroing of unused PERM slots (src/sht_main.f90:137-146).  The
-// reference scatters one m at a time and relies on ascending m with
-// set-then-accumulate to fold the orders that alias onto the same ring
-// frequency (rings shorter than 2(lmax+1)).  Here every output slot gathers
-// its aliases explicitly: slot k of a ring of length n receives the orders
-// m = k, n-k, n+k, 2n-k, ... (m <= lmax, ring kept for m), rotated by
-// e^{+i m phi0}, conjugated for negative frequencies and doubled on the DC and
-// Nyquist slots for m > 0 (src/sht_data_init.f90:542-573).
+// Replaces mat2fft_fast (src/fft_mapping.f90:84-190) plus the zeroing of
+// unused PERM slots (src/sht_main.f90:137-146).  The reference scatters one m
+// at a time and relies on ascending m with set-then-accumulate to fold the
+// orders that alias onto the same ring frequency (rings shorter than
+// 2(lmax+1)).  Here every output slot gathers its aliases explicitly: slot k
+// of a ring of length n receives the orders m = k, n-k, n+k, 2n-k, ...
+// (m <= lmax, ring kept for m), rotated by e^{+i m phi0}, conjugated for
+// negative frequencies and doubled on the DC and Nyquist slots for m > 0
+// (src/sht_data_init.f90:542-573).  The spectra are written map-major (the
+// layout the inverse ring FFT reads), so no transpose sits between them.
 //
-// residual = 0: out = X            (alm2t / eb2qu: spectrum of the synthesis)
-// residual = 1: out = Xt - n * X   (Jacobi residual in the ring-Fourier domain,
-//                                   src/fft_mapping.f90:177-183)
+// The gathers read the forward FFT's map-major spectra directly as well, and
+// the Jacobi residual is formed from the first gathered operand F0 =
+// gather(spectrum of the input maps): by linearity
+//   gather(Xt - n * fold(G)) = F0 - n * gather(fold(G)),
+// so the refinement sweeps never touch the spectra again.
 #include "fsht_internal.h"
 
 namespace fsht {
@@ -36,75 +39,88 @@ constexpr int kGatherRings = FSHT_GATHER_RINGS;  // rings per gather workgroup
 template <int NCOMP>
 __global__ __launch_bounds__(256) void k_fold(const RingGeo* __restrict__ geo, const int* __restrict__ i0tab,
                                               const double2* __restrict__ tw, const double* __restrict__ G,
-                                              const long long* __restrict__ goff, const double* __restrict__ Xt,
-                                              double* __restrict__ out, int residual, int lmax, int nhr,
-                                              int nnorth, int nsim, int nsimp) {
-  if (!cols_active(blockIdx.z * kFoldBlk.x, (blockIdx.z + 1) * kFoldBlk.x, nsim, nsimp)) return;  // no map here
+                                              const long long* __restrict__ goff, double* __restrict__ spec,
+                                              int lmax, int nhr, int nnorth, long long npix, int nsim, int nsimp) {
+  if (!cols_active(blockIdx.z * 16, (blockIdx.z + 1) * 16, nsim, nsimp)) return;  // no map here
   const int i = blockIdx.x;  // north ring
-  const int kk = blockIdx.y * kFoldBlk.y + threadIdx.y;
-  const int col = blockIdx.z * kFoldBlk.x + threadIdx.x;  // spectrum column: comp*nsimp + sim
-  const int ldT = NCOMP * nsimp;
-  if (col >= ldT) return;
   const RingGeo g = geo[i];
   const int n = g.nfft;
-  if (kk > n / 2) return;
+  const int k0 = blockIdx.y * 16;
+  if (k0 > n / 2) return;  // whole workgroup
+  __shared__ double tile[4][16][17];  // [value][column][slot]
   const bool pair = g.i1S != g.i1N;
-  const int comp = col / nsimp, sim = col - comp * nsimp;
+  const int ldT = NCOMP * nsimp;
   const int ncols = 2 * NCOMP * nsimp;
-  const int cre = (comp * 2) * nsimp + sim, cim = (comp * 2 + 1) * nsimp + sim;
-  const bool edge = (kk == 0) || (kk == n / 2);
-
-  double XNr = 0.0, XNi = 0.0, XSr = 0.0, XSi = 0.0;
-  // aliases in ascending m: k, n-k, n+k, 2n-k, ...  (edge slots: k, k+n, ...)
-  for (int base = 0;; base += n) {
-    bool stop = false;
-    for (int side = 0; side < 2; ++side) {
-      int m;
-      if (side == 0) {
-        m = base + kk;
-      } else {
-        if (edge) break;
-        m = base + n - kk;
+  // sum role: lane = column (G rows are read 16 columns at a time), then slot
+  {
+    const int kk = k0 + (int)(threadIdx.x >> 4);
+    const int col = blockIdx.z * 16 + (int)(threadIdx.x & 15);  // spectrum column: comp*nsimp + sim
+    const int comp = col / nsimp, sim = col - comp * nsimp;
+    const int cre = (comp * 2) * nsimp + sim, cim = (comp * 2 + 1) * nsimp + sim;
+    const bool edge = (kk == 0) || (kk == n / 2);
+    double XNr = 0.0, XNi = 0.0, XSr = 0.0, XSi = 0.0;
+    // aliases in ascending m: k, n-k, n+k, 2n-k, ...  (edge slots: k, k+n, ...)
+    for (int base = 0; col < ldT && kk <= n / 2; base += n) {
+      bool stop = false;
+      for (int side = 0; side < 2; ++side) {
+        int m;
+        if (side == 0) {
+          m = base + kk;
+        } else {
+          if (edge) break;
+          m = base + n - kk;
+        }
+        if (m > lmax) {
+          stop = true;
+          break;
+        }
+        const int i0 = i0tab[m];
+        if (i < i0) {  // ring not kept for this (and, i0 being monotone, any larger) m
+          stop = true;
+          break;
+        }
+        const int nr = nhr + 1 - i0;
+        const long long gb = goff[m] / (NCOMP == 2 ? 1 : 2) + (long long)(i - i0) * ncols;
+        const long long go = (long long)nr * ncols;  // offset of the odd class
+        const double er = G[gb + cre], ei = G[gb + cim];
+        const double orr = G[gb + go + cre], oi = G[gb + go + cim];
+        const double2 cs = tw[(long long)m * nnorth + i];
+        // north = even + odd, south = even - odd; rotate by e^{+i m phi0}
+        const double nr_ = er + orr, ni_ = ei + oi;
+        const double WNr = nr_ * cs.x - ni_ * cs.y, WNi = nr_ * cs.y + ni_ * cs.x;
+        const double sr_ = er - orr, si_ = ei - oi;
+        const double WSr = sr_ * cs.x - si_ * cs.y, WSi = sr_ * cs.y + si_ * cs.x;
+        if (edge) {
+          const double f = (m == 0) ? 1.0 : 2.0;
+          XNr += f * WNr;
+          XSr += f * WSr;
+        } else if (side == 0) {
+          XNr += WNr;
+          XNi += WNi;
+          XSr += WSr;
+          XSi += WSi;
+        } else {
+          XNr += WNr;
+          XNi -= WNi;
+          XSr += WSr;
+          XSi -= WSi;
+        }
       }
-      if (m > lmax) {
-        stop = true;
-        break;
-      }
-      const int i0 = i0tab[m];
-      if (i < i0) {  // ring not kept for this (and, i0 being monotone, any larger) m
-        stop = true;
-        break;
-      }
-      const int nr = nhr + 1 - i0;
-      const long long gb = goff[m] / (NCOMP == 2 ? 1 : 2) + (long long)(i - i0) * ncols;
-      const long long go = (long long)nr * ncols;  // offset of the odd class
-      const double er = G[gb + cre], ei = G[gb + cim];
-      const double orr = G[gb + go + cre], oi = G[gb + go + cim];
-      const double2 cs = tw[(long long)m * nnorth + i];
-      // north = even + odd, south = even - odd; rotate by e^{+i m phi0}
-      const double nr_ = er + orr, ni_ = ei + oi;
-      const double WNr = nr_ * cs.x - ni_ * cs.y, WNi = nr_ * cs.y + ni_ * cs.x;
-      const double sr_ = er - orr, si_ = ei - oi;
-      const double WSr = sr_ * cs.x - si_ * cs.y, WSi = sr_ * cs.y + si_ * cs.x;
-      if (edge) {
-        const double f = (m == 0) ? 1.0 : 2.0;
-        XNr += f * WNr;
-        XSr += f * WSr;
-      } else if (side == 0) {
-        XNr += WNr;
-        XNi += WNi;
-        XSr += WSr;
-        XSi += WSi;
-      } else {
-        XNr += WNr;
-        XNi -= WNi;
-        XSr += WSr;
-        XSi -= WSi;
-      }
+      if (stop) break;
     }
-    if (stop) break;
+    const int tc = threadIdx.x & 15, tk = threadIdx.x >> 4;
+    tile[0][tc][tk] = XNr;
+    tile[1][tc][tk] = XNi;
+    tile[2][tc][tk] = XSr;
+    tile[3][tc][tk] = XSi;
   }
-  // write PERM slots
+  __syncthreads();
+  // write role: lane = slot (a map's PERM slots are adjacent), then column
+  const int tk = threadIdx.x & 15, tc = threadIdx.x >> 4;
+  const int kk = k0 + tk;
+  const int col = blockIdx.z * 16 + tc;
+  const int comp = col / nsimp, sim = col - comp * nsimp;
+  if (col >= ldT || sim >= nsim || kk > n / 2) return;  // padding column (no map) or past the ring's last slot
   int s0, s1 = -1;
   if (kk == 0)
     s0 = 0;
@@ -114,24 +130,21 @@ __global__ __launch_bounds__(256) void k_fold(const RingGeo* __restrict__ geo, c
     s0 = 2 * kk;
     s1 = 2 * kk + 1;
   }
-  const double nf = (double)n;
-  auto put = [&](long long pix, double v) {
-    const long long a = pix * ldT + col;
-    out[a] = residual ? Xt[a] - nf * v : v;
-  };
-  put(g.i1N + s0, XNr);
-  if (s1 >= 0) put(g.i1N + s1, XNi);
+  double* out = spec + ((long long)comp * nsim + sim) * npix;
+  out[g.i1N + s0] = tile[0][tc][tk];
+  if (s1 >= 0) out[g.i1N + s1] = tile[1][tc][tk];
   if (pair) {
-    put(g.i1S + s0, XSr);
-    if (s1 >= 0) put(g.i1S + s1, XSi);
+    out[g.i1S + s0] = tile[2][tc][tk];
+    if (s1 >= 0) out[g.i1S + s1] = tile[3][tc][tk];
   }
 }
 
 // Mixed precision: the per-(m, class, column) exponent of max |F| over the
 // rings, for the int8 slicing of the analysis operand (k_mixed.hip).  A
-// thread tracks its columns' maxima (Re/Im of both parity classes), the
-// block reduces them over its rings and folds them into fexp (index
-// (2 m + class) * ncols + col) with atomicMax.
+// thread tracks its column's maxima (Re/Im of both parity classes) over its
+// rings and folds them into fexp (index (2 m + class) * ncols + col) with
+// atomicMax; publish_rings first reduces over the workgroup's rings
+// (threadIdx.y) in LDS.
 struct ColMax {
   double mx[4] = {0.0, 0.0, 0.0, 0.0};  // class 0 Re, class 0 Im, class 1 Re, class 1 Im
   __device__ void add(double v0, double v1, double v2, double v3) {
@@ -140,7 +153,14 @@ struct ColMax {
     mx[2] = fmax(mx[2], fabs(v2));
     mx[3] = fmax(mx[3], fabs(v3));
   }
-  __device__ void publish(int* fexp, int m, int ncols, int cre, int cim, bool colok) {
+  __device__ void publish(int* fexp, int m, int ncols, int cre, int cim) const {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int e = mix_exp(mx[k]);
+      if (e != kMixZeroE) atomicMax(&fexp[(long long)(2 * m + (k >> 1)) * ncols + ((k & 1) ? cim : cre)], e);
+    }
+  }
+  __device__ void publish_rings(int* fexp, int m, int ncols, int cre, int cim, bool colok) {
     __shared__ int red[4][kFoldBlk.y][kFoldBlk.x];
 #pragma unroll
     for (int k = 0; k < 4; ++k) red[k][threadIdx.y][threadIdx.x] = mix_exp(mx[k]);
@@ -163,52 +183,87 @@ struct ColMax {
 //   X_S the same on the mirror ring, then the parity folds
 //   F_even = X_N + X_S, F_odd = X_N - X_S   (equator: both = X_N).
 // Layout (same as the synthesis buffer): [m][fold][ring - i0(m)][comp][re/im][sim].
+// The spectra are map-major (as the ring FFT writes them).  Workgroup = 16
+// orders x 16 spectrum columns over the rings [blockIdx.y, blockIdx.y + 1) *
+// kGatherRings; per ring the threads first read with lanes along the orders
+// of one map (adjacent PERM slots: whole cache lines), then, after an LDS
+// transpose, write with lanes along the columns of one order (one 128-byte F
+// line per 16 columns).
 // ---------------------------------------------------------------------------
+constexpr int kGatherM = 16;  // orders per gather workgroup
+
 template <int NCOMP>
 __global__ __launch_bounds__(256) void k_gather(const RingGeo* __restrict__ geo, const int* __restrict__ i0tab,
                                                 const double2* __restrict__ tw, const int* __restrict__ slot,
-                                                const double* __restrict__ specT, double* __restrict__ F,
-                                                const long long* __restrict__ goff, int nhr, int nnorth,
-                                                int nsim, int nsimp, int* __restrict__ fexp) {
-  if (!cols_active(blockIdx.z * kFoldBlk.x, (blockIdx.z + 1) * kFoldBlk.x, nsim, nsimp)) return;  // no map here
-  const int m = blockIdx.x;
-  const int i0 = i0tab[m];
-  const int ldT = NCOMP * nsimp;
-  const int colx = blockIdx.z * kFoldBlk.x + threadIdx.x;  // spectrum column comp*nsimp + sim
-  const bool colok = colx < ldT;
-  const int col = colok ? colx : ldT - 1;
-  const int comp = col / nsimp, sim = col - comp * nsimp;
-  const int nr = nhr + 1 - i0;
-  const int ncols = 2 * NCOMP * nsimp;
-  const long long fb = goff[m] / (NCOMP == 2 ? 1 : 2);
-  const long long odd = (long long)nr * ncols;
-  const int cre = (comp * 2) * nsimp + sim, cim = cre + nsimp;
-  const long long mrow = (long long)m * nnorth;
+                                                const double* __restrict__ spec, double* __restrict__ F,
+                                                const long long* __restrict__ goff, int nhr, int nnorth, int lmax,
+                                                long long npix, int nsim, int nsimp, int* __restrict__ fexp) {
+  if (!cols_active(blockIdx.z * 16, (blockIdx.z + 1) * 16, nsim, nsimp)) return;  // no map here
+  const int m0 = blockIdx.x * kGatherM, c0 = blockIdx.z * 16;
+  // i0 is non-decreasing in m: no order of the block keeps a ring below i0(m0)
+  const int ibeg = max((int)blockIdx.y * kGatherRings, i0tab[m0]);
+  const int iend = min(nhr + 1, ((int)blockIdx.y + 1) * kGatherRings);
+  if (ibeg >= iend) return;  // whole workgroup
+  __shared__ double tile[4][16][kGatherM + 1];  // [value][column][order]
+  const int ldT = NCOMP * nsimp, ncols = 2 * NCOMP * nsimp;
+  // read role: lane = order, then column
+  const int mr = m0 + (int)(threadIdx.x & 15), cr = c0 + (int)(threadIdx.x >> 4);
+  const bool okr = mr <= lmax && cr < ldT;
+  const int compr = cr / nsimp, simr = cr - compr * nsimp;
+  const bool has = okr && simr < nsim;  // padding columns gather zeros
+  const double* sp = spec + ((long long)compr * nsim + (has ? simr : 0)) * npix;
+  const int i0r = i0tab[min(mr, lmax)];
+  const long long mrow = (long long)min(mr, lmax) * nnorth;
+  // write role: lane = column, then order
+  const int mw = m0 + (int)(threadIdx.x >> 4), cw = c0 + (int)(threadIdx.x & 15);
+  const bool okw = mw <= lmax && cw < ldT;
+  const int mwc = min(mw, lmax);
+  const int compw = cw / nsimp, simw = cw - compw * nsimp;
+  const int i0w = i0tab[mwc];
+  const long long fb = goff[mwc] / (NCOMP == 2 ? 1 : 2);
+  const long long odd = (long long)(nhr + 1 - i0w) * ncols;
+  const int cre = (compw * 2) * nsimp + simw, cim = cre + nsimp;
   ColMax cm;
-  for (int i = i0 + blockIdx.y * kGatherRings + threadIdx.y;
-       colok && i <= nhr && i < i0 + ((int)blockIdx.y + 1) * kGatherRings; i += kFoldBlk.y) {
-    const int sinfo = slot[mrow + i];
-    const int rel = sinfo >> 2, cls = sinfo & 3;
-    const double sg = cls == 1 ? 1.0 : (cls == 2 ? -1.0 : 0.0);
-    const double2 cs = tw[mrow + i];
-    const RingGeo g = geo[i];
-    const double* pN = specT + (g.i1N + rel) * (long long)ldT + col;
-    const double* pS = specT + (g.i1S + rel) * (long long)ldT + col;
-    const double xr = pN[0], xi = pN[ldT], yr = pS[0], yi = pS[ldT];
-    const double Nr = xr * cs.x + sg * xi * cs.y;
-    const double Ni = sg * xi * cs.x - xr * cs.y;
-    const double Sr = yr * cs.x + sg * yi * cs.y;
-    const double Si = sg * yi * cs.x - yr * cs.y;
-    const double fs = (g.i1S != g.i1N) ? 1.0 : 0.0;
-    const long long row = fb + (long long)(i - i0) * ncols;
-    const double v0 = fma(fs, Sr, Nr), v1 = fma(fs, Si, Ni), v2 = fma(-fs, Sr, Nr), v3 = fma(-fs, Si, Ni);
-    F[row + cre] = v0;
-    F[row + cim] = v1;
-    F[row + odd + cre] = v2;
-    F[row + odd + cim] = v3;
-    cm.add(v0, v1, v2, v3);
+  for (int i = ibeg; i < iend; ++i) {
+    double v0 = 0.0, v1 = 0.0, v2 = 0.0, v3 = 0.0;
+    if (has && i >= i0r) {
+      const int sinfo = slot[mrow + i];
+      const int rel = sinfo >> 2, cls = sinfo & 3;
+      const double sg = cls == 1 ? 1.0 : (cls == 2 ? -1.0 : 0.0);
+      const double2 cs = tw[mrow + i];
+      const RingGeo g = geo[i];
+      // slot rel and the next one (rel + 1 < n: every ring has at least 4 slots)
+      const double xr = sp[g.i1N + rel], xi = sp[g.i1N + rel + 1];
+      const double yr = sp[g.i1S + rel], yi = sp[g.i1S + rel + 1];
+      const double Nr = xr * cs.x + sg * xi * cs.y;
+      const double Ni = sg * xi * cs.x - xr * cs.y;
+      const double Sr = yr * cs.x + sg * yi * cs.y;
+      const double Si = sg * yi * cs.x - yr * cs.y;
+      const double fs = (g.i1S != g.i1N) ? 1.0 : 0.0;
+      v0 = fma(fs, Sr, Nr);
+      v1 = fma(fs, Si, Ni);
+      v2 = fma(-fs, Sr, Nr);
+      v3 = fma(-fs, Si, Ni);
+    }
+    const int tr = threadIdx.x & 15, tc = threadIdx.x >> 4;
+    tile[0][tc][tr] = v0;
+    tile[1][tc][tr] = v1;
+    tile[2][tc][tr] = v2;
+    tile[3][tc][tr] = v3;
+    __syncthreads();
+    if (okw && i >= i0w) {
+      const int wr = threadIdx.x >> 4, wc = threadIdx.x & 15;
+      const double w0 = tile[0][wc][wr], w1 = tile[1][wc][wr], w2 = tile[2][wc][wr], w3 = tile[3][wc][wr];
+      const long long row = fb + (long long)(i - i0w) * ncols;
+      F[row + cre] = w0;
+      F[row + cim] = w1;
+      F[row + odd + cre] = w2;
+      F[row + odd + cim] = w3;
+      cm.add(w0, w1, w2, w3);
+    }
+    __syncthreads();
   }
-  if (fexp) cm.publish(fexp, m, ncols, cre, cim, colok);
+  if (fexp && okw) cm.publish(fexp, mwc, ncols, cre, cim);
 }
 
 // ---------------------------------------------------------------------------
@@ -220,16 +275,18 @@ __global__ __launch_bounds__(256) void k_gather(const RingGeo* __restrict__ geo,
 //   R(m, i) = Xt(slot) - n * sum_{m' <= m, m' ~ slot} X_{m'}(i)
 // (every synthesis uses the alm from before the sweep).  One thread computes
 // that prefix for its (m, ring, column) from the synthesis buffer G, then
-// applies the gather of k_gather.  G and F are distinct buffers.
+// applies the gather of k_gather, as F0 - n * gather(sum) with F0 =
+// gather(Xt) written by k_gather.  G, F0 and F are distinct buffers.
 //
 // full = 1 sums the whole alias chain instead (every m' <= lmax whose ring
-// set includes ring i): the Jacobi residual of t2alm_iter_old, i.e. k_fold's
-// residual followed by k_gather, without writing the residual spectrum.
+// set includes ring i): the Jacobi residual of t2alm_iter_old
+// (src/fft_mapping.f90:177-183 followed by fft2mat_fast), without forming the
+// residual spectrum.
 // ---------------------------------------------------------------------------
 template <int NCOMP>
 __global__ __launch_bounds__(256) void k_gather_prefix(const RingGeo* __restrict__ geo, const int* __restrict__ i0tab,
                                                        const double2* __restrict__ tw, const int* __restrict__ slot,
-                                                       const double* __restrict__ G, const double* __restrict__ specT,
+                                                       const double* __restrict__ G, const double* __restrict__ F0,
                                                        double* __restrict__ F, const long long* __restrict__ goff,
                                                        int nhr, int nnorth, int nsim, int nsimp, int lmax, int full,
                                                        int* __restrict__ fexp) {
@@ -305,74 +362,70 @@ __global__ __launch_bounds__(256) void k_gather_prefix(const RingGeo* __restrict
         }
       }
     }
-    const int sinfo = slot[mrow + i];
-    const int rel = sinfo >> 2, cls = sinfo & 3;
+    // gather of the synthesised alias sum (k_gather's arithmetic; the Im
+    // slot does not exist on DC / Nyquist, where sg = 0)
+    const int cls = slot[mrow + i] & 3;
     const double sg = cls == 1 ? 1.0 : (cls == 2 ? -1.0 : 0.0);
     const double nf = (double)n;
-    const bool pair = g.i1S != g.i1N;
-    const double* pN = specT + (g.i1N + rel) * (long long)ldT + col;
-    const double* pS = specT + (g.i1S + rel) * (long long)ldT + col;
-    // residual slot pair (the Im slot does not exist on DC / Nyquist: sg = 0 there)
-    const double xr = pN[0] - nf * XNr;
-    const double xi = edge ? 0.0 : pN[ldT] - nf * XNi;
-    const double yr = pair ? pS[0] - nf * XSr : 0.0;
-    const double yi = (pair && !edge) ? pS[ldT] - nf * XSi : 0.0;
     const double2 cs = tw[mrow + i];
-    const double Nr = xr * cs.x + sg * xi * cs.y;
-    const double Ni = sg * xi * cs.x - xr * cs.y;
-    const double Sr = yr * cs.x + sg * yi * cs.y;
-    const double Si = sg * yi * cs.x - yr * cs.y;
-    const double fs = pair ? 1.0 : 0.0;
+    const double Nr = XNr * cs.x + sg * XNi * cs.y;
+    const double Ni = sg * XNi * cs.x - XNr * cs.y;
+    const double Sr = XSr * cs.x + sg * XSi * cs.y;
+    const double Si = sg * XSi * cs.x - XSr * cs.y;
+    const double fs = (g.i1S != g.i1N) ? 1.0 : 0.0;
     const long long row = fb + (long long)(i - i0) * ncols;
-    const double v0 = fma(fs, Sr, Nr), v1 = fma(fs, Si, Ni), v2 = fma(-fs, Sr, Nr), v3 = fma(-fs, Si, Ni);
+    const double v0 = fma(-nf, fma(fs, Sr, Nr), F0[row + cre]);
+    const double v1 = fma(-nf, fma(fs, Si, Ni), F0[row + cim]);
+    const double v2 = fma(-nf, fma(-fs, Sr, Nr), F0[row + odd + cre]);
+    const double v3 = fma(-nf, fma(-fs, Si, Ni), F0[row + odd + cim]);
     F[row + cre] = v0;
     F[row + cim] = v1;
     F[row + odd + cre] = v2;
     F[row + odd + cim] = v3;
     cm.add(v0, v1, v2, v3);
   }
-  if (fexp) cm.publish(fexp, m, ncols, cre, cim, colok);
+  if (fexp) cm.publish_rings(fexp, m, ncols, cre, cim, colok);
 }
 
-hipError_t launch_gather_prefix(Handle* h, const double* G, const double* specT, double* F, int full, int* fexp) {
+hipError_t launch_gather_prefix(Handle* h, const double* G, const double* F0, double* F, int full, int* fexp) {
   const int ncomp = h->spin ? 2 : 1;
   dim3 block(kFoldBlk.x, kFoldBlk.y);
   dim3 grid(h->lmax + 1, (h->nnorth + kGatherRings - 1) / kGatherRings,
             (ncomp * h->nsimp + kFoldBlk.x - 1) / kFoldBlk.x);
   if (ncomp == 2)
-    hipLaunchKernelGGL(k_gather_prefix<2>, grid, block, 0, h->stream, h->d_geo, h->d_i0, h->d_tw, h->d_slot, G,
-                       specT, F, h->d_goff, h->nhr, h->nnorth, h->nsim, h->nsimp, h->lmax, full, fexp);
+    hipLaunchKernelGGL(k_gather_prefix<2>, grid, block, 0, h->stream, h->d_geo, h->d_i0, h->d_tw, h->d_slot, G, F0,
+                       F, h->d_goff, h->nhr, h->nnorth, h->nsim, h->nsimp, h->lmax, full, fexp);
   else
-    hipLaunchKernelGGL(k_gather_prefix<1>, grid, block, 0, h->stream, h->d_geo, h->d_i0, h->d_tw, h->d_slot, G,
-                       specT, F, h->d_goff, h->nhr, h->nnorth, h->nsim, h->nsimp, h->lmax, full, fexp);
+    hipLaunchKernelGGL(k_gather_prefix<1>, grid, block, 0, h->stream, h->d_geo, h->d_i0, h->d_tw, h->d_slot, G, F0,
+                       F, h->d_goff, h->nhr, h->nnorth, h->nsim, h->nsimp, h->lmax, full, fexp);
   return hipGetLastError();
 }
 
-hipError_t launch_gather(Handle* h, const double* specT, double* F, int* fexp) {
+hipError_t launch_gather(Handle* h, const double* spec, double* F, int* fexp) {
   const int ncomp = h->spin ? 2 : 1;
-  dim3 block(kFoldBlk.x, kFoldBlk.y);
-  dim3 grid(h->lmax + 1, (h->nnorth + kGatherRings - 1) / kGatherRings,
-            (ncomp * h->nsimp + kFoldBlk.x - 1) / kFoldBlk.x);
+  dim3 block(256);
+  dim3 grid((h->lmax + kGatherM) / kGatherM, (h->nnorth + kGatherRings - 1) / kGatherRings,
+            (ncomp * h->nsimp + 15) / 16);
   if (ncomp == 2)
-    hipLaunchKernelGGL(k_gather<2>, grid, block, 0, h->stream, h->d_geo, h->d_i0, h->d_tw, h->d_slot, specT, F,
-                       h->d_goff, h->nhr, h->nnorth, h->nsim, h->nsimp, fexp);
+    hipLaunchKernelGGL(k_gather<2>, grid, block, 0, h->stream, h->d_geo, h->d_i0, h->d_tw, h->d_slot, spec, F,
+                       h->d_goff, h->nhr, h->nnorth, h->lmax, h->npix, h->nsim, h->nsimp, fexp);
   else
-    hipLaunchKernelGGL(k_gather<1>, grid, block, 0, h->stream, h->d_geo, h->d_i0, h->d_tw, h->d_slot, specT, F,
-                       h->d_goff, h->nhr, h->nnorth, h->nsim, h->nsimp, fexp);
+    hipLaunchKernelGGL(k_gather<1>, grid, block, 0, h->stream, h->d_geo, h->d_i0, h->d_tw, h->d_slot, spec, F,
+                       h->d_goff, h->nhr, h->nnorth, h->lmax, h->npix, h->nsim, h->nsimp, fexp);
   return hipGetLastError();
 }
 
-hipError_t launch_fold(Handle* h, const double* G, const double* specT_t, double* specT_out, int residual) {
+hipError_t launch_fold(Handle* h, const double* G, double* spec) {
   const int ncomp = h->spin ? 2 : 1;
   const int maxk = 2 * h->nside;  // n/2 for the longest ring
-  dim3 block(kFoldBlk.x, kFoldBlk.y);
-  dim3 grid(h->nnorth, (maxk + 1 + kFoldBlk.y - 1) / kFoldBlk.y, (ncomp * h->nsimp + kFoldBlk.x - 1) / kFoldBlk.x);
+  dim3 block(256);
+  dim3 grid(h->nnorth, (maxk + 1 + 15) / 16, (ncomp * h->nsimp + 15) / 16);
   if (ncomp == 2)
-    hipLaunchKernelGGL(k_fold<2>, grid, block, 0, h->stream, h->d_geo, h->d_i0, h->d_tw, G, h->d_goff, specT_t,
-                       specT_out, residual, h->lmax, h->nhr, h->nnorth, h->nsim, h->nsimp);
+    hipLaunchKernelGGL(k_fold<2>, grid, block, 0, h->stream, h->d_geo, h->d_i0, h->d_tw, G, h->d_goff, spec,
+                       h->lmax, h->nhr, h->nnorth, h->npix, h->nsim, h->nsimp);
   else
-    hipLaunchKernelGGL(k_fold<1>, grid, block, 0, h->stream, h->d_geo, h->d_i0, h->d_tw, G, h->d_goff, specT_t,
-                       specT_out, residual, h->lmax, h->nhr, h->nnorth, h->nsim, h->nsimp);
+    hipLaunchKernelGGL(k_fold<1>, grid, block, 0, h->stream, h->d_geo, h->d_i0, h->d_tw, G, h->d_goff, spec,
+                       h->lmax, h->nhr, h->nnorth, h->npix, h->nsim, h->nsimp);
   return hipGetLastError();
 }
 

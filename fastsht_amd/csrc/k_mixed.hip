@@ -1145,7 +1145,8 @@ static void plan_offsets(const Handle* h, MixPlan& P) {
 
 // The analysis operand FQ lives in the map-major spectrum buffer when it
 // fits there: that buffer only holds data between a transform's ring FFTs
-// and its transposes, while FQ only lives between a gather and the analysis.
+// and its first gather (analysis) or between the fold and the inverse FFTs
+// (synthesis), while FQ only lives between a gather and the analysis.
 static bool fq_fits_spec(const Handle* h, long long fq_bytes) {
   return fq_bytes <= (long long)h->ncomp_ws * h->nsim_cap * h->npix * 8;
 }

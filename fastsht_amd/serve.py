@@ -289,6 +289,7 @@ class Server:
     evictions: int = 0
     lane_factory: object = Lane  # (server, nside, lmax, capacity) -> lane
     device_events: bool = True   # record an input-ready event at submission
+    known: dict = field(default_factory=dict)  # (nside, lmax) -> bytes of a lane of that shape, held before
 
     def clock(self):
         return time.perf_counter() - self._t0
@@ -300,6 +301,16 @@ class Server:
         while cap > 16 and estimate_lane_bytes(nside, lmax, cap) > self.hbm_budget / 4:
             cap //= 2  # (below 16 maps the padded workspace no longer shrinks)
         return cap
+
+    def reserve(self, key, mixed, mpol):
+        """Bytes a new lane of shape key = (nside, lmax) is admitted against:
+        what a lane of that shape held before (its measured allocation and
+        the library's planned mixed-precision bytes), else the upper
+        estimate."""
+        if key not in self.known:
+            return estimate_lane_bytes(*key, self.cap_for(*key), mixed, mpol)
+        base, mix, pol = self.known[key]
+        return base + (mix if mixed or mpol else 0) + (pol if mpol else 0)
 
     def lane_bytes(self):
         return sum(l.footprint for l in self.lanes.values())
@@ -327,7 +338,7 @@ class Server:
             mixed = r.precision_used == "mixed"
             mpol = mixed and r.pol
             if lane is None:
-                need += estimate_lane_bytes(r.nside, r.lmax, self.cap_for(*key), mixed, mpol)
+                need += self.reserve(key, mixed, mpol)
             elif (mixed and not lane.mixed_ready) or (mpol and not lane.mixed_pol_ready):
                 need += (lane.mixed_bytes(mixed or lane.mixed_ready, mpol or lane.mixed_pol_ready)
                          - lane.mixed_bytes(lane.mixed_ready, lane.mixed_pol_ready))
@@ -340,6 +351,8 @@ class Server:
             if lane is None:
                 lane = self.lane_factory(self, r.nside, r.lmax, self.cap_for(*key))
                 self.lanes[key] = lane
+                mix = lane.mixed_bytes(True, False)
+                self.known[key] = (lane.footprint, mix, lane.mixed_bytes(True, True) - mix)
             if r.precision_used == "mixed":
                 lane.enable_mixed(pol=r.pol)
             self.held += r.nbytes()

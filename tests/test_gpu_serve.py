@@ -26,9 +26,9 @@ def rel(a, b):
     return float((a - b).abs().max() / b.abs().max())
 
 
-def make_request(torch, rng, nside, direction, pol, nsim, precision):
+def make_request(torch, rng, nside, direction, pol, nsim, precision, lmax=None):
     from fastsht_amd.serve import Request
-    lmax = 2 * nside
+    lmax = 2 * nside if lmax is None else lmax
     L, npix = lmax + 1, 12 * nside * nside
     ncomp = 3 if pol else 1
     if direction == "forward":
@@ -107,20 +107,27 @@ def test_mix_invariance(torch_cuda):
 
 def test_split_requests_and_eviction(torch_cuda):
     """Capacity 8 splits larger requests over batches; a budget that holds
-    one lane at a time closes idle lanes for new shapes."""
+    one lane at a time closes idle lanes for new shapes.  Two shapes of one
+    nside (lmax 128 and 95) make lanes of similar size, each larger than any
+    request; a closed lane is re-admitted against the bytes it held."""
     torch = torch_cuda
     from fastsht_amd.serve import Server
     rng = np.random.default_rng(9)
     srv = Server(hbm_budget=100e9, niter=2, capacity=lambda nside, lmax: 8)
     first = srv.submit(make_request(torch, rng, 64, "forward", True, 10, "mixed"))
+    warm = srv.submit(make_request(torch, rng, 64, "forward", True, 2, "mixed", lmax=95))
     srv.run_until_idle()
-    # room for the nside-64 lane and its largest request, not for a second lane
-    srv.hbm_budget = 1.2 * srv.lane_bytes() + 2 * first.nbytes()
-    shapes = [(32, "forward", False, 19, "fp64"), (64, "forward", True, 10, "mixed"), (32, "backward", True, 13, "fp64"),
-              (64, "backward", False, 9, "fp64")]
-    reqs = [srv.submit(make_request(torch, rng, *s)) for s in shapes]
+    shapes = [(64, "forward", False, 19, "fp64", 95), (64, "forward", True, 10, "mixed"),
+              (64, "backward", True, 13, "fp64", 95), (64, "backward", False, 9, "fp64")]
+    reqs = [make_request(torch, rng, *s) for s in shapes]
+    # room for either lane as held and the largest request, not for both lanes
+    big = max(r.nbytes() for r in reqs)
+    srv.hbm_budget = max(l.footprint for l in srv.lanes.values()) + big + 1e6
+    assert srv.lane_bytes() > srv.hbm_budget, (srv.lane_bytes(), srv.hbm_budget)
+    for r in reqs:
+        srv.submit(r)
     print(f"case serve split/evict: budget {srv.hbm_budget / 1e9:.3f} GB", flush=True)
     srv.run_until_idle()
-    assert srv.evictions >= 1, srv.evictions
-    check(torch, [first] + reqs, 2)
+    assert srv.evictions >= 2, srv.evictions
+    check(torch, [first, warm] + reqs, 2)
     srv.close()

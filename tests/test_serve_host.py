@@ -132,3 +132,30 @@ def test_spin2_buffers_charged_on_first_polarised_mixed_request():
     srv.submit(req(32, 2, pol=True, precision="mixed"))
     srv.run_until_idle()
     assert lane.footprint == pytest.approx(full)  # charged once
+
+
+def test_recreated_lane_reserves_what_it_held():
+    """A lane closed for memory is re-admitted against the bytes it held
+    (here half its estimate), not against the estimate."""
+    class HalfLane(FakeLane):
+        def __init__(self, server, nside, lmax, cap):
+            super().__init__(server, nside, lmax, cap)
+            self.footprint = 0.5 * estimate_lane_bytes(nside, lmax, cap)
+
+    est16, est32 = estimate_lane_bytes(16, 32, 4), estimate_lane_bytes(32, 64, 4)
+    srv = Server(lane_factory=HalfLane, device_events=False, capacity=lambda ns, lm: 4, hbm_budget=1e13)
+    srv.submit(req(16, 3))
+    srv.submit(req(32, 3))
+    srv.run_until_idle()
+    assert srv.reserve((32, 64), False, False) == pytest.approx(0.5 * est32)
+    assert srv.reserve((64, 128), False, False) == pytest.approx(estimate_lane_bytes(64, 128, 4))
+    # room for either lane as held, not for both, and less than the nside-32 estimate
+    srv.hbm_budget = 0.5 * est32 + 0.5 * est16 - 1.0
+    assert srv.hbm_budget < est32
+    srv.submit(req(32, 2))
+    srv.run_until_idle()
+    assert srv.evictions == 1 and list(srv.lanes) == [(32, 64)]
+    srv.submit(req(16, 2))
+    srv.submit(req(32, 2))
+    done = srv.run_until_idle()
+    assert len(done) == 2 and srv.evictions == 3
