@@ -208,176 +208,165 @@ __device__ inline double2 cadd(double2 a, double2 b) { return make_double2(a.x +
 __device__ inline double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
 
 // The transform length of an LDS class is a compile-time constant (every job
-// of the class has M = the class size), so each pass is unrolled and issues
-// all its twiddle loads up front.  Passes are radix 4 (two radix-2 stages
-// fused, half the LDS round trips and barriers), with one radix-2 stage when
-// log2(M) is odd.  FftShape<M>::TPB threads (512 for the long transforms,
-// whose LDS footprint leaves one workgroup per CU); tw[k] = exp(-2 pi i k /
-// Mmax) for k < Mmax.
+// of the class has M = the class size), so the passes are unrolled.  A pass
+// applies R consecutive radix-2 stages (R = kFftR while that many remain,
+// then the rest) to groups of 2^R elements held in registers: one LDS read
+// and one LDS write per element for R stages.  The stages are those of the
+// radix-2 transform, so the forward (DIF) output is in bit-reversed order and
+// the inverse (DIT) takes bit-reversed input.  tw[k] = exp(-2 pi i k / Mmax)
+// for k < Mmax.
 
 __device__ inline double2 mul_mi(double2 a) { return make_double2(a.y, -a.x); }  // a * (-i)
 __device__ inline double2 mul_pi(double2 a) { return make_double2(-a.y, a.x); }  // a * (+i)
 
+// Stages per pass: 2 (radix 4).  Radix-8 and radix-16 passes (3 and 4) halve
+// the LDS round trips but run slower on gfx950 (4096-point class, 200 maps:
+// 21-24 against 18.6 ms per direction): their register groups cut the waves
+// per SIMD, and with two 64 KB workgroups per CU the transform is bound by
+// latency, not by LDS bandwidth.
+#ifndef FSHT_FFT_RMAX
+#define FSHT_FFT_RMAX 2
+#endif
+#ifndef FSHT_FFT_TPB4096
+#define FSHT_FFT_TPB4096 512
+#endif
+#ifndef FSHT_FFT_TPB8192
+#define FSHT_FFT_TPB8192 1024
+#endif
+constexpr int kFftR = FSHT_FFT_RMAX;  // radix-2 stages per pass (at most)
+
 template <int M>
 struct FftShape {
   static constexpr int LG = (M >= 2) ? __builtin_ctz(M) : 0;
-  static constexpr bool ODD = (LG & 1) != 0;
-  static constexpr int TPB = (M >= 8192) ? 1024 : ((M >= 2048) ? 512 : 256);  // threads per workgroup
-  static constexpr int NB2 = (M / 2 + TPB - 1) / TPB;    // radix-2 butterflies per thread
-  static constexpr int NB4 = (M / 4 + TPB - 1) / TPB;    // radix-4 butterflies per thread
+  // threads per workgroup (the long transforms' LDS footprint leaves one or
+  // two workgroups per CU)
+  static constexpr int TPB = (M >= 8192) ? FSHT_FFT_TPB8192 : ((M >= 2048) ? FSHT_FFT_TPB4096 : 256);
+  static constexpr int MINB = (M >= 8192) ? 1 : 2;  // workgroups per CU the registers must allow
 };
 
-// DIF radix-2 stage of span len (natural order positions)
-template <int M>
-__device__ inline void dif2(double2* buf, int len, int Mmax, const double2* __restrict__ tw) {
-  constexpr int NB = FftShape<M>::NB2;
-  const int half = len >> 1, stride = Mmax / len;
-  double2 w[NB];
-#pragma unroll
-  for (int r = 0; r < NB; ++r) {
-    const int t = threadIdx.x + FftShape<M>::TPB * r;
-    w[r] = (t < M / 2) ? tw[(t & (half - 1)) * stride] : make_double2(1.0, 0.0);
+// exp(-2 pi i j / (2h)) for j < h <= 8 (the stage roots inside a pass)
+__device__ inline double2 stage_root(int h, int j) {
+  constexpr double c1 = 0.92387953251128675613, s1 = 0.38268343236508977173;  // cos, sin of pi/8
+  constexpr double r2 = 0.70710678118654752440;                                // cos pi/4
+  if (j == 0) return make_double2(1.0, 0.0);
+  const int e = j * (8 / h);  // exponent in units of pi/8
+  switch (e) {
+    case 1: return make_double2(c1, -s1);
+    case 2: return make_double2(r2, -r2);
+    case 3: return make_double2(s1, -c1);
+    case 4: return make_double2(0.0, -1.0);
+    case 5: return make_double2(-s1, -c1);
+    case 6: return make_double2(-r2, -r2);
+    default: return make_double2(-c1, -s1);
   }
+}
+
+// DIF pass: stages of span LEN, LEN/2, ..., LEN/2^(R-1) on the groups
+// {b*LEN + p + k*Q : k < 2^R}, Q = LEN / 2^R.  Stage t pairs k and k + h
+// (h = 2^(R-1-t), bit h of k clear) with the twiddle W_S^(p + (k mod h) Q)
+// = W_S^p * exp(-2 pi i (k mod h) / (2h)), S = LEN / 2^t, W_S^p = (W_LEN^p)^(2^t).
+template <int M, int R, int LEN>
+__device__ inline void dif_pass(double2* buf, int Mmax, const double2* __restrict__ tw) {
+  constexpr int NR = 1 << R, Q = LEN / NR, G = M / NR, TPB = FftShape<M>::TPB, NG = (G + TPB - 1) / TPB;
+  const int stride = Mmax / LEN;
 #pragma unroll
-  for (int r = 0; r < NB; ++r) {
-    const int t = threadIdx.x + FftShape<M>::TPB * r;
-    if (t < M / 2) {
-      const int pos = t & (half - 1);
-      const int i = (t - pos) * 2 + pos;
-      const double2 u = buf[pz(i)], v = buf[pz(i + half)];
-      buf[pz(i)] = cadd(u, v);
-      buf[pz(i + half)] = cmul(csub(u, v), w[r]);
+  for (int r = 0; r < NG; ++r) {
+    const int g = threadIdx.x + TPB * r;
+    if (G % TPB == 0 || g < G) {
+      const int p = g & (Q - 1), i0 = (g - p) * NR + p;
+      double2 x[NR];
+#pragma unroll
+      for (int k = 0; k < NR; ++k) x[k] = buf[pz(i0 + k * Q)];
+      double2 wb = tw[p * stride];
+#pragma unroll
+      for (int t = 0; t < R; ++t) {
+        const int h = NR >> (t + 1);
+#pragma unroll
+        for (int k = 0; k < NR; ++k) {
+          if (k & h) continue;
+          const int j = k & (h - 1);
+          const double2 w = (j == 0) ? wb : cmul(wb, stage_root(h, j));
+          const double2 u = x[k], v = x[k + h];
+          x[k] = cadd(u, v);
+          x[k + h] = cmul(csub(u, v), w);
+        }
+        if (t + 1 < R) wb = cmul(wb, wb);
+      }
+#pragma unroll
+      for (int k = 0; k < NR; ++k) buf[pz(i0 + k * Q)] = x[k];
     }
   }
   __syncthreads();
 }
 
-// DIF radix-4 pass = stages of span len and len/2
-template <int M>
-__device__ inline void dif4(double2* buf, int len, int Mmax, const double2* __restrict__ tw) {
-  constexpr int NB = FftShape<M>::NB4;
-  const int q = len >> 2, stride = Mmax / len;
-  double2 w1[NB], w2[NB], w3[NB];
+// DIT pass (inverse twiddles): stages of span 2Q, 4Q, ..., LEN on the same
+// groups.  Stage t pairs k and k + h (h = 2^t) after multiplying element
+// k + h by conj(W_S^(p + (k mod h) Q)), S = 2hQ, W_S^p = (W_LEN^p)^(2^(R-1-t)).
+template <int M, int R, int LEN>
+__device__ inline void dit_pass(double2* buf, int Mmax, const double2* __restrict__ tw) {
+  constexpr int NR = 1 << R, Q = LEN / NR, G = M / NR, TPB = FftShape<M>::TPB, NG = (G + TPB - 1) / TPB;
+  const int stride = Mmax / LEN;
 #pragma unroll
-  for (int r = 0; r < NB; ++r) {
-    const int t = threadIdx.x + FftShape<M>::TPB * r;
-    const int pos = t & (q - 1);
-    const bool ok = t < M / 4;
-    // one coalesced table load; w^2 and w^3 in registers (the table reads at
-    // 2*pos and 3*pos would touch two and three times the cache lines)
-    w1[r] = ok ? tw[pos * stride] : make_double2(1.0, 0.0);
-    w2[r] = cmul(w1[r], w1[r]);
-    w3[r] = cmul(w2[r], w1[r]);
-  }
+  for (int r = 0; r < NG; ++r) {
+    const int g = threadIdx.x + TPB * r;
+    if (G % TPB == 0 || g < G) {
+      const int p = g & (Q - 1), i0 = (g - p) * NR + p;
+      double2 x[NR];
 #pragma unroll
-  for (int r = 0; r < NB; ++r) {
-    const int t = threadIdx.x + FftShape<M>::TPB * r;
-    if (t < M / 4) {
-      const int pos = t & (q - 1);
-      const int i0 = (t - pos) * 4 + pos;
-      const double2 a0 = buf[pz(i0)], a1 = buf[pz(i0 + q)], a2 = buf[pz(i0 + 2 * q)], a3 = buf[pz(i0 + 3 * q)];
-      const double2 s02 = cadd(a0, a2), d02 = csub(a0, a2);
-      const double2 s13 = cadd(a1, a3), d13 = mul_mi(csub(a1, a3));
-      buf[pz(i0)] = cadd(s02, s13);
-      buf[pz(i0 + q)] = cmul(csub(s02, s13), w2[r]);
-      buf[pz(i0 + 2 * q)] = cmul(cadd(d02, d13), w1[r]);
-      buf[pz(i0 + 3 * q)] = cmul(csub(d02, d13), w3[r]);
+      for (int k = 0; k < NR; ++k) x[k] = buf[pz(i0 + k * Q)];
+      double2 wp[R];
+      wp[R - 1] = cconj(tw[p * stride]);
+#pragma unroll
+      for (int t = R - 2; t >= 0; --t) wp[t] = cmul(wp[t + 1], wp[t + 1]);
+#pragma unroll
+      for (int t = 0; t < R; ++t) {
+        const int h = 1 << t;
+#pragma unroll
+        for (int k = 0; k < NR; ++k) {
+          if (k & h) continue;
+          const int j = k & (h - 1);
+          const double2 w = (j == 0) ? wp[t] : cmul(wp[t], cconj(stage_root(h, j)));
+          const double2 u = x[k], v = cmul(x[k + h], w);
+          x[k] = cadd(u, v);
+          x[k + h] = csub(u, v);
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < NR; ++k) buf[pz(i0 + k * Q)] = x[k];
     }
   }
   __syncthreads();
+}
+
+// Passes of kFftR stages while that many remain, then the rest.
+template <int M, int LEN>
+__device__ inline void dif_from(double2* buf, int Mmax, const double2* __restrict__ tw) {
+  if constexpr (LEN >= 2) {
+    constexpr int LGL = __builtin_ctz(LEN), R = LGL < kFftR ? LGL : kFftR;
+    dif_pass<M, R, LEN>(buf, Mmax, tw);
+    dif_from<M, (LEN >> R)>(buf, Mmax, tw);
+  }
+}
+// The same passes in reverse order (smallest span first).
+template <int M, int LEN>
+__device__ inline void dit_upto(double2* buf, int Mmax, const double2* __restrict__ tw) {
+  if constexpr (LEN >= 2) {
+    constexpr int LGL = __builtin_ctz(LEN), R = LGL < kFftR ? LGL : kFftR;
+    dit_upto<M, (LEN >> R)>(buf, Mmax, tw);
+    dit_pass<M, R, LEN>(buf, Mmax, tw);
+  }
 }
 
 // Decimation in frequency, forward (e^{-2 pi i jk/M}): natural in, bit-reversed out.
 template <int M>
 __device__ void lds_dif(double2* buf, int Mmax, const double2* __restrict__ tw) {
-  int len = M;
-  if (FftShape<M>::ODD) {
-    dif2<M>(buf, len, Mmax, tw);
-    len >>= 1;
-  }
-#pragma unroll
-  for (int k = 0; k < FftShape<M>::LG / 2; ++k) {
-    dif4<M>(buf, len, Mmax, tw);
-    len >>= 2;
-  }
-}
-
-// DIT radix-2 stage of span len, inverse twiddles
-template <int M>
-__device__ inline void dit2(double2* buf, int len, int Mmax, const double2* __restrict__ tw) {
-  constexpr int NB = FftShape<M>::NB2;
-  const int half = len >> 1, stride = Mmax / len;
-  double2 w[NB];
-#pragma unroll
-  for (int r = 0; r < NB; ++r) {
-    const int t = threadIdx.x + FftShape<M>::TPB * r;
-    w[r] = (t < M / 2) ? cconj(tw[(t & (half - 1)) * stride]) : make_double2(1.0, 0.0);
-  }
-#pragma unroll
-  for (int r = 0; r < NB; ++r) {
-    const int t = threadIdx.x + FftShape<M>::TPB * r;
-    if (t < M / 2) {
-      const int pos = t & (half - 1);
-      const int i = (t - pos) * 2 + pos;
-      const double2 u = buf[pz(i)];
-      const double2 v = cmul(buf[pz(i + half)], w[r]);
-      buf[pz(i)] = cadd(u, v);
-      buf[pz(i + half)] = csub(u, v);
-    }
-  }
-  __syncthreads();
-}
-
-// DIT radix-4 pass = stages of span len/2 then len (inverse twiddles)
-template <int M>
-__device__ inline void dit4(double2* buf, int len, int Mmax, const double2* __restrict__ tw) {
-  constexpr int NB = FftShape<M>::NB4;
-  const int q = len >> 2, stride = Mmax / len;
-  double2 w1[NB], w2[NB], w3[NB];
-#pragma unroll
-  for (int r = 0; r < NB; ++r) {
-    const int t = threadIdx.x + FftShape<M>::TPB * r;
-    const int pos = t & (q - 1);
-    const bool ok = t < M / 4;
-    w1[r] = ok ? cconj(tw[pos * stride]) : make_double2(1.0, 0.0);
-    w2[r] = cmul(w1[r], w1[r]);
-    w3[r] = cmul(w2[r], w1[r]);
-  }
-#pragma unroll
-  for (int r = 0; r < NB; ++r) {
-    const int t = threadIdx.x + FftShape<M>::TPB * r;
-    if (t < M / 4) {
-      const int pos = t & (q - 1);
-      const int i0 = (t - pos) * 4 + pos;
-      const double2 a0 = buf[pz(i0)], a1 = cmul(buf[pz(i0 + q)], w2[r]);
-      const double2 a2 = cmul(buf[pz(i0 + 2 * q)], w1[r]), a3 = cmul(buf[pz(i0 + 3 * q)], w3[r]);
-      // span len/2: (a0, a1) and (a2, a3); span len: pairs (0, 2) and (1, 3)
-      const double2 b0 = cadd(a0, a1), b1 = csub(a0, a1);
-      const double2 b2 = cadd(a2, a3), b3 = mul_pi(csub(a2, a3));
-      buf[pz(i0)] = cadd(b0, b2);
-      buf[pz(i0 + q)] = cadd(b1, b3);
-      buf[pz(i0 + 2 * q)] = csub(b0, b2);
-      buf[pz(i0 + 3 * q)] = csub(b1, b3);
-    }
-  }
-  __syncthreads();
+  dif_from<M, M>(buf, Mmax, tw);
 }
 
 // Decimation in time, inverse (e^{+2 pi i jk/M}, unnormalised): bit-reversed in, natural out.
 template <int M>
 __device__ void lds_dit_inv(double2* buf, int Mmax, const double2* __restrict__ tw) {
-  int len = 2;
-  if (FftShape<M>::ODD) {
-    dit2<M>(buf, len, Mmax, tw);
-    len = 8;
-  } else {
-    len = 4;
-  }
-#pragma unroll
-  for (int k = 0; k < FftShape<M>::LG / 2; ++k) {
-    dit4<M>(buf, len, Mmax, tw);
-    len <<= 2;
-  }
+  dit_upto<M, M>(buf, Mmax, tw);
 }
 
 __device__ inline int brev(int x, int lg) { return (int)(__brev((unsigned)x) >> (32 - lg)); }
@@ -456,7 +445,7 @@ __device__ void ring_fft_half(double2* buf, int n, const double2* __restrict__ w
 // Maps staged from the host are transformed in place (in == out), so the map
 // and spectrum pointers are not restrict-qualified.
 template <int MAXM>
-__global__ __launch_bounds__(FftShape<MAXM>::TPB) void k_ring_fft(const FftJob* __restrict__ jobs, int job0,
+__global__ __launch_bounds__(FftShape<MAXM>::TPB, FftShape<MAXM>::MINB) void k_ring_fft(const FftJob* __restrict__ jobs, int job0,
                                                   const RingGeo* __restrict__ geo, const BlueInfo* __restrict__ blue,
                                                   const double2* __restrict__ chirp, const double2* __restrict__ filt,
                                                   const double2* __restrict__ tw, int Mmax, long long npix, int nsim,
