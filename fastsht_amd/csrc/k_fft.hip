@@ -219,16 +219,20 @@ __device__ inline double2 csub(double2 a, double2 b) { return make_double2(a.x -
 __device__ inline double2 mul_mi(double2 a) { return make_double2(a.y, -a.x); }  // a * (-i)
 __device__ inline double2 mul_pi(double2 a) { return make_double2(-a.y, a.x); }  // a * (+i)
 
-// Stages per pass: 2 (radix 4).  Radix-8 and radix-16 passes (3 and 4) halve
-// the LDS round trips but run slower on gfx950 (4096-point class, 200 maps:
-// 21-24 against 18.6 ms per direction): their register groups cut the waves
-// per SIMD, and with two 64 KB workgroups per CU the transform is bound by
-// latency, not by LDS bandwidth.
+// Stages per pass: 3 (radix 8), with 512 threads and registers capped for
+// four waves per SIMD in the 2048- and 4096-point classes, so two 64 KB
+// workgroups share a CU: 4 LDS round trips for 4096 points instead of 6 at
+// radix 4 (16.6 / 18.0 against 18.6 / 18.5 ms forward / inverse, 200 maps at
+// nside 1024).  Radix 16 takes 211-248 VGPRs, which leaves one 512-thread
+// workgroup per CU (or idle threads at 256), and runs at 21-24 ms.
 #ifndef FSHT_FFT_RMAX
-#define FSHT_FFT_RMAX 2
+#define FSHT_FFT_RMAX 3
 #endif
 #ifndef FSHT_FFT_TPB4096
 #define FSHT_FFT_TPB4096 512
+#endif
+#ifndef FSHT_FFT_WPE
+#define FSHT_FFT_WPE 4
 #endif
 #ifndef FSHT_FFT_TPB8192
 #define FSHT_FFT_TPB8192 1024
@@ -241,7 +245,8 @@ struct FftShape {
   // threads per workgroup (the long transforms' LDS footprint leaves one or
   // two workgroups per CU)
   static constexpr int TPB = (M >= 8192) ? FSHT_FFT_TPB8192 : ((M >= 2048) ? FSHT_FFT_TPB4096 : 256);
-  static constexpr int MINB = (M >= 8192) ? 1 : 2;  // workgroups per CU the registers must allow
+  // __launch_bounds__ minimum waves per SIMD the register allocation must allow
+  static constexpr int MINB = (M >= 8192) ? 1 : ((M >= 2048) ? FSHT_FFT_WPE : 2);
 };
 
 // exp(-2 pi i j / (2h)) for j < h <= 8 (the stage roots inside a pass)
