@@ -22,6 +22,7 @@
 // class over the north rings, with the folded operand N +- S.
 #include <algorithm>
 #include <cstdlib>
+#include <tuple>
 #include <vector>
 
 #include "fsht_internal.h"
@@ -576,8 +577,14 @@ __global__ __launch_bounds__(256) void k_quant_A(const MixOff* __restrict__ mo, 
 // ---------------------------------------------------------------------------
 // Slice GEMM building blocks
 // ---------------------------------------------------------------------------
+#ifndef FSHT_SLICE_MIX
+#define FSHT_SLICE_MIX 1  // 0: a pass's 4-tile blocks and its last column tiles in separate launches
+#endif
 #ifndef FSHT_SLICE_PP
 #define FSHT_SLICE_PP 1  // 0: the single-group slice GEMM for every slice count
+#endif
+#if FSHT_SLICE_MIX && !FSHT_SLICE_PP
+#error "the merged work lists are launched by the two-group kernel"
 #endif
 constexpr int kLdsBudget = 147456;  // bytes of LDS the slice GEMM may use
 
@@ -863,19 +870,27 @@ __global__ __launch_bounds__(512, 1) void k_slice_gemm(const SliceArgs A) {
 // s_waitcnt lgkmcnt(0) with the other counters unconstrained (gfx9 encoding)
 __device__ inline void wait_lgkm0() { __builtin_amdgcn_s_waitcnt(0xF | (0x3 << 14) | (0x7 << 4)); }
 
-template <int K, int SYN, int NB>
-__global__ __launch_bounds__(512, 1) void k_slice_gemm_pp(const SliceArgs A) {
+// Shape of the two-group kernel's pipeline for K slices and NB column tiles.
+template <int K, int NB>
+struct PpShape {
   static_assert(NB == 4 || NB == 1, "column tiles per workgroup");
-  constexpr int NG = 4;                                 // waves per group
-  constexpr int NA = NB == 4 ? 4 : 8;                   // row tiles
-  constexpr int TPW = NB == 4 ? 2 : 1;                  // column tiles per wave
-  constexpr int NBLK = (NA + NB) * K;                   // 1 KB blocks per step
-  constexpr int PER = (NBLK + NG - 1) / NG;             // loads per group-0 wave per step (the last may repeat)
-  constexpr int NS = kLdsBudget / (NBLK * 1024) < 4 ? kLdsBudget / (NBLK * 1024) : 4;  // stages
+  static constexpr int NG = 4;                                 // waves per group
+  static constexpr int NA = NB == 4 ? 4 : 8;                   // row tiles
+  static constexpr int TPW = NB == 4 ? 2 : 1;                  // column tiles per wave
+  static constexpr int NBLK = (NA + NB) * K;                   // 1 KB blocks per step
+  static constexpr int PER = (NBLK + NG - 1) / NG;             // loads per group-0 wave per step (the last may repeat)
+  static constexpr int NS = kLdsBudget / (NBLK * 1024) < 4 ? kLdsBudget / (NBLK * 1024) : 4;  // stages
+  static constexpr int BYTES = NS * NBLK * 1024;               // LDS of the stage ring
   static_assert(NS >= 3, "pipeline needs three stages");
   static_assert((NS - 1) * PER < 64, "vmcnt range");
-  __shared__ __align__(16) signed char st[NS][NBLK * 1024];
-  const LegItem it = A.items[blockIdx.x];
+};
+
+template <int K, int SYN, int NB>
+__device__ __forceinline__ void slice_gemm_pp_body(const SliceArgs& A, const LegItem it,
+                                                   signed char* __restrict__ lds) {
+  using S = PpShape<K, NB>;
+  constexpr int NG = S::NG, NA = S::NA, TPW = S::TPW, NBLK = S::NBLK, PER = S::PER, NS = S::NS;
+  auto st = [&](int stage) { return lds + stage * (NBLK * 1024); };
   if (it.m < 0) return;  // whole workgroup
   if (!cols_active(it.colblk * NB * 32, (it.colblk + 1) * NB * 32, A.nsim, A.nsimp)) return;  // no map in the block
   const int m = it.m, cls = it.par;
@@ -903,7 +918,7 @@ __global__ __launch_bounds__(512, 1) void k_slice_gemm_pp(const SliceArgs A) {
   auto issue = [&](int stage) {
 #pragma unroll
     for (int k = 0; k < PER; ++k) {
-      __builtin_amdgcn_global_load_lds((const void*)(gsrc[k] + L * 16), (lds_ptr)(st[stage] + lofs[k]), 16, 0, 0);
+      __builtin_amdgcn_global_load_lds((const void*)(gsrc[k] + L * 16), (lds_ptr)(st(stage) + lofs[k]), 16, 0, 0);
       gsrc[k] += gstep[k];
     }
   };
@@ -926,7 +941,7 @@ __global__ __launch_bounds__(512, 1) void k_slice_gemm_pp(const SliceArgs A) {
   for (int d = 0; d < K; ++d) D0[d] = D1[d] = v16i{};
   v4i ra[K], rb[TPW][K];
   auto load_regs = [&](int stage) {
-    const signed char* sb = st[stage] + L * 16;
+    const signed char* sb = st(stage) + L * 16;
 #pragma unroll
     for (int sl = 0; sl < K; ++sl) ra[sl] = *reinterpret_cast<const v4i*>(sb + (wa * K + sl) * 1024);
 #pragma unroll
@@ -987,6 +1002,31 @@ __global__ __launch_bounds__(512, 1) void k_slice_gemm_pp(const SliceArgs A) {
   slice_epilogue<K, SYN, TPW>(D0, D1, A, o, m, cls, bcls, ta, tb0, L);
 }
 
+template <int K, int SYN, int NB>
+__global__ __launch_bounds__(512, 1) void k_slice_gemm_pp(const SliceArgs A) {
+  __shared__ __align__(16) signed char st[PpShape<K, NB>::BYTES];
+  slice_gemm_pp_body<K, SYN, NB>(A, A.items[blockIdx.x], st);
+}
+
+// One launch for a pass's 4-tile blocks and its single remainder tiles
+// (colblk >= kSingleTile: the tile colblk - kSingleTile, 8 row tiles), in one
+// list ordered by m: the remainder items of an order run beside its 4-tile
+// blocks and find its P rows in cache instead of streaming them again.
+constexpr int kSingleTile = 1 << 20;
+
+template <int K, int SYN>
+__global__ __launch_bounds__(512, 1) void k_slice_gemm_pp_mix(const SliceArgs A) {
+  constexpr int B4 = PpShape<K, 4>::BYTES, B1 = PpShape<K, 1>::BYTES;
+  __shared__ __align__(16) signed char st[B4 > B1 ? B4 : B1];
+  LegItem it = A.items[blockIdx.x];
+  if (it.colblk >= kSingleTile) {
+    it.colblk -= kSingleTile;
+    slice_gemm_pp_body<K, SYN, 1>(A, it, st);
+  } else {
+    slice_gemm_pp_body<K, SYN, 4>(A, it, st);
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Host side
 // ---------------------------------------------------------------------------
@@ -1045,10 +1085,41 @@ static hipError_t fill_int(Handle* h, int* p, long long n, int v) {
 // column tiles as one padded block of 4 (six slices).  Every (m, class, row
 // tiles, column tiles) writes its outputs, starting at the first step with a
 // nonzero table digit of its row tiles.
+#if FSHT_SLICE_MIX
+// A list of extra items (`rows` times the row tiles of a 4-tile item, colblk
+// tagged with `tag`) merged into the 4-tile list, both in (m, class, row
+// block) order: each extra item follows the 4-tile items of its row blocks.
+static void merge_after(const std::vector<LegItem>& full, const std::vector<double>& full_cost,
+                        std::vector<LegItem>& extra, std::vector<double>& extra_cost, int rows, int tag) {
+  std::vector<LegItem> out;
+  std::vector<double> cost;
+  auto key = [](const LegItem& t, int r) { return std::make_tuple(t.m, t.par, t.blk / r); };
+  size_t j = 0;
+  auto take = [&]() {
+    LegItem t = extra[j];
+    t.colblk += tag;
+    out.push_back(t);
+    cost.push_back(extra_cost[j++]);
+  };
+  for (size_t i = 0; i < full.size(); ++i) {
+    while (j < extra.size() && key(extra[j], 1) < key(full[i], rows)) take();
+    out.push_back(full[i]);
+    cost.push_back(full_cost[i]);
+  }
+  while (j < extra.size()) take();
+  extra.swap(out);
+  extra_cost.swap(cost);
+}
+#endif
+
 static hipError_t build_q_lists(Handle* h, int m0, int m1, int nct, const std::vector<int>& firstA,
                                 const std::vector<int>& firstS, LegItem** ana_out, int* nana, LegItem** syn_out,
                                 int* nsyn) {
   const int nfull = nct / 4, nrem = nct % 4;
+#if FSHT_SLICE_MIX
+  std::vector<LegItem> ana0, syn0;
+  std::vector<double> ana0_cost, syn0_cost;
+#endif
   for (int k = 0; k < 3; ++k) {
     ana_out[k] = syn_out[k] = nullptr;
     nana[k] = nsyn[k] = 0;
@@ -1086,6 +1157,15 @@ static hipError_t build_q_lists(Handle* h, int m0, int m1, int nct, const std::v
           }
         }
       }
+#if FSHT_SLICE_MIX
+    // lists 1 and 2 also carry the 4-tile blocks: a pass is one launch
+    if (k == 0) {
+      ana0 = ana, syn0 = syn, ana0_cost = ana_cost, syn0_cost = syn_cost;
+    } else {
+      merge_after(ana0, ana0_cost, ana, ana_cost, k == 1 ? 2 : 1, k == 1 ? kSingleTile : 0);
+      merge_after(syn0, syn0_cost, syn, syn_cost, k == 1 ? 2 : 1, k == 1 ? kSingleTile : 0);
+    }
+#endif
     hipError_t e = upload_items(ana, ana_cost, &ana_out[k], &nana[k]);
     if (e == hipSuccess) e = upload_items(syn, syn_cost, &syn_out[k], &nsyn[k]);
     if (e != hipSuccess) return e;
@@ -1416,6 +1496,21 @@ hipError_t launch_quant_A(Handle* h, const double* alm0, const double* alm1, int
 // block).
 template <int SYN>
 static hipError_t launch_gemm(Handle* h, SliceArgs A, int K, LegItem* const* items, const int* nitems) {
+#if FSHT_SLICE_MIX && FSHT_SLICE_PP
+  if (K <= 5 && nitems[1] > 0) {  // list 1: the 4-tile blocks and the single tiles
+    A.items = items[1];
+    if (K == 4)
+      hipLaunchKernelGGL((k_slice_gemm_pp_mix<4, SYN>), dim3(nitems[1]), dim3(512), 0, h->stream, A);
+    else
+      hipLaunchKernelGGL((k_slice_gemm_pp_mix<5, SYN>), dim3(nitems[1]), dim3(512), 0, h->stream, A);
+    return hipGetLastError();
+  }
+  if (K == 6 && nitems[2] > 0) {  // list 2: the 4-tile blocks and the padded last block
+    A.items = items[2];
+    hipLaunchKernelGGL((k_slice_gemm<6, SYN, 4>), dim3(nitems[2]), dim3(512), 0, h->stream, A);
+    return hipGetLastError();
+  }
+#endif
   const int lists[2] = {0, K <= 5 ? 1 : 2};
   for (int q = 0; q < 2; ++q) {
     const int k = lists[q];
