@@ -586,7 +586,14 @@ __global__ __launch_bounds__(256) void k_quant_A(const MixOff* __restrict__ mo, 
 #if FSHT_SLICE_MIX && !FSHT_SLICE_PP
 #error "the merged work lists are launched by the two-group kernel"
 #endif
-constexpr int kLdsBudget = 147456;  // bytes of LDS the slice GEMM may use
+#ifndef FSHT_SLICE_LDS
+#define FSHT_SLICE_LDS 147456
+#endif
+constexpr int kLdsBudget = FSHT_SLICE_LDS;  // bytes of LDS the slice GEMM may use
+#ifndef FSHT_SLICE_NSMAX
+#define FSHT_SLICE_NSMAX 4
+#endif
+constexpr int kNsMax = FSHT_SLICE_NSMAX;  // stages of the operand ring, at most
 
 // s_waitcnt with vmcnt <= N and the other counters unconstrained (gfx9 encoding)
 template <int N>
@@ -770,7 +777,7 @@ __global__ __launch_bounds__(512, 1) void k_slice_gemm(const SliceArgs A) {
   constexpr int TPW = NB == 4 ? 2 : 1;                  // column tiles per wave
   constexpr int NBLK = (NA + NB) * K;                   // 1 KB blocks per step
   constexpr int PER = (NBLK + NW - 1) / NW;             // loads per wave per step (the last may repeat)
-  constexpr int NS = kLdsBudget / (NBLK * 1024) < 4 ? kLdsBudget / (NBLK * 1024) : 4;  // stages
+  constexpr int NS = kLdsBudget / (NBLK * 1024) < kNsMax ? kLdsBudget / (NBLK * 1024) : kNsMax;  // stages
   static_assert(NS >= 3, "pipeline needs three stages");
   __shared__ __align__(16) signed char st[NS][NBLK * 1024];
   const LegItem it = A.items[blockIdx.x];
@@ -879,7 +886,7 @@ struct PpShape {
   static constexpr int TPW = NB == 4 ? 2 : 1;                  // column tiles per wave
   static constexpr int NBLK = (NA + NB) * K;                   // 1 KB blocks per step
   static constexpr int PER = (NBLK + NG - 1) / NG;             // loads per group-0 wave per step (the last may repeat)
-  static constexpr int NS = kLdsBudget / (NBLK * 1024) < 4 ? kLdsBudget / (NBLK * 1024) : 4;  // stages
+  static constexpr int NS = kLdsBudget / (NBLK * 1024) < kNsMax ? kLdsBudget / (NBLK * 1024) : kNsMax;  // stages
   static constexpr int BYTES = NS * NBLK * 1024;               // LDS of the stage ring
   static_assert(NS >= 3, "pipeline needs three stages");
   static_assert((NS - 1) * PER < 64, "vmcnt range");
