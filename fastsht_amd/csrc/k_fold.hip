@@ -20,9 +20,10 @@
 
 namespace fsht {
 
-// Thread blocks of the gather / fold kernels: x = 16 spectrum columns (one
-// 128-byte line of a pixel-major row; nsimp is a multiple of 16, so column
-// blocks are never partly empty), y = rings (gathers) or slots (fold).
+// Thread block of the residual gather: x = 16 columns (one 128-byte line of
+// an F row; nsimp is a multiple of 16, so column blocks are never partly
+// empty), y = rings.  The gather and the fold use 256 threads as 16 x 16
+// tiles of their own (below).
 #ifndef FSHT_FOLD_BX
 #define FSHT_FOLD_BX 16
 #endif
