@@ -274,10 +274,16 @@ class Server:
     """hbm_budget: device bytes the loop may hold (lanes, batch buffers,
     request inputs and outputs).  capacity(nside, lmax) -> maps per
     batch (default: 64, down to 16 where a lane's estimate would pass a
-    quarter of the budget).  mixed_max_nside: largest nside whose mixed-precision tables
-    are built; mixed requests above it run in FP64 (precision_used says so)."""
+    quarter of the budget).  Mixed requests run in mixed precision for
+    mixed_min_nside <= nside <= mixed_max_nside and in FP64 otherwise
+    (precision_used says which): above mixed_max_nside the int8 tables do not
+    fit the budget, and below mixed_min_nside the transforms are too short
+    for the int8 contractions to pay (t2alm(3) + alm2t on one MI355X: mixed
+    0.88x FP64 at nside 128, 1.06x at 256, 1.10x at 512, 1.32x at 1024 for
+    32-64 maps), so FP64 is as fast and exact."""
     hbm_budget: float = 200 * GB
     niter: int = 3
+    mixed_min_nside: int = 256
     mixed_max_nside: int = 1024
     capacity: object = None
     lanes: dict = field(default_factory=dict)
@@ -324,7 +330,7 @@ class Server:
         if self.device_events:
             req.ready = _torch().cuda.Event()
             req.ready.record()
-        mixed_ok = req.precision == "mixed" and req.nside <= self.mixed_max_nside
+        mixed_ok = req.precision == "mixed" and self.mixed_min_nside <= req.nside <= self.mixed_max_nside
         req.precision_used = "mixed" if mixed_ok else "fp64"
         self.waiting.append(req)
         return req

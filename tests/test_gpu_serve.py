@@ -91,7 +91,7 @@ def test_mix_invariance(torch_cuda):
     torch = torch_cuda
     from fastsht_amd.serve import Server
     rng = np.random.default_rng(7)
-    srv = Server(hbm_budget=60e9, niter=3)
+    srv = Server(hbm_budget=60e9, niter=3, mixed_min_nside=0)  # mixed lanes at every nside here
     shapes = [(16, "forward", False, 5, "fp64"), (32, "backward", True, 9, "fp64"), (16, "forward", True, 3, "mixed"),
               (64, "forward", False, 17, "mixed"), (32, "forward", False, 2, "fp64"), (64, "backward", False, 7, "mixed"),
               (16, "backward", True, 11, "fp64"), (32, "forward", True, 6, "mixed"), (64, "forward", True, 4, "fp64"),
@@ -113,7 +113,7 @@ def test_split_requests_and_eviction(torch_cuda):
     torch = torch_cuda
     from fastsht_amd.serve import Server
     rng = np.random.default_rng(9)
-    srv = Server(hbm_budget=100e9, niter=2, capacity=lambda nside, lmax: 8)
+    srv = Server(hbm_budget=100e9, niter=2, capacity=lambda nside, lmax: 8, mixed_min_nside=0)
     first = srv.submit(make_request(torch, rng, 64, "forward", True, 10, "mixed"))
     warm = srv.submit(make_request(torch, rng, 64, "forward", True, 2, "mixed", lmax=95))
     srv.run_until_idle()

@@ -82,7 +82,7 @@ def test_packing_order_split_and_pair_offsets():
 
 
 def test_precisions_batch_separately_and_mixed_cap():
-    srv = server(hbm_budget=1e12, capacity=lambda ns, lm: 64, mixed_max_nside=32)
+    srv = server(hbm_budget=1e12, capacity=lambda ns, lm: 64, mixed_min_nside=0, mixed_max_nside=32)
     a = srv.submit(req(16, 3, precision="mixed"))
     b = srv.submit(req(16, 3))
     c = srv.submit(req(64, 3, precision="mixed"))  # above mixed_max_nside: served in FP64
@@ -117,7 +117,7 @@ def test_request_above_budget_raises():
 def test_spin2_buffers_charged_on_first_polarised_mixed_request():
     """A lane is charged the spin-2 chunk buffers only when its first
     polarised mixed request is admitted, once."""
-    srv = server(hbm_budget=1e13, capacity=lambda ns, lm: 16)
+    srv = server(hbm_budget=1e13, capacity=lambda ns, lm: 16, mixed_min_nside=0)
     t = srv.submit(req(32, 3, precision="mixed"))
     srv.run_until_idle()
     lane = [l for l in FakeLane.made if l.nside == 32][-1]
@@ -159,3 +159,15 @@ def test_recreated_lane_reserves_what_it_held():
     srv.submit(req(32, 2))
     done = srv.run_until_idle()
     assert len(done) == 2 and srv.evictions == 3
+
+
+def test_small_mixed_requests_run_in_fp64():
+    """Below mixed_min_nside (default 256) mixed requests run in FP64, and
+    their lane never builds mixed-precision state."""
+    srv = server(hbm_budget=1e13, capacity=lambda ns, lm: 16)
+    a = srv.submit(req(128, 3, precision="mixed"))
+    b = srv.submit(req(256, 3, precision="mixed"))
+    srv.run_until_idle()
+    assert a.precision_used == "fp64" and b.precision_used == "mixed"
+    lane128 = [l for l in FakeLane.made if l.nside == 128][0]
+    assert not lane128.mixed_ready and lane128.footprint == pytest.approx(estimate_lane_bytes(128, 256, 16))
