@@ -479,6 +479,10 @@ __global__ __launch_bounds__(256, 2) void k_ftab_dig(const int2* __restrict__ it
   }
 }
 
+#ifndef FSHT_QUANT_CT_FAST
+#define FSHT_QUANT_CT_FAST 1  // 0: the orders are the fastest grid dimension of the operand slicers
+#endif
+
 // ---------------------------------------------------------------------------
 // Analysis operand: the m-major folded spectrum F (FP64, layout of k_gather,
 // ncomp field components) -> FQ slices with the per-column exponents the
@@ -492,7 +496,8 @@ __global__ __launch_bounds__(256) void k_quant_F(const MixOff* __restrict__ mo, 
                                                  const int* __restrict__ FQe, signed char* __restrict__ FQ, int nhr,
                                                  int ncomp, int nsim, int nsimp) {
   const int ncols = 2 * ncomp * nsimp;
-  const int m = blockIdx.x, cls = blockIdx.y & 1, kb = blockIdx.y >> 1, ct = blockIdx.z;
+  const int m = FSHT_QUANT_CT_FAST ? blockIdx.z : blockIdx.x, cls = blockIdx.y & 1, kb = blockIdx.y >> 1;
+  const int ct = FSHT_QUANT_CT_FAST ? blockIdx.x : blockIdx.z;
   if (!cols_active(ct * 32, ct * 32 + 32, nsim, nsimp)) return;  // no map in the tile
   const MixOff o = mo[2 * m + cls];
   const int cl = threadIdx.x & 31, q = threadIdx.x >> 5;
@@ -528,7 +533,8 @@ __global__ __launch_bounds__(256) void k_quant_A(const MixOff* __restrict__ mo, 
                                                  int* __restrict__ AQe, int ncomp, int nsim, int nsimp, int lmax) {
   __shared__ double red[8][32];
   __shared__ int ex[32];
-  const int m = blockIdx.x, cls = blockIdx.y, ct = blockIdx.z;
+  const int m = FSHT_QUANT_CT_FAST ? blockIdx.z : blockIdx.x, cls = blockIdx.y;
+  const int ct = FSHT_QUANT_CT_FAST ? blockIdx.x : blockIdx.z;
   if (!cols_active(ct * 32, ct * 32 + 32, nsim, nsimp)) return;  // no map in the tile
   const MixOff o = mo[2 * m + cls];
   const int cl = threadIdx.x & 31, q = threadIdx.x >> 5;
@@ -1463,9 +1469,15 @@ hipError_t launch_fexp_reset(Handle* h) {
   return fill_int(h, h->d_FQe, 2LL * (h->lmax + 1) * 2 * cur_ncomp(h) * h->nsimp, kMixZeroE);
 }
 
+// Grid of the operand slicers: (orders, y, column tiles), or with the column
+// tiles first (FSHT_QUANT_CT_FAST) so that the tiles of one row run together.
+static dim3 quant_grid(int orders, int y, int tiles) {
+  return FSHT_QUANT_CT_FAST ? dim3(tiles, y, orders) : dim3(orders, y, tiles);
+}
+
 hipError_t launch_quant_F(Handle* h, const double* F, int K) {
   const int ncomp = cur_ncomp(h);
-  dim3 grid(h->lmax + 1, 2 * h->mix_nrb_max, ncomp * h->nsimp / 16);
+  dim3 grid = quant_grid(h->lmax + 1, 2 * h->mix_nrb_max, ncomp * h->nsimp / 16);
 #define FSHT_QF(KV)                                                                                              \
   hipLaunchKernelGGL(k_quant_F<KV>, grid, dim3(256), 0, h->stream, h->d_mix_off, h->d_i0, h->d_goff, F, h->d_FQe, \
                      h->d_FQ, h->nhr, ncomp, h->nsim, h->nsimp)
@@ -1481,7 +1493,7 @@ hipError_t launch_quant_F(Handle* h, const double* F, int K) {
 
 hipError_t launch_quant_A(Handle* h, const double* alm0, const double* alm1, int K, int have_exp) {
   const int ncomp = cur_ncomp(h);
-  dim3 grid(h->lmax + 1, 2, ncomp * h->nsimp / 16);
+  dim3 grid = quant_grid(h->lmax + 1, 2, ncomp * h->nsimp / 16);
 #define FSHT_QA(KV, E)                                                                                               \
   hipLaunchKernelGGL((k_quant_A<KV, E>), grid, dim3(256), 0, h->stream, h->d_mix_off, alm0, alm1, h->d_AQ, h->d_AQe, \
                      ncomp, h->nsim, h->nsimp, h->lmax)
