@@ -35,6 +35,19 @@ constexpr FoldBlk kFoldBlk{FSHT_FOLD_BX, 256 / FSHT_FOLD_BX};
 #define FSHT_GATHER_RINGS 32
 #endif
 constexpr int kGatherRings = FSHT_GATHER_RINGS;  // rings per gather workgroup
+// Grid order of the gathers and the fold: the column blocks are the fastest
+// grid dimension (x), so the workgroups that share the rows of one order or
+// ring run together and whole rows are fetched at once (alias fold and
+// residual gathers 44-47 -> 42-43 ms, gather 8.5-9.0 -> 7.8-8.1 ms per step
+// at nside 1024 / 200 maps); FSHT_FOLD_CT_FAST=0 puts the orders / rings first.
+#ifndef FSHT_FOLD_CT_FAST
+#define FSHT_FOLD_CT_FAST 1
+#endif
+__device__ inline int blk_major() { return FSHT_FOLD_CT_FAST ? blockIdx.z : blockIdx.x; }
+__device__ inline int blk_cols() { return FSHT_FOLD_CT_FAST ? blockIdx.x : blockIdx.z; }
+static dim3 fold_grid(unsigned major, unsigned y, unsigned cols) {
+  return FSHT_FOLD_CT_FAST ? dim3(cols, y, major) : dim3(major, y, cols);
+}
 
 
 template <int NCOMP>
@@ -42,8 +55,8 @@ __global__ __launch_bounds__(256) void k_fold(const RingGeo* __restrict__ geo, c
                                               const double2* __restrict__ tw, const double* __restrict__ G,
                                               const long long* __restrict__ goff, double* __restrict__ spec,
                                               int lmax, int nhr, int nnorth, long long npix, int nsim, int nsimp) {
-  if (!cols_active(blockIdx.z * 16, (blockIdx.z + 1) * 16, nsim, nsimp)) return;  // no map here
-  const int i = blockIdx.x;  // north ring
+  if (!cols_active(blk_cols() * 16, (blk_cols() + 1) * 16, nsim, nsimp)) return;  // no map here
+  const int i = blk_major();  // north ring
   const RingGeo g = geo[i];
   const int n = g.nfft;
   const int k0 = blockIdx.y * 16;
@@ -55,7 +68,7 @@ __global__ __launch_bounds__(256) void k_fold(const RingGeo* __restrict__ geo, c
   // sum role: lane = column (G rows are read 16 columns at a time), then slot
   {
     const int kk = k0 + (int)(threadIdx.x >> 4);
-    const int col = blockIdx.z * 16 + (int)(threadIdx.x & 15);  // spectrum column: comp*nsimp + sim
+    const int col = blk_cols() * 16 + (int)(threadIdx.x & 15);  // spectrum column: comp*nsimp + sim
     const int comp = col / nsimp, sim = col - comp * nsimp;
     const int cre = (comp * 2) * nsimp + sim, cim = (comp * 2 + 1) * nsimp + sim;
     const bool edge = (kk == 0) || (kk == n / 2);
@@ -119,7 +132,7 @@ __global__ __launch_bounds__(256) void k_fold(const RingGeo* __restrict__ geo, c
   // write role: lane = slot (a map's PERM slots are adjacent), then column
   const int tk = threadIdx.x & 15, tc = threadIdx.x >> 4;
   const int kk = k0 + tk;
-  const int col = blockIdx.z * 16 + tc;
+  const int col = blk_cols() * 16 + tc;
   const int comp = col / nsimp, sim = col - comp * nsimp;
   if (col >= ldT || sim >= nsim || kk > n / 2) return;  // padding column (no map) or past the ring's last slot
   int s0, s1 = -1;
@@ -199,8 +212,8 @@ __global__ __launch_bounds__(256) void k_gather(const RingGeo* __restrict__ geo,
                                                 const double* __restrict__ spec, double* __restrict__ F,
                                                 const long long* __restrict__ goff, int nhr, int nnorth, int lmax,
                                                 long long npix, int nsim, int nsimp, int* __restrict__ fexp) {
-  if (!cols_active(blockIdx.z * 16, (blockIdx.z + 1) * 16, nsim, nsimp)) return;  // no map here
-  const int m0 = blockIdx.x * kGatherM, c0 = blockIdx.z * 16;
+  if (!cols_active(blk_cols() * 16, (blk_cols() + 1) * 16, nsim, nsimp)) return;  // no map here
+  const int m0 = blk_major() * kGatherM, c0 = blk_cols() * 16;
   // i0 is non-decreasing in m: no order of the block keeps a ring below i0(m0)
   const int ibeg = max((int)blockIdx.y * kGatherRings, i0tab[m0]);
   const int iend = min(nhr + 1, ((int)blockIdx.y + 1) * kGatherRings);
@@ -291,11 +304,11 @@ __global__ __launch_bounds__(256) void k_gather_prefix(const RingGeo* __restrict
                                                        double* __restrict__ F, const long long* __restrict__ goff,
                                                        int nhr, int nnorth, int nsim, int nsimp, int lmax, int full,
                                                        int* __restrict__ fexp) {
-  if (!cols_active(blockIdx.z * kFoldBlk.x, (blockIdx.z + 1) * kFoldBlk.x, nsim, nsimp)) return;  // no map here
-  const int m = blockIdx.x;
+  if (!cols_active(blk_cols() * kFoldBlk.x, (blk_cols() + 1) * kFoldBlk.x, nsim, nsimp)) return;  // no map here
+  const int m = blk_major();
   const int i0 = i0tab[m];
   const int ldT = NCOMP * nsimp;
-  const int colx = blockIdx.z * kFoldBlk.x + threadIdx.x;
+  const int colx = blk_cols() * kFoldBlk.x + threadIdx.x;
   const bool colok = colx < ldT;
   const int col = colok ? colx : ldT - 1;
   const int comp = col / nsimp, sim = col - comp * nsimp;
@@ -391,8 +404,8 @@ __global__ __launch_bounds__(256) void k_gather_prefix(const RingGeo* __restrict
 hipError_t launch_gather_prefix(Handle* h, const double* G, const double* F0, double* F, int full, int* fexp) {
   const int ncomp = h->spin ? 2 : 1;
   dim3 block(kFoldBlk.x, kFoldBlk.y);
-  dim3 grid(h->lmax + 1, (h->nnorth + kGatherRings - 1) / kGatherRings,
-            (ncomp * h->nsimp + kFoldBlk.x - 1) / kFoldBlk.x);
+  dim3 grid = fold_grid(h->lmax + 1, (h->nnorth + kGatherRings - 1) / kGatherRings,
+                        (ncomp * h->nsimp + kFoldBlk.x - 1) / kFoldBlk.x);
   if (ncomp == 2)
     hipLaunchKernelGGL(k_gather_prefix<2>, grid, block, 0, h->stream, h->d_geo, h->d_i0, h->d_tw, h->d_slot, G, F0,
                        F, h->d_goff, h->nhr, h->nnorth, h->nsim, h->nsimp, h->lmax, full, fexp);
@@ -405,8 +418,8 @@ hipError_t launch_gather_prefix(Handle* h, const double* G, const double* F0, do
 hipError_t launch_gather(Handle* h, const double* spec, double* F, int* fexp) {
   const int ncomp = h->spin ? 2 : 1;
   dim3 block(256);
-  dim3 grid((h->lmax + kGatherM) / kGatherM, (h->nnorth + kGatherRings - 1) / kGatherRings,
-            (ncomp * h->nsimp + 15) / 16);
+  dim3 grid = fold_grid((h->lmax + kGatherM) / kGatherM, (h->nnorth + kGatherRings - 1) / kGatherRings,
+                        (ncomp * h->nsimp + 15) / 16);
   if (ncomp == 2)
     hipLaunchKernelGGL(k_gather<2>, grid, block, 0, h->stream, h->d_geo, h->d_i0, h->d_tw, h->d_slot, spec, F,
                        h->d_goff, h->nhr, h->nnorth, h->lmax, h->npix, h->nsim, h->nsimp, fexp);
@@ -420,7 +433,7 @@ hipError_t launch_fold(Handle* h, const double* G, double* spec) {
   const int ncomp = h->spin ? 2 : 1;
   const int maxk = 2 * h->nside;  // n/2 for the longest ring
   dim3 block(256);
-  dim3 grid(h->nnorth, (maxk + 1 + 15) / 16, (ncomp * h->nsimp + 15) / 16);
+  dim3 grid = fold_grid(h->nnorth, (maxk + 1 + 15) / 16, (ncomp * h->nsimp + 15) / 16);
   if (ncomp == 2)
     hipLaunchKernelGGL(k_fold<2>, grid, block, 0, h->stream, h->d_geo, h->d_i0, h->d_tw, G, h->d_goff, spec,
                        h->lmax, h->nhr, h->nnorth, h->npix, h->nsim, h->nsimp);
