@@ -48,6 +48,13 @@ __device__ inline int blk_cols() { return FSHT_FOLD_CT_FAST ? blockIdx.x : block
 static dim3 fold_grid(unsigned major, unsigned y, unsigned cols) {
   return FSHT_FOLD_CT_FAST ? dim3(cols, y, major) : dim3(major, y, cols);
 }
+// The fold writes each map's ring segment 16 slots at a time: with its slot
+// blocks as the fastest grid dimension, consecutive workgroups write
+// consecutive segments of the same maps (fold 10.3 -> 9.1 ms at nside 1024 /
+// 200 maps); FSHT_FOLD_SLOT_FAST=0 uses the gathers' order.
+#ifndef FSHT_FOLD_SLOT_FAST
+#define FSHT_FOLD_SLOT_FAST 1
+#endif
 
 
 template <int NCOMP>
@@ -55,11 +62,12 @@ __global__ __launch_bounds__(256) void k_fold(const RingGeo* __restrict__ geo, c
                                               const double2* __restrict__ tw, const double* __restrict__ G,
                                               const long long* __restrict__ goff, double* __restrict__ spec,
                                               int lmax, int nhr, int nnorth, long long npix, int nsim, int nsimp) {
-  if (!cols_active(blk_cols() * 16, (blk_cols() + 1) * 16, nsim, nsimp)) return;  // no map here
-  const int i = blk_major();  // north ring
+  const int bcol = FSHT_FOLD_SLOT_FAST ? (int)blockIdx.y : blk_cols();
+  if (!cols_active(bcol * 16, (bcol + 1) * 16, nsim, nsimp)) return;  // no map here
+  const int i = FSHT_FOLD_SLOT_FAST ? (int)blockIdx.z : blk_major();  // north ring
   const RingGeo g = geo[i];
   const int n = g.nfft;
-  const int k0 = blockIdx.y * 16;
+  const int k0 = (FSHT_FOLD_SLOT_FAST ? (int)blockIdx.x : (int)blockIdx.y) * 16;
   if (k0 > n / 2) return;  // whole workgroup
   __shared__ double tile[4][16][17];  // [value][column][slot]
   const bool pair = g.i1S != g.i1N;
@@ -68,7 +76,7 @@ __global__ __launch_bounds__(256) void k_fold(const RingGeo* __restrict__ geo, c
   // sum role: lane = column (G rows are read 16 columns at a time), then slot
   {
     const int kk = k0 + (int)(threadIdx.x >> 4);
-    const int col = blk_cols() * 16 + (int)(threadIdx.x & 15);  // spectrum column: comp*nsimp + sim
+    const int col = bcol * 16 + (int)(threadIdx.x & 15);  // spectrum column: comp*nsimp + sim
     const int comp = col / nsimp, sim = col - comp * nsimp;
     const int cre = (comp * 2) * nsimp + sim, cim = (comp * 2 + 1) * nsimp + sim;
     const bool edge = (kk == 0) || (kk == n / 2);
@@ -132,7 +140,7 @@ __global__ __launch_bounds__(256) void k_fold(const RingGeo* __restrict__ geo, c
   // write role: lane = slot (a map's PERM slots are adjacent), then column
   const int tk = threadIdx.x & 15, tc = threadIdx.x >> 4;
   const int kk = k0 + tk;
-  const int col = blk_cols() * 16 + tc;
+  const int col = bcol * 16 + tc;
   const int comp = col / nsimp, sim = col - comp * nsimp;
   if (col >= ldT || sim >= nsim || kk > n / 2) return;  // padding column (no map) or past the ring's last slot
   int s0, s1 = -1;
@@ -433,7 +441,8 @@ hipError_t launch_fold(Handle* h, const double* G, double* spec) {
   const int ncomp = h->spin ? 2 : 1;
   const int maxk = 2 * h->nside;  // n/2 for the longest ring
   dim3 block(256);
-  dim3 grid = fold_grid(h->nnorth, (maxk + 1 + 15) / 16, (ncomp * h->nsimp + 15) / 16);
+  dim3 grid = FSHT_FOLD_SLOT_FAST ? dim3((maxk + 1 + 15) / 16, (ncomp * h->nsimp + 15) / 16, h->nnorth)
+                                  : fold_grid(h->nnorth, (maxk + 1 + 15) / 16, (ncomp * h->nsimp + 15) / 16);
   if (ncomp == 2)
     hipLaunchKernelGGL(k_fold<2>, grid, block, 0, h->stream, h->d_geo, h->d_i0, h->d_tw, G, h->d_goff, spec,
                        h->lmax, h->nhr, h->nnorth, h->npix, h->nsim, h->nsimp);
