@@ -33,7 +33,8 @@ struct FftJob {
 };
 
 struct BlueInfo {
-  long long chirp_off;  // offset of w_j = exp(-i pi j^2 / L), j < L (L = Bluestein length)
+  long long chirp_off;  // offset of w_j = exp(-i pi j^2 / L), j < L (L = Bluestein length),
+                        // then exp(-i pi k / L), k = 0..L (the real-spectrum split roots)
   long long filt_off;   // offset of FFT_M(b)/M (M entries, bit-reversed order)
 };
 

@@ -60,16 +60,11 @@ static void host_fft(std::vector<double2>& a, int sign) {
 
 // Job list (grouped by LDS size class), twiddles and Bluestein tables.
 // kMaxLds is the longest transform held in LDS (8192 complex doubles, 128 KB).
-// A polar ring of length n = 4q (q not a power of two) is transformed with
-// its mirror as one length-n Bluestein transform when its filter length
-// next_pow2(2n-1) fits; longer rings (nside 2048, n in (4096, 8192)) are
-// transformed one real ring at a time as a complex length-n/2 Bluestein
-// transform of the even/odd pixel pairs.
+// A power-of-two ring is transformed with its mirror (one job per pair); a
+// polar ring of length n = 4q (q not a power of two) is transformed one real
+// ring at a time as a complex length-n/2 Bluestein transform of its even/odd
+// pixel pairs (filter length next_pow2(n - 1), half the length-n transform's).
 static constexpr int kMaxLds = 8192;
-#ifndef FSHT_FFT_HALF_ALL
-#define FSHT_FFT_HALF_ALL 1
-#endif
-static constexpr bool kHalfAll = FSHT_FFT_HALF_ALL != 0;
 
 hipError_t build_bluestein(Handle* h) {
   const int nside = h->nside;
@@ -82,8 +77,6 @@ hipError_t build_bluestein(Handle* h) {
     const int n = (i < nside) ? 4 * (i + 1) : 4 * nside;
     if (is_pow2(n)) {
       jobs.push_back({i, n, n, 0, 0});
-    } else if (!kHalfAll && next_pow2(2 * n - 1) <= kMaxLds) {
-      jobs.push_back({i, n, next_pow2(2 * n - 1), n, 0});
     } else {
       const int L = n / 2;  // polar rings are never the equator, so both halves exist
       jobs.push_back({i, n, next_pow2(2 * L - 1), L, 1});
@@ -110,7 +103,7 @@ hipError_t build_bluestein(Handle* h) {
     h->blue_n.push_back(j.L);
     h->blue_M.push_back(j.M);
     h->blue_host.push_back(BlueInfo{coff, foff});
-    coff += j.L;
+    coff += 2 * j.L + 1;  // chirp (L) and split roots (L + 1)
     foff += j.M;
   }
   const int nb = (int)h->blue_n.size();
@@ -132,6 +125,12 @@ hipError_t build_bluestein(Handle* h) {
             chirp[c0 + j] = make_double2(c, -s);
             bx[j] = make_double2(c, s);  // conj(w_j)
             if (j) bx[M - j] = make_double2(c, s);
+          }
+          // split roots exp(-2 pi i k / (2n)) = exp(-pi i k / n), k = 0..n, after the
+          // chirp (n here is the half-length transform's L)
+          for (int k = 0; k <= n; ++k) {
+            const long double ang = 3.141592653589793238462643383279502884L * (long double)k / (long double)n;
+            chirp[c0 + n + k] = make_double2((double)cosl(ang), -(double)sinl(ang));
           }
           host_fft(bx, -1);
           // stored in bit-reversed order: the kernels multiply the bit-reversed
@@ -197,9 +196,23 @@ hipError_t build_bluestein(Handle* h) {
 // ---------------------------------------------------------------------------
 // Device FFT helpers (one workgroup, data in LDS).
 // ---------------------------------------------------------------------------
-// LDS position of transform element i: one pad element per 16 spreads the
-// short-span butterflies over the banks
-__device__ inline int pz(int i) { return i + (i >> 4); }
+// LDS position of element i of an M-point transform.  One pad element per 8
+// puts the lanes of the span-8 pass (elements 8 apart, 128 B without the pad)
+// on distinct banks; one more per M/8 does the same for indices read or
+// written in bit-reversed order (8 consecutive indices differ only in their
+// top 3 bits, M/8 elements apart).  Consecutive elements stay contiguous
+// within each aligned group of 8.
+template <int M>
+__device__ inline int pz(int i) {
+  constexpr int LG = (M >= 2) ? __builtin_ctz(M) : 0;
+  if constexpr (LG >= 6)
+    return i + (i >> 3) + (i >> (LG - 3));
+  else
+    return i + (i >> 3);
+}
+// LDS elements of an M-point transform: pz<M>(M - 1) < kLdsLen<M>
+template <int M>
+constexpr int kLdsLen = M + M / 8 + 8;
 __device__ inline double2 cmul(double2 a, double2 b) {
   return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
@@ -281,7 +294,7 @@ __device__ inline void dif_pass(double2* buf, int Mmax, const double2* __restric
       const int p = g & (Q - 1), i0 = (g - p) * NR + p;
       double2 x[NR];
 #pragma unroll
-      for (int k = 0; k < NR; ++k) x[k] = buf[pz(i0 + k * Q)];
+      for (int k = 0; k < NR; ++k) x[k] = buf[pz<M>(i0 + k * Q)];
       double2 wb = tw[p * stride];
 #pragma unroll
       for (int t = 0; t < R; ++t) {
@@ -298,7 +311,7 @@ __device__ inline void dif_pass(double2* buf, int Mmax, const double2* __restric
         if (t + 1 < R) wb = cmul(wb, wb);
       }
 #pragma unroll
-      for (int k = 0; k < NR; ++k) buf[pz(i0 + k * Q)] = x[k];
+      for (int k = 0; k < NR; ++k) buf[pz<M>(i0 + k * Q)] = x[k];
     }
   }
   __syncthreads();
@@ -318,7 +331,7 @@ __device__ inline void dit_pass(double2* buf, int Mmax, const double2* __restric
       const int p = g & (Q - 1), i0 = (g - p) * NR + p;
       double2 x[NR];
 #pragma unroll
-      for (int k = 0; k < NR; ++k) x[k] = buf[pz(i0 + k * Q)];
+      for (int k = 0; k < NR; ++k) x[k] = buf[pz<M>(i0 + k * Q)];
       double2 wp[R];
       wp[R - 1] = cconj(tw[p * stride]);
 #pragma unroll
@@ -337,7 +350,7 @@ __device__ inline void dit_pass(double2* buf, int Mmax, const double2* __restric
         }
       }
 #pragma unroll
-      for (int k = 0; k < NR; ++k) buf[pz(i0 + k * Q)] = x[k];
+      for (int k = 0; k < NR; ++k) buf[pz<M>(i0 + k * Q)] = x[k];
     }
   }
   __syncthreads();
@@ -376,69 +389,95 @@ __device__ void lds_dit_inv(double2* buf, int Mmax, const double2* __restrict__ 
 
 __device__ inline int brev(int x, int lg) { return (int)(__brev((unsigned)x) >> (32 - lg)); }
 
+// A table pointer the compiler cannot see through: loads through it stay
+// after the LDS passes instead of being hoisted above them and held (and
+// spilled) across the pass calls.
+__device__ inline const double2* after_passes(const double2* p) {
+  asm volatile("" : "+s"(p));
+  return p;
+}
+
+// The thread-strided loops below step by the compile-time workgroup size and
+// are unrolled, with their global reads made unconditional (indices clamped
+// inside the ring), so all of a thread's loads are in flight together.
+
 // One real ring of length n = 2L (L not a power of two) transformed alone:
 // y_j = x_{2j} + i x_{2j+1}, j < L, goes through a length-L Bluestein
-// transform (chirp w, filter fl), and the real spectrum is split out of it.
+// transform (chirp w, filter fl), and the real spectrum is split out of it
+// (split roots exp(-2 pi i k / n), k = 0..L, stored after the chirp).
 //   forward: A_k = (Y_k + conj Y_{L-k})/2, B_k = (Y_k - conj Y_{L-k})/(2i),
 //            X_k = A_k + e^{-2 pi i k/n} B_k for k = 0..L (Y indices mod L);
 //   inverse: Y'_k = (X_k + conj X_{L-k}) + i e^{+2 pi i k/n} (X_k - conj X_{L-k})
 //            for k < L; y = unnormalised inverse DFT_L(Y'), x_{2j} = Re y_j,
 //            x_{2j+1} = Im y_j.
-// in / out point at the ring's first pixel (PERM spectrum or pixels).
+// in / out point at the ring's first pixel (PERM spectrum or pixels); L <= M/2.
 template <int M>
 __device__ void ring_fft_half(double2* buf, int n, const double2* __restrict__ w, const double2* __restrict__ fl,
                               const double2* __restrict__ tw, int Mmax, const double* in, double* out,
                               int inverse) {
+  constexpr int TPB = FftShape<M>::TPB, NS = (M + TPB - 1) / TPB, NH = (M / 2 + TPB) / TPB;
   const int L = n >> 1;
-  const double rn = 2.0 / (double)n;
+  const double2* sr = w + L;  // split roots exp(-2 pi i k / n), k = 0..L
   if (!inverse) {
-    for (int j = threadIdx.x; j < M; j += blockDim.x)
-      buf[pz(j)] = (j < L) ? cmul(make_double2(in[2 * j], in[2 * j + 1]), w[j]) : make_double2(0.0, 0.0);
+#pragma unroll
+    for (int r = 0; r < NS; ++r) {
+      const int j = threadIdx.x + r * TPB, jc = min(j, L - 1);
+      const double2 y = cmul(make_double2(in[2 * jc], in[2 * jc + 1]), w[jc]);
+      if (M % TPB == 0 || j < M) buf[pz<M>(j)] = (j < L) ? y : make_double2(0.0, 0.0);
+    }
     __syncthreads();
     lds_dif<M>(buf, Mmax, tw);
-    for (int r = threadIdx.x; r < M; r += blockDim.x) buf[pz(r)] = cmul(buf[pz(r)], fl[r]);
+#pragma unroll
+    for (int r = 0; r < NS; ++r) {
+      const int j = threadIdx.x + r * TPB;
+      if (M % TPB == 0 || j < M) buf[pz<M>(j)] = cmul(buf[pz<M>(j)], fl[j]);
+    }
     __syncthreads();
     lds_dit_inv<M>(buf, Mmax, tw);
-    for (int k = threadIdx.x; k <= L; k += blockDim.x) {
+    const double2* wa = after_passes(w);
+    // Lanes past k = L take k = L and store the value its own lane stores, so
+    // the stores need no branch (a branch would hold each chirp load back
+    // until the previous store's block).
+#pragma unroll
+    for (int r = 0; r < NH; ++r) {
+      const int k = min((int)threadIdx.x + r * TPB, L);
       const int k1 = (k == L) ? 0 : k, k2 = (k == 0) ? 0 : L - k;
-      const double2 Yk = cmul(buf[pz(k1)], w[k1]);
-      const double2 Yc = cconj(cmul(buf[pz(k2)], w[k2]));
+      const double2 Yk = cmul(buf[pz<M>(k1)], wa[k1]);
+      const double2 Yc = cconj(cmul(buf[pz<M>(k2)], wa[k2]));
       const double2 A = make_double2(0.5 * (Yk.x + Yc.x), 0.5 * (Yk.y + Yc.y));
       const double2 B = make_double2(0.5 * (Yk.y - Yc.y), -0.5 * (Yk.x - Yc.x));
-      double s, c;
-      sincospi((double)k * rn, &s, &c);
-      const double2 X = cadd(A, cmul(B, make_double2(c, -s)));
-      if (k == 0) {
-        out[0] = X.x;
-      } else if (k == L) {
-        out[1] = X.x;
-      } else {
-        out[2 * k] = X.x;
-        out[2 * k + 1] = X.y;
-      }
+      const double2 X = cadd(A, cmul(B, wa[L + k]));
+      // PERM: slot 0 = Re X_0, slot 1 = Re X_L, slots 2k, 2k+1 = Re, Im X_k
+      const bool mid = k > 0 && k < L;
+      const int ia = (k == 0) ? 0 : (k == L ? 1 : 2 * k);
+      out[ia] = X.x;
+      out[mid ? 2 * k + 1 : ia] = mid ? X.y : X.x;
     }
   } else {
-    for (int k = threadIdx.x; k < M; k += blockDim.x) {
+    for (int k = threadIdx.x; k < M; k += TPB) {
       double2 Z = make_double2(0.0, 0.0);
       if (k < L) {
         const int kc = L - k;  // 1..L
+        // slot 0 holds Re X_0, slot 1 Re X_L (both with Im 0)
         const double2 Xk = (k == 0) ? make_double2(in[0], 0.0) : make_double2(in[2 * k], in[2 * k + 1]);
         const double2 Xc = (kc == L) ? make_double2(in[1], 0.0) : make_double2(in[2 * kc], -in[2 * kc + 1]);
-        double s, c;
-        sincospi((double)k * rn, &s, &c);
-        const double2 D = cmul(csub(Xk, Xc), make_double2(c, s));
+        const double2 D = cmul(csub(Xk, Xc), cconj(sr[k]));
         const double2 Yp = cadd(cadd(Xk, Xc), mul_pi(D));
         Z = cmul(cconj(Yp), w[k]);
       }
-      buf[pz(k)] = Z;
+      buf[pz<M>(k)] = Z;
     }
     __syncthreads();
     lds_dif<M>(buf, Mmax, tw);
-    for (int r = threadIdx.x; r < M; r += blockDim.x) buf[pz(r)] = cmul(buf[pz(r)], fl[r]);
+#pragma unroll
+    for (int r = 0; r < NS; ++r) {
+      const int j = threadIdx.x + r * TPB;
+      if (M % TPB == 0 || j < M) buf[pz<M>(j)] = cmul(buf[pz<M>(j)], fl[j]);
+    }
     __syncthreads();
     lds_dit_inv<M>(buf, Mmax, tw);
-    for (int j = threadIdx.x; j < L; j += blockDim.x) {
-      const double2 y = cconj(cmul(buf[pz(j)], w[j]));
+    for (int j = threadIdx.x; j < L; j += TPB) {
+      const double2 y = cconj(cmul(buf[pz<M>(j)], w[j]));
       out[2 * j] = y.x;
       out[2 * j + 1] = y.y;
     }
@@ -446,9 +485,11 @@ __device__ void ring_fft_half(double2* buf, int n, const double2* __restrict__ w
 }
 
 // Forward: maps (sim-major, npix per map) -> spec (same layout, PERM per ring).
-// Inverse: spec -> maps.  grid.x = map index, grid.y = job within the class.
+// Inverse: spec -> maps.  One workgroup per (map, job within the class).
 // Maps staged from the host are transformed in place (in == out), so the map
-// and spectrum pointers are not restrict-qualified.
+// and spectrum pointers are not restrict-qualified.  A job is either a
+// power-of-two ring with its mirror (n = M, z = x_N + i x_S; the equator
+// ring has no mirror and reads itself as S) or one half-length Bluestein ring.
 template <int MAXM>
 __global__ __launch_bounds__(FftShape<MAXM>::TPB, FftShape<MAXM>::MINB) void k_ring_fft(const FftJob* __restrict__ jobs, int job0,
                                                   const RingGeo* __restrict__ geo, const BlueInfo* __restrict__ blue,
@@ -457,7 +498,10 @@ __global__ __launch_bounds__(FftShape<MAXM>::TPB, FftShape<MAXM>::MINB) void k_r
                                                   const double* in0, const double* in1, double* out0,
                                                   double* out1, int inverse,
                                                   int map0) {
-  __shared__ double2 buf[MAXM + MAXM / 16];
+  __shared__ double2 buf[kLdsLen<MAXM>];
+  constexpr int M = MAXM;  // every job of a class has M equal to the class size
+  constexpr int TPB = FftShape<M>::TPB, NS = (M + TPB - 1) / TPB, NH = (M / 2 + TPB) / TPB;
+  constexpr int lg = FftShape<M>::LG;
   const FftJob jb = jobs[job0 + blockIdx.y];
   const int mapi = map0 + blockIdx.x;
   const int comp = mapi / nsim, k = mapi - comp * nsim;
@@ -465,107 +509,84 @@ __global__ __launch_bounds__(FftShape<MAXM>::TPB, FftShape<MAXM>::MINB) void k_r
   double* out = (comp ? out1 : out0) + (long long)k * npix;
   const RingGeo g = geo[jb.ring];
   const bool pair = g.i1S != g.i1N;
-  constexpr int M = MAXM;  // every job of a class has M equal to the class size
-  const int n = jb.n, lg = ilog2(M);
-  const bool isblue = jb.blue >= 0;
-  const double2* w = isblue ? chirp + blue[jb.blue].chirp_off : nullptr;
-  const double2* fl = isblue ? filt + blue[jb.blue].filt_off : nullptr;
   if constexpr (M >= 8) {
     if (jb.half) {
       const long long i1 = (jb.half == 1) ? g.i1N : g.i1S;
-      ring_fft_half<M>(buf, n, w, fl, tw, Mmax, in + i1, out + i1, inverse);
+      const BlueInfo bi = blue[jb.blue];
+      ring_fft_half<M>(buf, jb.n, chirp + bi.chirp_off, filt + bi.filt_off, tw, Mmax, in + i1, out + i1, inverse);
       return;
     }
   }
-
+  // power-of-two pair: n = M
+  const double* inN = in + g.i1N;
+  const double* inS = in + g.i1S;
+  double* outN = out + g.i1N;
+  double* outS = out + g.i1S;
   if (!inverse) {
     // ---- load z = x_N + i x_S
-    for (int j = threadIdx.x; j < M; j += blockDim.x) {
-      double2 z = make_double2(0.0, 0.0);
-      if (j < n) {
-        z.x = in[g.i1N + j];
-        z.y = pair ? in[g.i1S + j] : 0.0;
-        if (isblue) z = cmul(z, w[j]);
+#pragma unroll
+    for (int r = 0; r < NS; ++r) {
+      const int j = threadIdx.x + r * TPB;
+      if (M % TPB == 0 || j < M) {
+        const double xN = inN[j], xS = inS[j];
+        buf[pz<M>(j)] = make_double2(xN, pair ? xS : 0.0);
       }
-      buf[pz(j)] = z;
     }
     __syncthreads();
     lds_dif<M>(buf, Mmax, tw);
-    if (isblue) {
-      for (int r = threadIdx.x; r < M; r += blockDim.x) buf[pz(r)] = cmul(buf[pz(r)], fl[r]);
-      __syncthreads();
-      lds_dit_inv<M>(buf, Mmax, tw);
-    }
-    // ---- separate the two real spectra, write PERM
-    for (int kk = threadIdx.x; kk <= n / 2; kk += blockDim.x) {
-      const int nk = (n - kk) % n;
-      double2 Zk, Zn;
-      if (isblue) {
-        Zk = cmul(buf[pz(kk)], w[kk]);
-        Zn = cmul(buf[pz(nk)], w[nk]);
-      } else {
-        Zk = buf[pz(brev(kk, lg))];
-        Zn = buf[pz(brev(nk, lg))];
-      }
+    // ---- separate the two real spectra (bit-reversed order in LDS), write PERM
+#pragma unroll
+    for (int r = 0; r < NH; ++r) {
+      const int kq = (int)threadIdx.x + r * TPB, kk = min(kq, M / 2);
+      const int nk = (M - kk) & (M - 1);
+      const double2 Zk = buf[pz<M>(brev(kk, lg))];
+      const double2 Zn = buf[pz<M>(brev(nk, lg))];
       const double2 XN = make_double2(0.5 * (Zk.x + Zn.x), 0.5 * (Zk.y - Zn.y));
       const double2 XS = make_double2(0.5 * (Zk.y + Zn.y), -0.5 * (Zk.x - Zn.x));
-      if (kk == 0) {
-        out[g.i1N] = XN.x;
-        if (pair) out[g.i1S] = XS.x;
-      } else if (kk == n / 2) {
-        out[g.i1N + 1] = XN.x;
-        if (pair) out[g.i1S + 1] = XS.x;
-      } else {
-        out[g.i1N + 2 * kk] = XN.x;
-        out[g.i1N + 2 * kk + 1] = XN.y;
-        if (pair) {
-          out[g.i1S + 2 * kk] = XS.x;
-          out[g.i1S + 2 * kk + 1] = XS.y;
+      if (kq <= M / 2) {
+        if (kk == 0) {
+          outN[0] = XN.x;
+          if (pair) outS[0] = XS.x;
+        } else if (kk == M / 2) {
+          outN[1] = XN.x;
+          if (pair) outS[1] = XS.x;
+        } else {
+          outN[2 * kk] = XN.x;
+          outN[2 * kk + 1] = XN.y;
+          if (pair) {
+            outS[2 * kk] = XS.x;
+            outS[2 * kk + 1] = XS.y;
+          }
         }
       }
     }
   } else {
-    // ---- build Z_k = X_N[k] + i X_S[k], k in [0, n)
-    for (int kk = threadIdx.x; kk < M; kk += blockDim.x) {
-      double2 Z = make_double2(0.0, 0.0);
-      if (kk < n) {
-        const int kh = kk <= n / 2 ? kk : n - kk;
-        const bool cj = kk > n / 2;
-        double2 XN, XS = make_double2(0.0, 0.0);
-        if (kh == 0) {
-          XN = make_double2(in[g.i1N], 0.0);
-          if (pair) XS = make_double2(in[g.i1S], 0.0);
-        } else if (kh == n / 2) {
-          XN = make_double2(in[g.i1N + 1], 0.0);
-          if (pair) XS = make_double2(in[g.i1S + 1], 0.0);
-        } else {
-          XN = make_double2(in[g.i1N + 2 * kh], in[g.i1N + 2 * kh + 1]);
-          if (pair) XS = make_double2(in[g.i1S + 2 * kh], in[g.i1S + 2 * kh + 1]);
-        }
-        if (cj) {
-          XN.y = -XN.y;
-          XS.y = -XS.y;
-        }
-        Z = make_double2(XN.x - XS.y, XN.y + XS.x);
-        if (isblue) Z = cmul(cconj(Z), w[kk]);
+    // ---- build Z_k = X_N[k] + i X_S[k], k < M, into bit-reversed positions
+    for (int kk = threadIdx.x; kk < M; kk += TPB) {
+      const int kh = kk <= M / 2 ? kk : M - kk;
+      double2 XN, XS = make_double2(0.0, 0.0);
+      if (kh == 0) {  // slot 0 holds Re X_0, slot 1 Re X_{M/2} (both with Im 0)
+        XN = make_double2(inN[0], 0.0);
+        if (pair) XS = make_double2(inS[0], 0.0);
+      } else if (kh == M / 2) {
+        XN = make_double2(inN[1], 0.0);
+        if (pair) XS = make_double2(inS[1], 0.0);
+      } else {
+        XN = make_double2(inN[2 * kh], inN[2 * kh + 1]);
+        if (pair) XS = make_double2(inS[2 * kh], inS[2 * kh + 1]);
       }
-      if (isblue)
-        buf[pz(kk)] = Z;
-      else if (kk < M)
-        buf[pz(brev(kk, lg))] = Z;
+      if (kk > M / 2) {
+        XN.y = -XN.y;
+        XS.y = -XS.y;
+      }
+      buf[pz<M>(brev(kk, lg))] = make_double2(XN.x - XS.y, XN.y + XS.x);
     }
     __syncthreads();
-    if (isblue) {
-      lds_dif<M>(buf, Mmax, tw);
-      for (int r = threadIdx.x; r < M; r += blockDim.x) buf[pz(r)] = cmul(buf[pz(r)], fl[r]);
-      __syncthreads();
-    }
     lds_dit_inv<M>(buf, Mmax, tw);
-    for (int j = threadIdx.x; j < n; j += blockDim.x) {
-      double2 z = buf[pz(j)];
-      if (isblue) z = cconj(cmul(z, w[j]));
-      out[g.i1N + j] = z.x;
-      if (pair) out[g.i1S + j] = z.y;
+    for (int j = threadIdx.x; j < M; j += TPB) {
+      const double2 z = buf[pz<M>(j)];
+      outN[j] = z.x;
+      if (pair) outS[j] = z.y;
     }
   }
 }
