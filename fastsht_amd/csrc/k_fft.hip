@@ -283,47 +283,56 @@ __device__ inline double2 stage_root(int h, int j) {
 // {b*LEN + p + k*Q : k < 2^R}, Q = LEN / 2^R.  Stage t pairs k and k + h
 // (h = 2^(R-1-t), bit h of k clear) with the twiddle W_S^(p + (k mod h) Q)
 // = W_S^p * exp(-2 pi i (k mod h) / (2h)), S = LEN / 2^t, W_S^p = (W_LEN^p)^(2^t).
+// dif_regs applies them to one group held in registers (x[k] = element k).
 template <int M, int R, int LEN>
-__device__ inline void dif_pass(double2* buf, int Mmax, const double2* __restrict__ tw) {
-  constexpr int NR = 1 << R, Q = LEN / NR, G = M / NR, TPB = FftShape<M>::TPB, NG = (G + TPB - 1) / TPB;
-  const int stride = Mmax / LEN;
+__device__ inline void dif_regs(double2 (&x)[1 << R], int p, int Mmax, const double2* __restrict__ tw) {
+  constexpr int NR = 1 << R;
+  double2 wb = tw[p * (Mmax / LEN)];
 #pragma unroll
-  for (int r = 0; r < NG; ++r) {
-    const int g = threadIdx.x + TPB * r;
-    if (G % TPB == 0 || g < G) {
-      const int p = g & (Q - 1), i0 = (g - p) * NR + p;
-      double2 x[NR];
+  for (int t = 0; t < R; ++t) {
+    const int h = NR >> (t + 1);
 #pragma unroll
-      for (int k = 0; k < NR; ++k) x[k] = buf[pz<M>(i0 + k * Q)];
-      double2 wb = tw[p * stride];
-#pragma unroll
-      for (int t = 0; t < R; ++t) {
-        const int h = NR >> (t + 1);
-#pragma unroll
-        for (int k = 0; k < NR; ++k) {
-          if (k & h) continue;
-          const int j = k & (h - 1);
-          const double2 w = (j == 0) ? wb : cmul(wb, stage_root(h, j));
-          const double2 u = x[k], v = x[k + h];
-          x[k] = cadd(u, v);
-          x[k + h] = cmul(csub(u, v), w);
-        }
-        if (t + 1 < R) wb = cmul(wb, wb);
-      }
-#pragma unroll
-      for (int k = 0; k < NR; ++k) buf[pz<M>(i0 + k * Q)] = x[k];
+    for (int k = 0; k < NR; ++k) {
+      if (k & h) continue;
+      const int j = k & (h - 1);
+      const double2 w = (j == 0) ? wb : cmul(wb, stage_root(h, j));
+      const double2 u = x[k], v = x[k + h];
+      x[k] = cadd(u, v);
+      x[k + h] = cmul(csub(u, v), w);
     }
+    if (t + 1 < R) wb = cmul(wb, wb);
   }
-  __syncthreads();
 }
 
 // DIT pass (inverse twiddles): stages of span 2Q, 4Q, ..., LEN on the same
 // groups.  Stage t pairs k and k + h (h = 2^t) after multiplying element
 // k + h by conj(W_S^(p + (k mod h) Q)), S = 2hQ, W_S^p = (W_LEN^p)^(2^(R-1-t)).
 template <int M, int R, int LEN>
-__device__ inline void dit_pass(double2* buf, int Mmax, const double2* __restrict__ tw) {
+__device__ inline void dit_regs(double2 (&x)[1 << R], int p, int Mmax, const double2* __restrict__ tw) {
+  constexpr int NR = 1 << R;
+  double2 wp[R];
+  wp[R - 1] = cconj(tw[p * (Mmax / LEN)]);
+#pragma unroll
+  for (int t = R - 2; t >= 0; --t) wp[t] = cmul(wp[t + 1], wp[t + 1]);
+#pragma unroll
+  for (int t = 0; t < R; ++t) {
+    const int h = 1 << t;
+#pragma unroll
+    for (int k = 0; k < NR; ++k) {
+      if (k & h) continue;
+      const int j = k & (h - 1);
+      const double2 w = (j == 0) ? wp[t] : cmul(wp[t], cconj(stage_root(h, j)));
+      const double2 u = x[k], v = cmul(x[k + h], w);
+      x[k] = cadd(u, v);
+      x[k + h] = csub(u, v);
+    }
+  }
+}
+
+// One pass over every group of the transform in LDS.
+template <int M, int R, int LEN, bool DIT>
+__device__ inline void lds_pass(double2* buf, int Mmax, const double2* __restrict__ tw) {
   constexpr int NR = 1 << R, Q = LEN / NR, G = M / NR, TPB = FftShape<M>::TPB, NG = (G + TPB - 1) / TPB;
-  const int stride = Mmax / LEN;
 #pragma unroll
   for (int r = 0; r < NG; ++r) {
     const int g = threadIdx.x + TPB * r;
@@ -332,23 +341,10 @@ __device__ inline void dit_pass(double2* buf, int Mmax, const double2* __restric
       double2 x[NR];
 #pragma unroll
       for (int k = 0; k < NR; ++k) x[k] = buf[pz<M>(i0 + k * Q)];
-      double2 wp[R];
-      wp[R - 1] = cconj(tw[p * stride]);
-#pragma unroll
-      for (int t = R - 2; t >= 0; --t) wp[t] = cmul(wp[t + 1], wp[t + 1]);
-#pragma unroll
-      for (int t = 0; t < R; ++t) {
-        const int h = 1 << t;
-#pragma unroll
-        for (int k = 0; k < NR; ++k) {
-          if (k & h) continue;
-          const int j = k & (h - 1);
-          const double2 w = (j == 0) ? wp[t] : cmul(wp[t], cconj(stage_root(h, j)));
-          const double2 u = x[k], v = cmul(x[k + h], w);
-          x[k] = cadd(u, v);
-          x[k + h] = csub(u, v);
-        }
-      }
+      if constexpr (DIT)
+        dit_regs<M, R, LEN>(x, p, Mmax, tw);
+      else
+        dif_regs<M, R, LEN>(x, p, Mmax, tw);
 #pragma unroll
       for (int k = 0; k < NR; ++k) buf[pz<M>(i0 + k * Q)] = x[k];
     }
@@ -356,35 +352,104 @@ __device__ inline void dit_pass(double2* buf, int Mmax, const double2* __restric
   __syncthreads();
 }
 
-// Passes of kFftR stages while that many remain, then the rest.
-template <int M, int LEN>
-__device__ inline void dif_from(double2* buf, int Mmax, const double2* __restrict__ tw) {
-  if constexpr (LEN >= 2) {
-    constexpr int LGL = __builtin_ctz(LEN), R = LGL < kFftR ? LGL : kFftR;
-    dif_pass<M, R, LEN>(buf, Mmax, tw);
-    dif_from<M, (LEN >> R)>(buf, Mmax, tw);
+// Radix-2 stages of the pass of span LEN: kFftR while that many remain, then the rest.
+template <int LEN>
+constexpr int pass_r() {
+  return __builtin_ctz(LEN) < kFftR ? __builtin_ctz(LEN) : kFftR;
+}
+// Span of the last DIF pass (= the first DIT pass): every pass of that
+// span has Q = 1, so a group is NR consecutive elements.
+template <int M>
+constexpr int kLastSpan = 1 << (__builtin_ctz(M) - kFftR * ((__builtin_ctz(M) - 1) / kFftR));
+
+// DIF passes of span LEN, LEN >> R, ... while the span exceeds STOP.
+template <int M, int LEN, int STOP>
+__device__ inline void dif_range(double2* buf, int Mmax, const double2* __restrict__ tw) {
+  if constexpr (LEN >= 2 && LEN > STOP) {
+    constexpr int R = pass_r<LEN>();
+    lds_pass<M, R, LEN, false>(buf, Mmax, tw);
+    dif_range<M, (LEN >> R), STOP>(buf, Mmax, tw);
   }
 }
-// The same passes in reverse order (smallest span first).
-template <int M, int LEN>
-__device__ inline void dit_upto(double2* buf, int Mmax, const double2* __restrict__ tw) {
-  if constexpr (LEN >= 2) {
-    constexpr int LGL = __builtin_ctz(LEN), R = LGL < kFftR ? LGL : kFftR;
-    dit_upto<M, (LEN >> R)>(buf, Mmax, tw);
-    dit_pass<M, R, LEN>(buf, Mmax, tw);
+// The DIT passes of the spans above START up to LEN, smallest span first.
+template <int M, int LEN, int START>
+__device__ inline void dit_range(double2* buf, int Mmax, const double2* __restrict__ tw) {
+  if constexpr (LEN >= 2 && LEN > START) {
+    constexpr int R = pass_r<LEN>();
+    dit_range<M, (LEN >> R), START>(buf, Mmax, tw);
+    lds_pass<M, R, LEN, true>(buf, Mmax, tw);
   }
 }
 
 // Decimation in frequency, forward (e^{-2 pi i jk/M}): natural in, bit-reversed out.
 template <int M>
-__device__ void lds_dif(double2* buf, int Mmax, const double2* __restrict__ tw) {
-  dif_from<M, M>(buf, Mmax, tw);
+__device__ __attribute__((noinline)) void lds_dif(double2* buf, int Mmax, const double2* __restrict__ tw) {
+  dif_range<M, M, 1>(buf, Mmax, tw);
 }
 
 // Decimation in time, inverse (e^{+2 pi i jk/M}, unnormalised): bit-reversed in, natural out.
 template <int M>
-__device__ void lds_dit_inv(double2* buf, int Mmax, const double2* __restrict__ tw) {
-  dit_upto<M, M>(buf, Mmax, tw);
+__device__ __attribute__((noinline)) void lds_dit_inv(double2* buf, int Mmax, const double2* __restrict__ tw) {
+  dit_range<M, M, 1>(buf, Mmax, tw);
+}
+
+// The long classes (first pass with one group per thread) run their first
+// DIF pass on the loaded inputs and their last DIT pass into the outputs in
+// registers, and the Bluestein transforms their last DIF pass, the filter
+// and the first DIT pass in one LDS round trip: 7 instead of 10 for a
+// Bluestein transform, 5 instead of 6 for a power-of-two pair.
+template <int M>
+constexpr bool kRegEnds = M >= 1024 && (M >> pass_r<M>()) % FftShape<M>::TPB == 0;
+
+// DIF passes after the first.
+template <int M>
+__device__ __attribute__((noinline)) void lds_dif_tail(double2* buf, int Mmax, const double2* __restrict__ tw) {
+  dif_range<M, (M >> pass_r<M>()), 1>(buf, Mmax, tw);
+}
+// DIF passes between the first and the last.
+template <int M>
+__device__ __attribute__((noinline)) void lds_dif_mid(double2* buf, int Mmax, const double2* __restrict__ tw) {
+  dif_range<M, (M >> pass_r<M>()), kLastSpan<M>>(buf, Mmax, tw);
+}
+// DIT passes after the first (all the remaining ones).
+template <int M>
+__device__ __attribute__((noinline)) void lds_dit_tail(double2* buf, int Mmax, const double2* __restrict__ tw) {
+  dit_range<M, M, kLastSpan<M>>(buf, Mmax, tw);
+}
+// DIT passes before the last.
+template <int M>
+__device__ __attribute__((noinline)) void lds_dit_head(double2* buf, int Mmax, const double2* __restrict__ tw) {
+  dit_range<M, (M >> pass_r<M>()), 1>(buf, Mmax, tw);
+}
+// DIT passes between the first and the last.
+template <int M>
+__device__ __attribute__((noinline)) void lds_dit_mid(double2* buf, int Mmax, const double2* __restrict__ tw) {
+  dit_range<M, (M >> pass_r<M>()), kLastSpan<M>>(buf, Mmax, tw);
+}
+// Last DIF pass, pointwise Bluestein filter (stored in bit-reversed order)
+// and first DIT pass on each group of kLastSpan consecutive elements.
+template <int M>
+__device__ __attribute__((noinline)) void lds_dif_filter_dit(double2* buf, int Mmax, const double2* __restrict__ tw,
+                                   const double2* __restrict__ fl) {
+  constexpr int LEN = kLastSpan<M>, R = __builtin_ctz(LEN), NR = LEN, G = M / NR, TPB = FftShape<M>::TPB,
+                NG = (G + TPB - 1) / TPB;
+#pragma unroll
+  for (int r = 0; r < NG; ++r) {
+    const int g = threadIdx.x + TPB * r;
+    if (G % TPB == 0 || g < G) {
+      const int i0 = g * NR;
+      double2 x[NR];
+#pragma unroll
+      for (int k = 0; k < NR; ++k) x[k] = buf[pz<M>(i0 + k)];
+      dif_regs<M, R, LEN>(x, 0, Mmax, tw);
+#pragma unroll
+      for (int k = 0; k < NR; ++k) x[k] = cmul(x[k], fl[i0 + k]);
+      dit_regs<M, R, LEN>(x, 0, Mmax, tw);
+#pragma unroll
+      for (int k = 0; k < NR; ++k) buf[pz<M>(i0 + k)] = x[k];
+    }
+  }
+  __syncthreads();
 }
 
 __device__ inline int brev(int x, int lg) { return (int)(__brev((unsigned)x) >> (32 - lg)); }
@@ -418,22 +483,42 @@ __device__ void ring_fft_half(double2* buf, int n, const double2* __restrict__ w
   constexpr int TPB = FftShape<M>::TPB, NS = (M + TPB - 1) / TPB, NH = (M / 2 + TPB) / TPB;
   const int L = n >> 1;
   const double2* sr = w + L;  // split roots exp(-2 pi i k / n), k = 0..L
+  constexpr int R0 = pass_r<M>(), NR0 = 1 << R0, Q0 = M >> R0;
   if (!inverse) {
+    if constexpr (kRegEnds<M>) {
+      // first DIF pass on the loaded inputs: thread g holds elements g + k Q0
+      const int g = threadIdx.x;
+      double2 x[NR0];
 #pragma unroll
-    for (int r = 0; r < NS; ++r) {
-      const int j = threadIdx.x + r * TPB, jc = min(j, L - 1);
-      const double2 y = cmul(make_double2(in[2 * jc], in[2 * jc + 1]), w[jc]);
-      if (M % TPB == 0 || j < M) buf[pz<M>(j)] = (j < L) ? y : make_double2(0.0, 0.0);
-    }
-    __syncthreads();
-    lds_dif<M>(buf, Mmax, tw);
+      for (int k = 0; k < NR0; ++k) {
+        const int j = g + k * Q0, jc = min(j, L - 1);
+        const double2 y = cmul(make_double2(in[2 * jc], in[2 * jc + 1]), w[jc]);
+        x[k] = (j < L) ? y : make_double2(0.0, 0.0);
+      }
+      dif_regs<M, R0, M>(x, g, Mmax, tw);
 #pragma unroll
-    for (int r = 0; r < NS; ++r) {
-      const int j = threadIdx.x + r * TPB;
-      if (M % TPB == 0 || j < M) buf[pz<M>(j)] = cmul(buf[pz<M>(j)], fl[j]);
+      for (int k = 0; k < NR0; ++k) buf[pz<M>(g + k * Q0)] = x[k];
+      __syncthreads();
+      lds_dif_mid<M>(buf, Mmax, tw);
+      lds_dif_filter_dit<M>(buf, Mmax, tw, fl);
+      lds_dit_tail<M>(buf, Mmax, tw);
+    } else {
+#pragma unroll
+      for (int r = 0; r < NS; ++r) {
+        const int j = threadIdx.x + r * TPB, jc = min(j, L - 1);
+        const double2 y = cmul(make_double2(in[2 * jc], in[2 * jc + 1]), w[jc]);
+        if (M % TPB == 0 || j < M) buf[pz<M>(j)] = (j < L) ? y : make_double2(0.0, 0.0);
+      }
+      __syncthreads();
+      lds_dif<M>(buf, Mmax, tw);
+#pragma unroll
+      for (int r = 0; r < NS; ++r) {
+        const int j = threadIdx.x + r * TPB;
+        if (M % TPB == 0 || j < M) buf[pz<M>(j)] = cmul(buf[pz<M>(j)], fl[j]);
+      }
+      __syncthreads();
+      lds_dit_inv<M>(buf, Mmax, tw);
     }
-    __syncthreads();
-    lds_dit_inv<M>(buf, Mmax, tw);
     const double2* wa = after_passes(w);
     // Lanes past k = L take k = L and store the value its own lane stores, so
     // the stores need no branch (a branch would hold each chirp load back
@@ -454,32 +539,58 @@ __device__ void ring_fft_half(double2* buf, int n, const double2* __restrict__ w
       out[mid ? 2 * k + 1 : ia] = mid ? X.y : X.x;
     }
   } else {
-    for (int k = threadIdx.x; k < M; k += TPB) {
-      double2 Z = make_double2(0.0, 0.0);
-      if (k < L) {
-        const int kc = L - k;  // 1..L
-        // slot 0 holds Re X_0, slot 1 Re X_L (both with Im 0)
-        const double2 Xk = (k == 0) ? make_double2(in[0], 0.0) : make_double2(in[2 * k], in[2 * k + 1]);
-        const double2 Xc = (kc == L) ? make_double2(in[1], 0.0) : make_double2(in[2 * kc], -in[2 * kc + 1]);
-        const double2 D = cmul(csub(Xk, Xc), cconj(sr[k]));
-        const double2 Yp = cadd(cadd(Xk, Xc), mul_pi(D));
-        Z = cmul(cconj(Yp), w[k]);
-      }
-      buf[pz<M>(k)] = Z;
-    }
-    __syncthreads();
-    lds_dif<M>(buf, Mmax, tw);
+    // the Bluestein input Z_k, k < M (zero from L on)
+    auto zin = [&](int k) -> double2 {
+      if (k >= L) return make_double2(0.0, 0.0);
+      const int kc = L - k;  // 1..L
+      // slot 0 holds Re X_0, slot 1 Re X_L (both with Im 0)
+      const double2 Xk = (k == 0) ? make_double2(in[0], 0.0) : make_double2(in[2 * k], in[2 * k + 1]);
+      const double2 Xc = (kc == L) ? make_double2(in[1], 0.0) : make_double2(in[2 * kc], -in[2 * kc + 1]);
+      const double2 D = cmul(csub(Xk, Xc), cconj(sr[k]));
+      const double2 Yp = cadd(cadd(Xk, Xc), mul_pi(D));
+      return cmul(cconj(Yp), w[k]);
+    };
+    if constexpr (kRegEnds<M>) {
+      const int g = threadIdx.x;
+      double2 x[NR0];
 #pragma unroll
-    for (int r = 0; r < NS; ++r) {
-      const int j = threadIdx.x + r * TPB;
-      if (M % TPB == 0 || j < M) buf[pz<M>(j)] = cmul(buf[pz<M>(j)], fl[j]);
-    }
-    __syncthreads();
-    lds_dit_inv<M>(buf, Mmax, tw);
-    for (int j = threadIdx.x; j < L; j += TPB) {
-      const double2 y = cconj(cmul(buf[pz<M>(j)], w[j]));
-      out[2 * j] = y.x;
-      out[2 * j + 1] = y.y;
+      for (int k = 0; k < NR0; ++k) x[k] = zin(g + k * Q0);
+      dif_regs<M, R0, M>(x, g, Mmax, tw);
+#pragma unroll
+      for (int k = 0; k < NR0; ++k) buf[pz<M>(g + k * Q0)] = x[k];
+      __syncthreads();
+      lds_dif_mid<M>(buf, Mmax, tw);
+      lds_dif_filter_dit<M>(buf, Mmax, tw, fl);
+      lds_dit_mid<M>(buf, Mmax, tw);
+      // last DIT pass into the outputs: thread g holds elements g + k Q0
+#pragma unroll
+      for (int k = 0; k < NR0; ++k) x[k] = buf[pz<M>(g + k * Q0)];
+      dit_regs<M, R0, M>(x, g, Mmax, tw);
+#pragma unroll
+      for (int k = 0; k < NR0; ++k) {
+        const int j = g + k * Q0;
+        if (j < L) {
+          const double2 y = cconj(cmul(x[k], w[j]));
+          out[2 * j] = y.x;
+          out[2 * j + 1] = y.y;
+        }
+      }
+    } else {
+      for (int k = threadIdx.x; k < M; k += TPB) buf[pz<M>(k)] = zin(k);
+      __syncthreads();
+      lds_dif<M>(buf, Mmax, tw);
+#pragma unroll
+      for (int r = 0; r < NS; ++r) {
+        const int j = threadIdx.x + r * TPB;
+        if (M % TPB == 0 || j < M) buf[pz<M>(j)] = cmul(buf[pz<M>(j)], fl[j]);
+      }
+      __syncthreads();
+      lds_dit_inv<M>(buf, Mmax, tw);
+      for (int j = threadIdx.x; j < L; j += TPB) {
+        const double2 y = cconj(cmul(buf[pz<M>(j)], w[j]));
+        out[2 * j] = y.x;
+        out[2 * j + 1] = y.y;
+      }
     }
   }
 }
@@ -524,16 +635,33 @@ __global__ __launch_bounds__(FftShape<MAXM>::TPB, FftShape<MAXM>::MINB) void k_r
   double* outS = out + g.i1S;
   if (!inverse) {
     // ---- load z = x_N + i x_S
+    if constexpr (kRegEnds<M>) {
+      // first DIF pass on the loaded inputs: thread g holds elements g + k Q0
+      constexpr int R0 = pass_r<M>(), NR0 = 1 << R0, Q0 = M >> R0;
+      const int g = threadIdx.x;
+      double2 x[NR0];
 #pragma unroll
-    for (int r = 0; r < NS; ++r) {
-      const int j = threadIdx.x + r * TPB;
-      if (M % TPB == 0 || j < M) {
-        const double xN = inN[j], xS = inS[j];
-        buf[pz<M>(j)] = make_double2(xN, pair ? xS : 0.0);
+      for (int k = 0; k < NR0; ++k) {
+        const double xN = inN[g + k * Q0], xS = inS[g + k * Q0];
+        x[k] = make_double2(xN, pair ? xS : 0.0);
       }
+      dif_regs<M, R0, M>(x, g, Mmax, tw);
+#pragma unroll
+      for (int k = 0; k < NR0; ++k) buf[pz<M>(g + k * Q0)] = x[k];
+      __syncthreads();
+      lds_dif_tail<M>(buf, Mmax, tw);
+    } else {
+#pragma unroll
+      for (int r = 0; r < NS; ++r) {
+        const int j = threadIdx.x + r * TPB;
+        if (M % TPB == 0 || j < M) {
+          const double xN = inN[j], xS = inS[j];
+          buf[pz<M>(j)] = make_double2(xN, pair ? xS : 0.0);
+        }
+      }
+      __syncthreads();
+      lds_dif<M>(buf, Mmax, tw);
     }
-    __syncthreads();
-    lds_dif<M>(buf, Mmax, tw);
     // ---- separate the two real spectra (bit-reversed order in LDS), write PERM
 #pragma unroll
     for (int r = 0; r < NH; ++r) {
@@ -582,11 +710,27 @@ __global__ __launch_bounds__(FftShape<MAXM>::TPB, FftShape<MAXM>::MINB) void k_r
       buf[pz<M>(brev(kk, lg))] = make_double2(XN.x - XS.y, XN.y + XS.x);
     }
     __syncthreads();
-    lds_dit_inv<M>(buf, Mmax, tw);
-    for (int j = threadIdx.x; j < M; j += TPB) {
-      const double2 z = buf[pz<M>(j)];
-      outN[j] = z.x;
-      if (pair) outS[j] = z.y;
+    if constexpr (kRegEnds<M>) {
+      // last DIT pass into the outputs: thread g holds elements g + k Q0
+      constexpr int R0 = pass_r<M>(), NR0 = 1 << R0, Q0 = M >> R0;
+      lds_dit_head<M>(buf, Mmax, tw);
+      const int g = threadIdx.x;
+      double2 x[NR0];
+#pragma unroll
+      for (int k = 0; k < NR0; ++k) x[k] = buf[pz<M>(g + k * Q0)];
+      dit_regs<M, R0, M>(x, g, Mmax, tw);
+#pragma unroll
+      for (int k = 0; k < NR0; ++k) {
+        outN[g + k * Q0] = x[k].x;
+        if (pair) outS[g + k * Q0] = x[k].y;
+      }
+    } else {
+      lds_dit_inv<M>(buf, Mmax, tw);
+      for (int j = threadIdx.x; j < M; j += TPB) {
+        const double2 z = buf[pz<M>(j)];
+        outN[j] = z.x;
+        if (pair) outS[j] = z.y;
+      }
     }
   }
 }
