@@ -5,12 +5,12 @@
 // (slot 0 = Re X_0, slot 1 = Re X_{n/2}, slots 2k,2k+1 = Re/Im X_k), inverse
 // unnormalised.  Here one workgroup transforms one north ring together with
 // its south mirror for one map (z = x_N + i x_S, one complex FFT, then the two
-// real spectra are separated).  Power-of-two rings use a radix-4 FFT in LDS;
+// real spectra are separated).  Power-of-two rings use a radix-8 FFT in LDS;
 // other polar rings (length 4q with q not a power of two, often with a large
-// prime factor) use Bluestein's chirp-z convolution with a power-of-two FFT.
-// At nside 2048 the polar rings longer than 4096 pixels would need a 16384-
-// point Bluestein filter (256 KB of LDS); they are transformed one real ring
-// per workgroup through a half-length complex transform instead.
+// prime factor) are transformed one real ring per workgroup: the even/odd
+// pixel pairs form a complex half-length sequence whose DFT comes from
+// Bluestein's chirp-z convolution with a power-of-two FFT (filter length
+// next_pow2(n - 1): at most 8192 points, 128 KB of LDS, at nside 2048).
 #include <cmath>
 #include <thread>
 
@@ -172,7 +172,7 @@ hipError_t build_bluestein(Handle* h) {
     return hipErrorInvalidValue;
   }
   h->Mmax = Mmax;
-  std::vector<double2> tw(Mmax);  // full circle: radix-4 passes read w^{3k}
+  std::vector<double2> tw(Mmax);  // W_Mmax^k; a pass of span LEN reads k = p Mmax / LEN, p < LEN / 2^R
   for (int k = 0; k < Mmax; ++k) {
     const long double ang = -2.0L * 3.141592653589793238462643383279502884L * k / Mmax;
     tw[k] = make_double2((double)cosl(ang), (double)sinl(ang));
