@@ -398,8 +398,9 @@ __device__ __attribute__((noinline)) void lds_dit_inv(double2* buf, int Mmax, co
 // registers, and the Bluestein transforms their last DIF pass, the filter
 // and the first DIT pass in one LDS round trip: 7 instead of 10 for a
 // Bluestein transform, 5 instead of 6 for a power-of-two pair.
+// The register passes give each thread exactly one group of the first pass.
 template <int M>
-constexpr bool kRegEnds = M >= 1024 && (M >> pass_r<M>()) % FftShape<M>::TPB == 0;
+constexpr bool kRegEnds = M >= 1024 && (M >> pass_r<M>()) == FftShape<M>::TPB;
 
 // DIF passes after the first.
 template <int M>
