@@ -463,9 +463,10 @@ __device__ inline const double2* after_passes(const double2* p) {
   return p;
 }
 
-// The thread-strided loops below step by the compile-time workgroup size and
-// are unrolled, with their global reads made unconditional (indices clamped
-// inside the ring), so all of a thread's loads are in flight together.
+// The forward loops below step by the compile-time workgroup size and are
+// unrolled, with their global reads made unconditional (indices clamped
+// inside the ring), so all of a thread's loads are in flight together.  The
+// inverse loops keep one element per iteration: unrolled, they ran slower.
 
 // One real ring of length n = 2L (L not a power of two) transformed alone:
 // y_j = x_{2j} + i x_{2j+1}, j < L, goes through a length-L Bluestein
